@@ -1,0 +1,141 @@
+"""Low-rank covariance parameterisation (cov3d/covar.py:9-458).
+
+``Covar`` stores normalised eigenvector directions and log sqrt-eigenvalues;
+its forward returns V = vectors * exp(log_sqrt_eigenvals) in the spatial
+domain, or fft3c(pad_{us*L}(V / grid_correction)) in the Fourier domain, as in
+covar.py:440-449.
+"""
+
+from __future__ import annotations
+
+from typing import Any, Dict, List, Optional, Tuple, Union
+
+import torch
+
+from .projection_funcs import centered_fft3, centered_ifft3
+
+
+def sinc_grid_correction(L: int, upsampling_factor: int, nufft_disc: str) -> torch.Tensor:
+    """Separable sinc (nearest) or sinc^2 (bilinear) correction (covar.py:45-65, mean.py:27-40)."""
+    assert nufft_disc in ("bilinear", "nearest"), "nufft_disc must be 'bilinear' or 'nearest'"
+    pos = torch.arange(-(L // 2), (L - 1) // 2 + 1, dtype=torch.float64) / L
+    pos = torch.pi * pos / upsampling_factor
+    s = torch.ones_like(pos)
+    nz = pos != 0
+    s[nz] = torch.sin(pos[nz]) / pos[nz]
+    s[s < 1e-6] = 1
+    if nufft_disc == "bilinear":
+        s = s**2
+    s = s.to(torch.float32)
+    return s[:, None, None] * s[None, :, None] * s[None, None, :]
+
+
+class VolumeBase(torch.nn.Module):
+    def __init__(self, resolution: int, dtype=torch.float32, fourier_domain: bool = False, upsampling_factor: int = 2):
+        super().__init__()
+        self.resolution = resolution
+        self.dtype = dtype
+        self._in_spatial_domain = not fourier_domain
+        self.upsampling_factor = upsampling_factor
+        self.grid_correction = None
+
+    def init_grid_correction(self, nufft_disc: str) -> None:
+        if nufft_disc not in ("bilinear", "nearest"):
+            self.grid_correction = None
+            return
+        self.grid_correction = sinc_grid_correction(self.resolution, self.upsampling_factor, nufft_disc).to(
+            self.device
+        )
+
+    def to(self, *args, **kwargs):
+        super().to(*args, **kwargs)
+        if self.grid_correction is not None:
+            self.grid_correction = self.grid_correction.to(*args, **kwargs)
+        return self
+
+    def state_dict(self, *args, **kwargs):
+        sd = super().state_dict(*args, **kwargs)
+        sd.update(
+            {
+                "_in_spatial_domain": self._in_spatial_domain,
+                "grid_correction": self.grid_correction.to("cpu") if self.grid_correction is not None else None,
+            }
+        )
+        return sd
+
+    def load_state_dict(self, state_dict, *args, **kwargs):
+        state_dict = dict(state_dict)
+        self._in_spatial_domain = state_dict.pop("_in_spatial_domain")
+        self.grid_correction = state_dict.pop("grid_correction")
+        return super().load_state_dict(state_dict, *args, **kwargs)
+
+
+class Covar(VolumeBase):
+    """Rank-r covariance Sigma = V^T V with V = vectors * exp(log_sqrt_eigenvals) (covar.py:278-458)."""
+
+    def __init__(
+        self,
+        resolution: int,
+        rank: int,
+        dtype: torch.dtype = torch.float32,
+        pixel_var_estimate: Union[float, torch.Tensor] = 1,
+        fourier_domain: bool = False,
+        upsampling_factor: int = 2,
+        vectors: Optional[torch.Tensor] = None,
+    ) -> None:
+        super().__init__(resolution, dtype, fourier_domain, upsampling_factor)
+        self.rank = rank
+        self.pixel_var_estimate = pixel_var_estimate
+        if vectors is None:
+            vectors = torch.randn((rank,) + (resolution,) * 3, dtype=dtype) * (float(pixel_var_estimate) ** 0.5)
+        else:
+            vectors = vectors.clone()
+        v, l = self._split(vectors)
+        self.vectors = torch.nn.Parameter(v)
+        self.log_sqrt_eigenvals = torch.nn.Parameter(l)
+
+    @staticmethod
+    def _split(vectors: torch.Tensor) -> Tuple[torch.Tensor, torch.Tensor]:
+        n = vectors.reshape(vectors.shape[0], -1).norm(dim=1)
+        return vectors / n.reshape(-1, 1, 1, 1), torch.log(n)
+
+    @property
+    def device(self) -> torch.device:
+        return self.vectors.device
+
+    def set_vectors(self, new_vectors: torch.Tensor) -> None:
+        v, l = self._split(new_vectors)
+        self.vectors.data.copy_(v)
+        self.log_sqrt_eigenvals.data.copy_(l)
+
+    def get_vectors_spatial_domain(self) -> torch.Tensor:
+        return self.vectors * torch.exp(self.log_sqrt_eigenvals).reshape(-1, 1, 1, 1)
+
+    def get_vectors_fourier_domain(self) -> torch.Tensor:
+        v = self.get_vectors_spatial_domain()
+        if self.grid_correction is not None:
+            v = v / self.grid_correction
+        return centered_fft3(v, padding_size=(self.resolution * self.upsampling_factor,) * 3)
+
+    def get_vectors(self) -> torch.Tensor:
+        return self.get_vectors_spatial_domain() if self._in_spatial_domain else self.get_vectors_fourier_domain()
+
+    def forward(self, dummy_var: Optional[Any] = None) -> torch.Tensor:
+        return self.get_vectors()
+
+    @property
+    def eigenvecs(self) -> Tuple[torch.Tensor, torch.Tensor]:
+        """Orthonormal eigenvolumes and eigenvalues from an SVD of V (covar.py:407-430)."""
+        with torch.no_grad():
+            v = self.get_vectors_spatial_domain().reshape(self.rank, -1)
+            _, s, vh = torch.linalg.svd(v, full_matrices=False)
+            return vh.reshape((self.rank,) + (self.resolution,) * 3), s**2
+
+    def grad_lr_factor(self) -> List[Dict[str, Union[torch.nn.Parameter, float]]]:
+        return [{"params": self.vectors, "lr": 1}, {"params": self.log_sqrt_eigenvals, "lr": 100}]
+
+    def orthogonal_projection(self) -> None:
+        with torch.no_grad():
+            v = self.get_vectors_spatial_domain().reshape(self.rank, -1)
+            _, s, vh = torch.linalg.svd(v, full_matrices=False)
+            self.set_vectors((s.reshape(-1, 1) * vh).reshape((self.rank,) + (self.resolution,) * 3))

@@ -1,0 +1,53 @@
+// Shared helpers of the cov3d HIP library: error reporting, complex types,
+// launch helpers.  All device code targets gfx950 (CDNA4, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+#include <string>
+
+#include "../../include/cov3d_hip.h"
+
+namespace cov3d {
+
+// Thread-local last-error message (returned by cov3d_last_error).
+void set_error(const char* fmt, ...);
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+template <typename T>
+struct Cplx {
+  T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ Cplx<T> cadd(Cplx<T> a, Cplx<T> b) {
+  return {a.x + b.x, a.y + b.y};
+}
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+}  // namespace cov3d
+
+#define COV3D_CHECK_ARG(cond, ...)       \
+  do {                                   \
+    if (!(cond)) {                       \
+      ::cov3d::set_error(__VA_ARGS__);   \
+      return COV3D_ERR_ARG;              \
+    }                                    \
+  } while (0)
+
+#define COV3D_HIP(call)                                                     \
+  do {                                                                      \
+    hipError_t e_ = (call);                                                 \
+    if (e_ != hipSuccess) {                                                 \
+      ::cov3d::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                         __FILE__, __LINE__);                               \
+      return COV3D_ERR_HIP;                                                 \
+    }                                                                       \
+  } while (0)
+
+// Checks the launch status of the kernel just queued.
+#define COV3D_LAUNCH_CHECK() COV3D_HIP(hipGetLastError())

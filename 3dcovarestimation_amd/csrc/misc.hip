@@ -1,0 +1,456 @@
+// Data-feed helpers (rotated slice frequencies, CTF), FP64 complex dot
+// products and the Toeplitz normal operator of the mean-volume CG, plus the
+// small MLP of the learned CG preconditioner.
+#include <math.h>
+#include <rocfft/rocfft.h>
+
+#include <map>
+#include <mutex>
+#include <utility>
+
+#include "common.h"
+
+namespace cov3d {
+
+constexpr int kMiscBlock = 256;
+
+// ----------------------------------------------------------------------------
+// pts_rot: p = wrap(flip_zyx(R . pi*[gx, gy, 0])), grid_2d(L, "yx") values
+// g_k = ceil(k - L/2) / (L/2) (x along columns, y along rows).
+// Follows PoseModule._init_grid/forward (cov3d/poses.py:197-269).
+// ----------------------------------------------------------------------------
+__global__ void k_pts_rot(const float* __restrict__ rot, float* __restrict__ pts, int64_t B,
+                          int L) {
+  const int64_t d = (int64_t)L * L;
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= B * d) return;
+  const int64_t i = idx / d;
+  const int64_t k = idx - i * d;
+  const int row = (int)(k / L), col = (int)(k % L);
+  const float half = 0.5f * (float)L;
+  const float gx = ceilf((float)col - half) / half;
+  const float gy = ceilf((float)row - half) / half;
+  const float px = (float)M_PI * gx, py = (float)M_PI * gy;
+  const float* R = rot + 9 * i;
+  float v[3];
+  for (int a = 0; a < 3; ++a) v[a] = R[3 * a] * px + R[3 * a + 1] * py;
+  const float twopi = 2.f * (float)M_PI;
+  for (int a = 0; a < 3; ++a) {
+    float p = v[2 - a] + (float)M_PI;
+    p = p - twopi * floorf(p / twopi);  // torch.remainder semantics
+    pts[(i * 3 + a) * d + k] = p - (float)M_PI;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// CTF (the formula of aspire's CTFFilter._evaluate on evaluate_grid's grid):
+//   omega = pi*[gx, gy]; f = omega / (2 pi pixel); angle = atan2(fy, fx) - ang;
+//   defocus = mean + diff*cos(2 angle) (0 at f = 0);
+//   gamma = -pi*lambda*defocus*|f|^2 + 0.5*pi*Cs*1e7*lambda^3*|f|^4;
+//   h = sqrt(1 - a^2) sin(gamma) - a cos(gamma).
+// ----------------------------------------------------------------------------
+__global__ void k_ctf(const float* __restrict__ prm, float* __restrict__ out, int64_t B, int L) {
+  const int64_t d = (int64_t)L * L;
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= B * d) return;
+  const int64_t i = idx / d;
+  const int64_t k = idx - i * d;
+  const int row = (int)(k / L), col = (int)(k % L);
+  const float* p = prm + 7 * i;
+  const double du = p[0], dv = p[1], ang = p[2], kv = p[3], cs = p[4], alpha = p[5], px = p[6];
+  const double half = 0.5 * L;
+  const double gx = ceil((double)col - half) / half, gy = ceil((double)row - half) / half;
+  const double fx = M_PI * gx / (2 * M_PI * px), fy = M_PI * gy / (2 * M_PI * px);
+  const double lam = 12.2643247 / sqrt(kv * 1e3 + 0.978466 * kv * kv);
+  const double eps = 2.220446049250313e-16;
+  double defocus = 0;
+  if (fabs(fx) > eps || fabs(fy) > eps) {
+    double a = atan2(fy, fx) - ang;
+    defocus = 0.5 * (du + dv) + 0.5 * (du - dv) * cos(2 * a);
+  }
+  const double r2 = fx * fx + fy * fy;
+  const double gamma = -M_PI * lam * defocus * r2 + 0.5 * M_PI * (cs * 1e7) * lam * lam * lam * r2 * r2;
+  out[idx] = (float)(sqrt(1 - alpha * alpha) * sin(gamma) - alpha * cos(gamma));
+}
+
+// ----------------------------------------------------------------------------
+// FP64-accumulated complex dot product sum conj(x) y.
+// ----------------------------------------------------------------------------
+__global__ void k_zero2(double* out) {
+  out[0] = 0;
+  out[1] = 0;
+}
+
+__global__ void __launch_bounds__(kMiscBlock)
+    k_vdot(const float2* __restrict__ x, const float2* __restrict__ y, int64_t n,
+           double* __restrict__ out) {
+  __shared__ double sr[kMiscBlock], si[kMiscBlock];
+  double ar = 0, ai = 0;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    float2 a = x[k], b = y[k];
+    ar += (double)a.x * b.x + (double)a.y * b.y;
+    ai += (double)a.x * b.y - (double)a.y * b.x;
+  }
+  sr[threadIdx.x] = ar;
+  si[threadIdx.x] = ai;
+  __syncthreads();
+  for (int s = kMiscBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      sr[threadIdx.x] += sr[threadIdx.x + s];
+      si[threadIdx.x] += si[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    atomicAdd(out, sr[0]);
+    atomicAdd(out + 1, si[0]);
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Toeplitz operator y = crop_L(ifft3c(lhs * fft3c(pad_nf(x)))) with rocFFT.
+// Index maps (per axis): pad  u[(i - L/2) mod nf] = x[i];
+//                        mult U[k] *= lhs[(k + nf/2) mod nf];
+//                        crop y[i] = V[(i - L/2) mod nf] / nf^3.
+// ----------------------------------------------------------------------------
+__global__ void k_tpl_pad(const float2* __restrict__ x, float2* __restrict__ u, int L, int nf) {
+  const int64_t n3 = (int64_t)nf * nf * nf;
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n3) return;
+  int j2 = (int)(idx % nf), j1 = (int)((idx / nf) % nf), j0 = (int)(idx / ((int64_t)nf * nf));
+  auto src = [&](int j) -> int {
+    int s = j < nf - L / 2 ? j : j - nf;  // signed offset
+    int i = s + L / 2;
+    return (i >= 0 && i < L) ? i : -1;
+  };
+  int i0 = src(j0), i1 = src(j1), i2 = src(j2);
+  float2 v = make_float2(0.f, 0.f);
+  if (i0 >= 0 && i1 >= 0 && i2 >= 0) v = x[((int64_t)i0 * L + i1) * L + i2];
+  u[idx] = v;
+}
+
+__global__ void k_tpl_mult(float2* __restrict__ u, const float2* __restrict__ lhs, int nf) {
+  const int64_t n3 = (int64_t)nf * nf * nf;
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n3) return;
+  int k2 = (int)(idx % nf), k1 = (int)((idx / nf) % nf), k0 = (int)(idx / ((int64_t)nf * nf));
+  int c0 = (k0 + nf / 2) % nf, c1 = (k1 + nf / 2) % nf, c2 = (k2 + nf / 2) % nf;
+  float2 a = u[idx], b = lhs[((int64_t)c0 * nf + c1) * nf + c2];
+  u[idx] = make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__global__ void k_tpl_crop(const float2* __restrict__ v, float2* __restrict__ y, int L, int nf,
+                           float scale) {
+  const int64_t n3 = (int64_t)L * L * L;
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n3) return;
+  int i2 = (int)(idx % L), i1 = (int)((idx / L) % L), i0 = (int)(idx / ((int64_t)L * L));
+  auto dst = [&](int i) -> int {
+    int s = i - L / 2;
+    return s >= 0 ? s : s + nf;
+  };
+  float2 a = v[((int64_t)dst(i0) * nf + dst(i1)) * nf + dst(i2)];
+  y[idx] = make_float2(a.x * scale, a.y * scale);
+}
+
+struct TplCtx {
+  rocfft_plan fwd = nullptr, bwd = nullptr;
+  rocfft_execution_info info = nullptr;
+  void* work = nullptr;
+  void* buf = nullptr;
+};
+
+static std::mutex g_tpl_mu;
+static std::map<std::pair<int64_t, int64_t>, TplCtx*> g_tpl;
+static std::once_flag g_rocfft_once_misc;
+
+static int tpl_get(int64_t L, int64_t nf, TplCtx** out) {
+  std::lock_guard<std::mutex> lk(g_tpl_mu);
+  auto it = g_tpl.find({L, nf});
+  if (it != g_tpl.end()) {
+    *out = it->second;
+    return 0;
+  }
+  std::call_once(g_rocfft_once_misc, []() { rocfft_setup(); });
+  TplCtx* c = new TplCtx();
+  size_t len[3] = {(size_t)nf, (size_t)nf, (size_t)nf};
+  if (rocfft_plan_create(&c->fwd, rocfft_placement_inplace, rocfft_transform_type_complex_forward,
+                         rocfft_precision_single, 3, len, 1, nullptr) != rocfft_status_success ||
+      rocfft_plan_create(&c->bwd, rocfft_placement_inplace, rocfft_transform_type_complex_inverse,
+                         rocfft_precision_single, 3, len, 1, nullptr) != rocfft_status_success ||
+      rocfft_execution_info_create(&c->info) != rocfft_status_success) {
+    set_error("rocfft plan creation failed for Toeplitz operator nf=%lld", (long long)nf);
+    delete c;
+    return COV3D_ERR_FFT;
+  }
+  size_t w1 = 0, w2 = 0;
+  rocfft_plan_get_work_buffer_size(c->fwd, &w1);
+  rocfft_plan_get_work_buffer_size(c->bwd, &w2);
+  size_t ws = w1 > w2 ? w1 : w2;
+  if (ws) {
+    if (hipMalloc(&c->work, ws) != hipSuccess) {
+      set_error("hipMalloc of rocFFT work buffer failed");
+      return COV3D_ERR_HIP;
+    }
+    rocfft_execution_info_set_work_buffer(c->info, c->work, ws);
+  }
+  if (hipMalloc(&c->buf, (size_t)nf * nf * nf * 8) != hipSuccess) {
+    set_error("hipMalloc of Toeplitz buffer failed");
+    return COV3D_ERR_HIP;
+  }
+  g_tpl[{L, nf}] = c;
+  *out = c;
+  return 0;
+}
+
+// ----------------------------------------------------------------------------
+// MLP (n_in -> 64 -> 64 -> 64 -> 1, GELU tanh form), one thread per row.
+// ----------------------------------------------------------------------------
+constexpr int kHid = 64;
+constexpr int kMaxIn = 8;
+
+__device__ __forceinline__ float gelu(float a) {
+  const float c = 0.7978845608028654f;  // sqrt(2/pi)
+  float u = c * (a + 0.044715f * a * a * a);
+  return 0.5f * a * (1.f + tanhf(u));
+}
+__device__ __forceinline__ float gelu_grad(float a) {
+  const float c = 0.7978845608028654f;
+  float u = c * (a + 0.044715f * a * a * a);
+  float t = tanhf(u);
+  float du = c * (1.f + 3.f * 0.044715f * a * a);
+  return 0.5f * (1.f + t) + 0.5f * a * (1.f - t * t) * du;
+}
+
+__host__ __device__ inline int64_t mlp_nparams(int n_in) {
+  return (int64_t)kHid * n_in + kHid + 2 * (kHid * kHid + kHid) + kHid + 1;
+}
+
+struct MlpOffsets {
+  int w1, b1, w2, b2, w3, b3, w4, b4;
+};
+__host__ __device__ inline MlpOffsets mlp_offsets(int n_in) {
+  MlpOffsets o;
+  o.w1 = 0;
+  o.b1 = o.w1 + kHid * n_in;
+  o.w2 = o.b1 + kHid;
+  o.b2 = o.w2 + kHid * kHid;
+  o.w3 = o.b2 + kHid;
+  o.b3 = o.w3 + kHid * kHid;
+  o.w4 = o.b3 + kHid;
+  o.b4 = o.w4 + kHid;
+  return o;
+}
+
+// Forward of one row; a1..a3 receive the pre-activations when non-null.
+__device__ float mlp_row(const float* P, int n_in, const float* x, float* a1, float* a2,
+                         float* a3) {
+  MlpOffsets o = mlp_offsets(n_in);
+  float h1[kHid], h2[kHid];
+  for (int j = 0; j < kHid; ++j) {
+    float s = P[o.b1 + j];
+    for (int q = 0; q < n_in; ++q) s += P[o.w1 + j * n_in + q] * x[q];
+    if (a1) a1[j] = s;
+    h1[j] = gelu(s);
+  }
+  for (int j = 0; j < kHid; ++j) {
+    float s = P[o.b2 + j];
+    for (int q = 0; q < kHid; ++q) s += P[o.w2 + j * kHid + q] * h1[q];
+    if (a2) a2[j] = s;
+    h2[j] = gelu(s);
+  }
+  float out = P[o.b4];
+  for (int j = 0; j < kHid; ++j) {
+    float s = P[o.b3 + j];
+    for (int q = 0; q < kHid; ++q) s += P[o.w3 + j * kHid + q] * h2[q];
+    if (a3) a3[j] = s;
+    out += P[o.w4 + j] * gelu(s);
+  }
+  return out;
+}
+
+__global__ void __launch_bounds__(kMiscBlock)
+    k_mlp_forward(const float* __restrict__ params, const float* __restrict__ x,
+                  float* __restrict__ y, int64_t n, int n_in) {
+  extern __shared__ float sP[];
+  const int64_t np = mlp_nparams(n_in);
+  for (int64_t e = threadIdx.x; e < np; e += blockDim.x) sP[e] = params[e];
+  __syncthreads();
+  int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  float xi[kMaxIn];
+  for (int q = 0; q < n_in; ++q) xi[q] = x[row * n_in + q];
+  y[row] = mlp_row(sP, n_in, xi, nullptr, nullptr, nullptr);
+}
+
+// Per-row backward; parameter gradients are summed in LDS then added to
+// grad_params with one atomic per parameter per block.
+__global__ void __launch_bounds__(kMiscBlock)
+    k_mlp_backward(const float* __restrict__ params, const float* __restrict__ x,
+                   const float* __restrict__ gy, float* __restrict__ gparams, int64_t n,
+                   int n_in) {
+  extern __shared__ float smem[];
+  const int64_t np = mlp_nparams(n_in);
+  float* sP = smem;
+  float* sG = smem + np;
+  for (int64_t e = threadIdx.x; e < np; e += blockDim.x) {
+    sP[e] = params[e];
+    sG[e] = 0.f;
+  }
+  __syncthreads();
+  int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row < n) {
+    MlpOffsets o = mlp_offsets(n_in);
+    float xi[kMaxIn];
+    for (int q = 0; q < n_in; ++q) xi[q] = x[row * n_in + q];
+    float a1[kHid], a2[kHid], a3[kHid];
+    (void)mlp_row(sP, n_in, xi, a1, a2, a3);
+    const float g = gy[row];
+    atomicAdd(&sG[o.b4], g);
+    float d3[kHid];
+    for (int j = 0; j < kHid; ++j) {
+      atomicAdd(&sG[o.w4 + j], g * gelu(a3[j]));
+      d3[j] = g * sP[o.w4 + j] * gelu_grad(a3[j]);
+    }
+    // layer 3: s3 = W3 h2 + b3
+    float d2[kHid];
+    for (int q = 0; q < kHid; ++q) d2[q] = 0.f;
+    for (int j = 0; j < kHid; ++j) {
+      atomicAdd(&sG[o.b3 + j], d3[j]);
+      for (int q = 0; q < kHid; ++q) {
+        atomicAdd(&sG[o.w3 + j * kHid + q], d3[j] * gelu(a2[q]));
+        d2[q] += sP[o.w3 + j * kHid + q] * d3[j];
+      }
+    }
+    for (int q = 0; q < kHid; ++q) d2[q] *= gelu_grad(a2[q]);
+    float d1[kHid];
+    for (int q = 0; q < kHid; ++q) d1[q] = 0.f;
+    for (int j = 0; j < kHid; ++j) {
+      atomicAdd(&sG[o.b2 + j], d2[j]);
+      for (int q = 0; q < kHid; ++q) {
+        atomicAdd(&sG[o.w2 + j * kHid + q], d2[j] * gelu(a1[q]));
+        d1[q] += sP[o.w2 + j * kHid + q] * d2[j];
+      }
+    }
+    for (int j = 0; j < kHid; ++j) {
+      float dj = d1[j] * gelu_grad(a1[j]);
+      atomicAdd(&sG[o.b1 + j], dj);
+      for (int q = 0; q < n_in; ++q) atomicAdd(&sG[o.w1 + j * n_in + q], dj * xi[q]);
+    }
+  }
+  __syncthreads();
+  for (int64_t e = threadIdx.x; e < np; e += blockDim.x)
+    if (sG[e] != 0.f) atomicAdd(&gparams[e], sG[e]);
+}
+
+}  // namespace cov3d
+
+extern "C" int cov3d_pts_rot(const void* rot, void* pts, int64_t B, int64_t L, void* stream) {
+  COV3D_CHECK_ARG(B >= 0 && L >= 1, "bad shape");
+  if (B == 0) return 0;
+  COV3D_CHECK_ARG(rot && pts, "NULL pointer");
+  int64_t n = B * L * L;
+  hipLaunchKernelGGL(cov3d::k_pts_rot, dim3((unsigned)cov3d::ceil_div(n, cov3d::kMiscBlock)),
+                     dim3(cov3d::kMiscBlock), 0, cov3d::as_stream(stream), (const float*)rot,
+                     (float*)pts, B, (int)L);
+  COV3D_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int cov3d_ctf_eval(const void* params, void* out, int64_t B, int64_t L, void* stream) {
+  COV3D_CHECK_ARG(B >= 0 && L >= 1, "bad shape");
+  if (B == 0) return 0;
+  COV3D_CHECK_ARG(params && out, "NULL pointer");
+  int64_t n = B * L * L;
+  hipLaunchKernelGGL(cov3d::k_ctf, dim3((unsigned)cov3d::ceil_div(n, cov3d::kMiscBlock)),
+                     dim3(cov3d::kMiscBlock), 0, cov3d::as_stream(stream), (const float*)params,
+                     (float*)out, B, (int)L);
+  COV3D_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int cov3d_vdot_c64_f64(const void* x, const void* y, int64_t n, void* out,
+                                  void* stream) {
+  COV3D_CHECK_ARG(n >= 0, "bad length");
+  COV3D_CHECK_ARG(out, "NULL output");
+  hipStream_t st = cov3d::as_stream(stream);
+  hipLaunchKernelGGL(cov3d::k_zero2, dim3(1), dim3(1), 0, st, (double*)out);
+  COV3D_LAUNCH_CHECK();
+  if (n == 0) return 0;
+  COV3D_CHECK_ARG(x && y, "NULL input");
+  int64_t nb = cov3d::ceil_div(n, cov3d::kMiscBlock);
+  if (nb > 2048) nb = 2048;
+  hipLaunchKernelGGL(cov3d::k_vdot, dim3((unsigned)nb), dim3(cov3d::kMiscBlock), 0, st,
+                     (const float2*)x, (const float2*)y, n, (double*)out);
+  COV3D_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int cov3d_toeplitz_apply(const void* x, const void* lhs, void* y, int64_t L, int64_t nf,
+                                    void* stream) {
+  COV3D_CHECK_ARG(L >= 1 && nf >= L, "need nf >= L >= 1");
+  COV3D_CHECK_ARG(x && lhs && y, "NULL pointer");
+  cov3d::TplCtx* c = nullptr;
+  int rc = cov3d::tpl_get(L, nf, &c);
+  if (rc) return rc;
+  hipStream_t st = cov3d::as_stream(stream);
+  const int64_t n3 = nf * nf * nf, l3 = L * L * L;
+  const int blk = cov3d::kMiscBlock;
+  hipLaunchKernelGGL(cov3d::k_tpl_pad, dim3((unsigned)cov3d::ceil_div(n3, blk)), dim3(blk), 0, st,
+                     (const float2*)x, (float2*)c->buf, (int)L, (int)nf);
+  COV3D_LAUNCH_CHECK();
+  void* bufs[1] = {c->buf};
+  if (rocfft_execution_info_set_stream(c->info, st) != rocfft_status_success ||
+      rocfft_execute(c->fwd, bufs, nullptr, c->info) != rocfft_status_success) {
+    cov3d::set_error("rocfft forward failed in Toeplitz operator");
+    return COV3D_ERR_FFT;
+  }
+  hipLaunchKernelGGL(cov3d::k_tpl_mult, dim3((unsigned)cov3d::ceil_div(n3, blk)), dim3(blk), 0, st,
+                     (float2*)c->buf, (const float2*)lhs, (int)nf);
+  COV3D_LAUNCH_CHECK();
+  if (rocfft_execute(c->bwd, bufs, nullptr, c->info) != rocfft_status_success) {
+    cov3d::set_error("rocfft inverse failed in Toeplitz operator");
+    return COV3D_ERR_FFT;
+  }
+  hipLaunchKernelGGL(cov3d::k_tpl_crop, dim3((unsigned)cov3d::ceil_div(l3, blk)), dim3(blk), 0, st,
+                     (const float2*)c->buf, (float2*)y, (int)L, (int)nf,
+                     (float)(1.0 / (double)n3));
+  COV3D_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int cov3d_mlp_forward(const void* params, const void* x, void* y, int64_t n,
+                                 int64_t n_in, void* stream) {
+  COV3D_CHECK_ARG(n_in >= 1 && n_in <= cov3d::kMaxIn, "n_in must be in [1, 8]");
+  COV3D_CHECK_ARG(n >= 0, "bad row count");
+  if (n == 0) return 0;
+  COV3D_CHECK_ARG(params && x && y, "NULL pointer");
+  size_t smem = (size_t)cov3d::mlp_nparams((int)n_in) * sizeof(float);
+  hipLaunchKernelGGL(cov3d::k_mlp_forward, dim3((unsigned)cov3d::ceil_div(n, cov3d::kMiscBlock)),
+                     dim3(cov3d::kMiscBlock), smem, cov3d::as_stream(stream),
+                     (const float*)params, (const float*)x, (float*)y, n, (int)n_in);
+  COV3D_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int cov3d_mlp_backward(const void* params, const void* x, const void* gy,
+                                  void* grad_params, int64_t n, int64_t n_in, void* stream) {
+  COV3D_CHECK_ARG(n_in >= 1 && n_in <= cov3d::kMaxIn, "n_in must be in [1, 8]");
+  COV3D_CHECK_ARG(n >= 0, "bad row count");
+  if (n == 0) return 0;
+  COV3D_CHECK_ARG(params && x && gy && grad_params, "NULL pointer");
+  size_t smem = (size_t)2 * cov3d::mlp_nparams((int)n_in) * sizeof(float);
+  static bool attr = false;
+  if (!attr) {
+    COV3D_HIP(hipFuncSetAttribute((const void*)cov3d::k_mlp_backward,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    attr = true;
+  }
+  hipLaunchKernelGGL(cov3d::k_mlp_backward, dim3((unsigned)cov3d::ceil_div(n, cov3d::kMiscBlock)),
+                     dim3(cov3d::kMiscBlock), smem, cov3d::as_stream(stream),
+                     (const float*)params, (const float*)x, (const float*)gy, (float*)grad_params,
+                     n, (int)n_in);
+  COV3D_LAUNCH_CHECK();
+  return 0;
+}

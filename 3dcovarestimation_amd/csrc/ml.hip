@@ -1,0 +1,350 @@
+// Maximum-likelihood contractions of rank-r projected eigenvectors.
+//
+// Replaces compute_latentMAP_batch (cov3d/wiener_coords.py:168-203) and the
+// Gram / solve / logdet / autograd work of cost_maximum_liklihood
+// (cov3d/covar_sgd.py:1238-1286).
+//
+// One workgroup per image.  The rows [Q_0 .. Q_{r-1}, y~] (complex, d pixels,
+// treated as 2d real components) are read once from HBM; the (r+1)x(r+1)
+// real Gram is accumulated either with FP32 FMAs (precision 0) or with
+// v_mfma_f32_32x32x16_bf16 on split-BF16 operands v = hi + lo, summing
+// hi*hi + hi*lo + lo*hi in FP32 (precision 1).  The r x r system is then
+// factorised in FP64 (Cholesky) inside the same workgroup.
+#include <math.h>
+
+#include "common.h"
+
+namespace cov3d {
+
+constexpr int kMlBlock = 256;
+constexpr int kRmax = 32;  // rows of the Gram tile (r + 1 <= 32)
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ unsigned short f32_to_bf16_bits(float f) {
+  unsigned u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf16_bits_to_f32(unsigned short h) {
+  return __uint_as_float(((unsigned)h) << 16);
+}
+
+// Scale of pixel k of image i: mask * ctf / L^2 (mask zeroes row/col 0 for even L).
+__device__ __forceinline__ float pixel_scale(const float* __restrict__ ctf, int64_t i, int64_t k,
+                                             int L, float inv_l2) {
+  if ((L & 1) == 0) {
+    int ky = (int)(k / L), kx = (int)(k - (int64_t)ky * L);
+    if (ky == 0 || kx == 0) return 0.f;
+  }
+  float s = inv_l2;
+  if (ctf) s *= ctf[i * (int64_t)L * L + k];
+  return s;
+}
+
+// Row value (complex) of pixel k for Gram row m of image i.
+__device__ __forceinline__ float2 gram_row_value(const float2* __restrict__ c,
+                                                 const float2* __restrict__ yhat,
+                                                 const float* __restrict__ ctf, int64_t i, int m,
+                                                 int r, int64_t B, int64_t d, int64_t k, int L,
+                                                 float inv_l, float inv_l2) {
+  if (k >= d || m > r) return make_float2(0.f, 0.f);
+  if (m == r) {
+    float2 v = yhat[i * d + k];
+    return make_float2(v.x * inv_l, v.y * inv_l);
+  }
+  float s = pixel_scale(ctf, i, k, L, inv_l2);
+  float2 v = c[((int64_t)m * B + i) * d + k];
+  return make_float2(v.x * s, v.y * s);
+}
+
+// FP64 Cholesky of the r x r matrix in A (leading dimension kRmax+1), in place
+// (lower triangle).  All threads of the block take part.
+__device__ void block_cholesky(double* A, int r) {
+  const int tid = threadIdx.x;
+  for (int k = 0; k < r; ++k) {
+    if (tid == 0) A[k * (kRmax + 1) + k] = sqrt(A[k * (kRmax + 1) + k]);
+    __syncthreads();
+    const double dkk = A[k * (kRmax + 1) + k];
+    for (int i = k + 1 + tid; i < r; i += blockDim.x) A[i * (kRmax + 1) + k] /= dkk;
+    __syncthreads();
+    const int n = r - k - 1;
+    for (int q = tid; q < n * n; q += blockDim.x) {
+      int i = k + 1 + q / n, j = k + 1 + q % n;
+      if (j <= i) A[i * (kRmax + 1) + j] -= A[i * (kRmax + 1) + k] * A[j * (kRmax + 1) + k];
+    }
+    __syncthreads();
+  }
+}
+
+// Solves (L L^T) x = b in place for one right-hand side (serial, r <= 31).
+__device__ void chol_solve(const double* A, double* x, int r) {
+  for (int i = 0; i < r; ++i) {
+    double s = x[i];
+    for (int k = 0; k < i; ++k) s -= A[i * (kRmax + 1) + k] * x[k];
+    x[i] = s / A[i * (kRmax + 1) + i];
+  }
+  for (int i = r - 1; i >= 0; --i) {
+    double s = x[i];
+    for (int k = i + 1; k < r; ++k) s -= A[k * (kRmax + 1) + i] * x[k];
+    x[i] = s / A[i * (kRmax + 1) + i];
+  }
+}
+
+template <int PREC>
+__global__ void __launch_bounds__(kMlBlock)
+    k_ml_forward(const float2* __restrict__ c, const float* __restrict__ ctf,
+                 const float2* __restrict__ yhat, const float* __restrict__ lam_inv, int64_t B,
+                 int r, int L, float noise_var, int mean_const, float* __restrict__ cost,
+                 float* __restrict__ z_out, float* __restrict__ m_out,
+                 float* __restrict__ minv_out, float* __restrict__ b_out) {
+  __shared__ float s_part[4][kRmax * kRmax];
+  __shared__ double s_G[kRmax * (kRmax + 1)];
+  __shared__ double s_A[kRmax * (kRmax + 1)];
+  __shared__ double s_z[kRmax];
+  __shared__ double s_minv[kRmax * (kRmax + 1)];
+  const int64_t i = blockIdx.x;
+  const int64_t d = (int64_t)L * L;
+  const float inv_l = 1.f / (float)L;
+  const float inv_l2 = inv_l * inv_l;
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6;
+  const int lane = tid & 63;
+
+  if (PREC == 1) {
+    // ---- split-BF16 MFMA Gram: wave w covers 32-pixel chunks w, w+4, ...
+    const int m = lane & 31;
+    const int h = lane >> 5;
+    f32x16 acc;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[v] = 0.f;
+    const int64_t nchunk = (d + 31) / 32;
+    for (int64_t ch = wave; ch < nchunk; ch += 4) {
+      const int64_t k0 = ch * 32 + 16 * h;
+      float2 vals[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q)
+        vals[q] = gram_row_value(c, yhat, ctf, i, m, r, B, d, k0 + q, L, inv_l, inv_l2);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        u16x8 hi, lo;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          float2 v = vals[4 * q + e];
+          unsigned short hx = f32_to_bf16_bits(v.x), hy = f32_to_bf16_bits(v.y);
+          hi[2 * e] = hx;
+          hi[2 * e + 1] = hy;
+          lo[2 * e] = f32_to_bf16_bits(v.x - bf16_bits_to_f32(hx));
+          lo[2 * e + 1] = f32_to_bf16_bits(v.y - bf16_bits_to_f32(hy));
+        }
+        bf16x8 ah = __builtin_bit_cast(bf16x8, hi);
+        bf16x8 al = __builtin_bit_cast(bf16x8, lo);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, ah, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, al, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, ah, acc, 0, 0, 0);
+      }
+    }
+    // D layout of the 32x32 tile: lane l, register v -> row 8*(v/4) + 4*(l/32) + v%4, col l%32.
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      int row = 8 * (v >> 2) + 4 * h + (v & 3);
+      s_part[wave][row * kRmax + m] = acc[v];
+    }
+    __syncthreads();
+  } else {
+    // ---- FP32 Gram: stage 32 pixels of all rows in LDS, one (t, s) pair per thread.
+    __shared__ float2 s_rows[kRmax][64];
+    const int np = (r + 1) * (r + 2) / 2;
+    int pt = -1, ps = -1;
+    {
+      int q = tid, t = 0;
+      while (t <= r && q >= (r + 1 - t)) {
+        q -= (r + 1 - t);
+        ++t;
+      }
+      if (tid < np) {
+        pt = t;
+        ps = t + q;
+      }
+    }
+    float accp = 0.f;
+    for (int64_t k0 = 0; k0 < d; k0 += 64) {
+      for (int e = tid; e < (r + 1) * 64; e += blockDim.x) {
+        int m = e / 64, kk = e % 64;
+        s_rows[m][kk] = gram_row_value(c, yhat, ctf, i, m, r, B, d, k0 + kk, L, inv_l, inv_l2);
+      }
+      __syncthreads();
+      if (pt >= 0) {
+        for (int kk = 0; kk < 64; ++kk) {
+          float2 a = s_rows[pt][kk], b = s_rows[ps][kk];
+          accp += a.x * b.x + a.y * b.y;
+        }
+      }
+      __syncthreads();
+    }
+    for (int e = tid; e < 4 * kRmax * kRmax; e += blockDim.x) (&s_part[0][0])[e] = 0.f;
+    __syncthreads();
+    if (pt >= 0) {
+      s_part[0][pt * kRmax + ps] = accp;
+      s_part[0][ps * kRmax + pt] = accp;
+    }
+    __syncthreads();
+  }
+
+  // ---- reduce partial Grams, form M, b
+  for (int e = tid; e < (r + 1) * (r + 1); e += blockDim.x) {
+    int a = e / (r + 1), b = e % (r + 1);
+    double g = 0;
+    for (int w = 0; w < 4; ++w) g += (double)s_part[w][a * kRmax + b];
+    s_G[a * (kRmax + 1) + b] = g;
+  }
+  __syncthreads();
+  const double inv_nv = 1.0 / (double)noise_var;
+  for (int e = tid; e < r * r; e += blockDim.x) {
+    int a = e / r, b = e % r;
+    double g = 0.5 * (s_G[a * (kRmax + 1) + b] + s_G[b * (kRmax + 1) + a]) * inv_nv;
+    g += lam_inv ? (double)lam_inv[a * r + b] : (a == b ? 1.0 : 0.0);
+    s_A[a * (kRmax + 1) + b] = g;
+    if (m_out) m_out[(i * r + a) * r + b] = (float)g;
+  }
+  __syncthreads();
+  block_cholesky(s_A, r);
+  if (tid == 0) {
+    double bvec[kRmax];
+    for (int a = 0; a < r; ++a) bvec[a] = s_G[a * (kRmax + 1) + r] * inv_nv;
+    double x[kRmax];
+    for (int a = 0; a < r; ++a) x[a] = bvec[a];
+    chol_solve(s_A, x, r);
+    double bz = 0, logdet = 0;
+    for (int a = 0; a < r; ++a) {
+      s_z[a] = x[a];
+      bz += bvec[a] * x[a];
+      logdet += 2.0 * log(s_A[a * (kRmax + 1) + a]);
+      if (b_out) b_out[i * r + a] = (float)bvec[a];
+      if (z_out) z_out[i * r + a] = (float)x[a];
+    }
+    double cst = 0.5 * (-bz + logdet);
+    if (mean_const) cst += 0.5 * s_G[r * (kRmax + 1) + r] * inv_nv;
+    if (cost) cost[i] = (float)cst;
+  }
+  if (minv_out && tid < r) {
+    double x[kRmax];
+    for (int a = 0; a < r; ++a) x[a] = (a == tid) ? 1.0 : 0.0;
+    chol_solve(s_A, x, r);
+    for (int a = 0; a < r; ++a) minv_out[(i * r + a) * r + tid] = (float)x[a];
+  }
+}
+
+// Gradient images: g_t = mask*ctf*(Minv Q + z (Q^T z - y~))_t * scale, one thread per pixel.
+// RM is a compile-time bound on r so the per-pixel rows stay in registers.
+template <int RM>
+__global__ void __launch_bounds__(kMlBlock)
+    k_ml_grad(const float2* __restrict__ c, const float* __restrict__ ctf,
+              const float2* __restrict__ yhat, const float* __restrict__ z,
+              const float* __restrict__ minv, float2* __restrict__ g, int64_t B, int r, int L,
+              float scale) {
+  __shared__ float s_minv[RM * RM];
+  __shared__ float s_z[RM];
+  const int64_t i = blockIdx.y;
+  const int64_t d = (int64_t)L * L;
+  for (int e = threadIdx.x; e < RM * RM; e += blockDim.x) {
+    int a = e / RM, b = e % RM;
+    s_minv[e] = (a < r && b < r) ? minv[(i * r + a) * r + b] : 0.f;
+  }
+  if (threadIdx.x < RM) s_z[threadIdx.x] = (int)threadIdx.x < r ? z[i * r + threadIdx.x] : 0.f;
+  __syncthreads();
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= d) return;
+  const float inv_l = 1.f / (float)L;
+  const float ps = pixel_scale(ctf, i, k, L, inv_l * inv_l);
+  float qx[RM], qy[RM];
+  float ex = 0.f, ey = 0.f;
+#pragma unroll
+  for (int s = 0; s < RM; ++s) {
+    float2 v = s < r ? c[((int64_t)s * B + i) * d + k] : make_float2(0.f, 0.f);
+    qx[s] = v.x * ps;
+    qy[s] = v.y * ps;
+    ex += s_z[s] * qx[s];
+    ey += s_z[s] * qy[s];
+  }
+  float2 yv = yhat[i * d + k];
+  ex -= yv.x * inv_l;
+  ey -= yv.y * inv_l;
+  const float gs = ps * scale * (float)(L * L);  // mask*ctf*scale (ps carries 1/L^2)
+  for (int t = 0; t < r; ++t) {
+    float ax = s_z[t] * ex, ay = s_z[t] * ey;
+#pragma unroll
+    for (int s = 0; s < RM; ++s) {
+      float mv = s_minv[t * RM + s];
+      ax += mv * qx[s];
+      ay += mv * qy[s];
+    }
+    g[((int64_t)t * B + i) * d + k] = make_float2(ax * gs, ay * gs);
+  }
+}
+
+}  // namespace cov3d
+
+extern "C" int cov3d_ml_forward(const void* c, const void* ctf, const void* yhat,
+                                const void* lam_inv, int64_t B, int64_t r, int64_t L,
+                                double noise_var, int mean_const, int precision, void* cost,
+                                void* z, void* m_out, void* minv, void* b_out, void* stream) {
+  COV3D_CHECK_ARG(r >= 1 && r <= 31, "rank r=%lld must be in [1, 31]", (long long)r);
+  COV3D_CHECK_ARG(B >= 0 && L >= 1, "bad batch/size");
+  COV3D_CHECK_ARG(noise_var > 0, "noise_var must be positive");
+  COV3D_CHECK_ARG(precision == 0 || precision == 1, "precision must be 0 or 1");
+  if (B == 0) return 0;
+  COV3D_CHECK_ARG(c && yhat, "c and yhat must not be NULL");
+  hipStream_t st = cov3d::as_stream(stream);
+  if (precision == 1)
+    hipLaunchKernelGGL((cov3d::k_ml_forward<1>), dim3((unsigned)B), dim3(cov3d::kMlBlock), 0, st,
+                       (const float2*)c, (const float*)ctf, (const float2*)yhat,
+                       (const float*)lam_inv, B, (int)r, (int)L, (float)noise_var, mean_const,
+                       (float*)cost, (float*)z, (float*)m_out, (float*)minv, (float*)b_out);
+  else
+    hipLaunchKernelGGL((cov3d::k_ml_forward<0>), dim3((unsigned)B), dim3(cov3d::kMlBlock), 0, st,
+                       (const float2*)c, (const float*)ctf, (const float2*)yhat,
+                       (const float*)lam_inv, B, (int)r, (int)L, (float)noise_var, mean_const,
+                       (float*)cost, (float*)z, (float*)m_out, (float*)minv, (float*)b_out);
+  COV3D_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int cov3d_ml_grad_images(const void* c, const void* ctf, const void* yhat,
+                                    const void* z, const void* minv, void* g, int64_t B,
+                                    int64_t r, int64_t L, double noise_var, double grad_scale,
+                                    void* stream) {
+  COV3D_CHECK_ARG(r >= 1 && r <= 31, "rank r=%lld must be in [1, 31]", (long long)r);
+  COV3D_CHECK_ARG(B >= 0 && L >= 1, "bad batch/size");
+  COV3D_CHECK_ARG(noise_var > 0, "noise_var must be positive");
+  if (B == 0) return 0;
+  COV3D_CHECK_ARG(c && yhat && z && minv && g, "NULL pointer");
+  COV3D_CHECK_ARG(B < 65536, "batch too large for one launch (B=%lld)", (long long)B);
+  hipStream_t st = cov3d::as_stream(stream);
+  const int64_t d = L * L;
+  const float scale = (float)(grad_scale / ((double)(L * L) * noise_var));
+  dim3 grid((unsigned)cov3d::ceil_div(d, cov3d::kMlBlock), (unsigned)B);
+#define COV3D_GRAD(RM)                                                                            \
+  hipLaunchKernelGGL(cov3d::k_ml_grad<RM>, grid, dim3(cov3d::kMlBlock), 0, st, (const float2*)c, \
+                     (const float*)ctf, (const float2*)yhat, (const float*)z, (const float*)minv, \
+                     (float2*)g, B, (int)r, (int)L, scale)
+  if (r <= 4)
+    COV3D_GRAD(4);
+  else if (r <= 8)
+    COV3D_GRAD(8);
+  else if (r <= 12)
+    COV3D_GRAD(12);
+  else if (r <= 16)
+    COV3D_GRAD(16);
+  else if (r <= 20)
+    COV3D_GRAD(20);
+  else if (r <= 24)
+    COV3D_GRAD(24);
+  else
+    COV3D_GRAD(32);
+#undef COV3D_GRAD
+  COV3D_LAUNCH_CHECK();
+  return 0;
+}

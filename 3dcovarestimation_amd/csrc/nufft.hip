@@ -1,0 +1,968 @@
+// 3-D type-1 / type-2 NUFFT for gfx950 with the exponential-of-semicircle
+// (ES) kernel, oversampling factor 2 and rocFFT for the dense fine-grid FFT.
+//
+// Replaces the finufft / cufinufft plans driven by cov3d/nufft_plan.py:263-392.
+// Method (published FINUFFT / cuFINUFFT design, re-derived here):
+//   type 2: deconvolve + zero-pad modes onto the (nf0, nf1, nf2) fine grid,
+//           FFT, then interpolate every point with a W^3 ES stencil;
+//   type 1: spread every point with the W^3 ES stencil, FFT, deconvolve+crop.
+// Points are bin-sorted once per setpts.  Interpolation and spreading work on
+// one bin at a time: the bin's fine-grid tile plus a W-wide halo is staged in
+// LDS with the transform index innermost ([z][y][x][t]), so each stencil tap
+// is a contiguous LDS read (type 2) or LDS atomic (type 1); the type-1 tile
+// is flushed to HBM with one float atomic per tile cell.
+#include <math.h>
+#include <rocfft/rocfft.h>
+
+#include <mutex>
+#include <vector>
+
+#include "common.h"
+
+namespace cov3d {
+
+// ----------------------------------------------------------------------------
+// Parameters
+// ----------------------------------------------------------------------------
+
+// Kernel width for a requested tolerance at oversampling 2: w = ceil(log10(1/eps)) + 1.
+static int es_width(double eps) {
+  int w = (int)ceil(log10(1.0 / eps)) + 1;
+  if (w < 2) w = 2;
+  if (w > 16) w = 16;
+  return w;
+}
+
+// ES shape parameter beta = c(w) * w at oversampling 2.
+static double es_beta(int w) {
+  double b = 2.30;
+  if (w == 2) b = 2.20;
+  else if (w == 3) b = 2.26;
+  else if (w == 4) b = 2.38;
+  return b * w;
+}
+
+// Smallest even integer >= n whose only prime factors are 2, 3, 5.
+static int64_t next235even(int64_t n) {
+  if (n <= 2) return 2;
+  if (n % 2) n++;
+  for (int64_t m = n;; m += 2) {
+    int64_t k = m;
+    while (k % 2 == 0) k /= 2;
+    while (k % 3 == 0) k /= 3;
+    while (k % 5 == 0) k /= 5;
+    if (k == 1) return m;
+  }
+}
+
+// Gauss-Legendre nodes/weights on [-1, 1] (Newton on P_n).
+static void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& wts) {
+  x.resize(n);
+  wts.resize(n);
+  for (int i = 0; i < n; ++i) {
+    double z = cos(M_PI * (i + 0.75) / (n + 0.5));
+    double pp = 0;
+    for (int it = 0; it < 100; ++it) {
+      double p1 = 1, p2 = 0;
+      for (int j = 1; j <= n; ++j) {
+        double p3 = p2;
+        p2 = p1;
+        p1 = ((2.0 * j - 1) * z * p2 - (j - 1.0) * p3) / j;
+      }
+      pp = n * (z * p1 - p2) / (z * z - 1);
+      double dz = p1 / pp;
+      z -= dz;
+      if (fabs(dz) < 1e-16) break;
+    }
+    x[i] = z;
+    wts[i] = 2.0 / ((1 - z * z) * pp * pp);
+  }
+}
+
+static inline double es_eval_host(double t, int w, double beta) {
+  double c = 4.0 / ((double)w * w);
+  double s = 1.0 - c * t * t;
+  if (s < 0) return 0;
+  return exp(beta * (sqrt(s) - 1.0));
+}
+
+// 1/p(n) for n in [-floor(N/2), ceil(N/2)-1], p(n) = int phi(t) cos(2 pi n t / nf) dt.
+static std::vector<double> deconv_factors(int64_t N, int64_t nf, int w, double beta) {
+  std::vector<double> gx, gw;
+  gauss_legendre(128, gx, gw);
+  std::vector<double> out(N);
+  double half = 0.5 * w;
+  for (int64_t i = 0; i < N; ++i) {
+    double n = (double)(i - N / 2);
+    double s = 0;
+    for (size_t q = 0; q < gx.size(); ++q) {
+      double t = half * gx[q];  // nodes on [-w/2, w/2]
+      s += half * gw[q] * es_eval_host(t, w, beta) * cos(2.0 * M_PI * n * t / (double)nf);
+    }
+    out[i] = 1.0 / s;
+  }
+  return out;
+}
+
+// ----------------------------------------------------------------------------
+// Device kernels
+// ----------------------------------------------------------------------------
+
+constexpr int kBlock = 256;
+constexpr int kMaxSub = 1024;  // points per subproblem
+
+// ----------------------------------------------------------------------------
+// Optional HIP-event timing of kernel classes on the stream they run on.
+// ----------------------------------------------------------------------------
+enum TimedKernel { kTInterp = 0, kTSpread = 1, kTDeconvPad = 2, kTDeconvCrop = 3, kTFft = 4, kTSort = 5, kTCount };
+
+struct TimingRegistry {
+  bool enabled = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[kTCount];
+};
+static TimingRegistry g_timing;
+
+struct ScopedTiming {
+  int k;
+  hipStream_t st;
+  hipEvent_t a = nullptr, b = nullptr;
+  ScopedTiming(int kind, hipStream_t s) : k(kind), st(s) {
+    if (!g_timing.enabled) return;
+    hipEventCreate(&a);
+    hipEventCreate(&b);
+    hipEventRecord(a, st);
+  }
+  ~ScopedTiming() {
+    if (!a) return;
+    hipEventRecord(b, st);
+    g_timing.ev[k].push_back({a, b});
+  }
+};
+
+template <typename T>
+__device__ __forceinline__ T es_eval(T t, T c, T beta) {
+  T s = (T)1 - c * t * t;
+  s = s > (T)0 ? s : (T)0;
+  return exp(beta * (sqrt(s) - (T)1));
+}
+
+__global__ void k_fill_int(int* p, int64_t n, int v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+template <typename T>
+__global__ void k_fill_zero(T* p, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = (T)0;
+}
+
+struct BinGeom {
+  int nf[3];
+  int bs[3];
+  int nb[3];
+};
+
+// Wrapped fine-grid coordinate (double) of one point, its bin and local offset.
+template <typename T>
+__device__ __forceinline__ int point_bin(const T* x, const T* y, const T* z, int64_t j,
+                                         const BinGeom& g, double u[3]) {
+  const double inv2pi = 0.15915494309189533577;
+  double p[3] = {(double)x[j], (double)y[j], (double)z[j]};
+  int b[3];
+  for (int k = 0; k < 3; ++k) {
+    double nf = (double)g.nf[k];
+    double X = p[k] * inv2pi * nf;
+    X -= nf * floor(X / nf);
+    if (X >= nf) X -= nf;
+    if (X < 0) X = 0;
+    int bi = (int)floor(X) / g.bs[k];
+    if (bi > g.nb[k] - 1) bi = g.nb[k] - 1;
+    b[k] = bi;
+    u[k] = X - (double)bi * g.bs[k];
+  }
+  return (b[0] * g.nb[1] + b[1]) * g.nb[2] + b[2];
+}
+
+template <typename T>
+__global__ void k_bin_count(const T* x, const T* y, const T* z, int64_t M, BinGeom g,
+                            int* bin_count, int* pt_bin, int* pt_pos) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= M) return;
+  double u[3];
+  int bin = point_bin(x, y, z, j, g, u);
+  pt_bin[j] = bin;
+  pt_pos[j] = atomicAdd(&bin_count[bin], 1);
+}
+
+// Single-workgroup exclusive scans of bin counts and subproblem counts.
+__global__ void __launch_bounds__(1024) k_bin_scan(const int* bin_count, int64_t nbins,
+                                                   int* bin_start, int* sub_start,
+                                                   int* nsub_total) {
+  __shared__ int s_pts[1024];
+  __shared__ int s_sub[1024];
+  const int tid = threadIdx.x;
+  const int64_t chunk = (nbins + 1023) / 1024;
+  const int64_t lo = tid * chunk;
+  const int64_t hi = lo + chunk < nbins ? lo + chunk : nbins;
+  int sp = 0, ss = 0;
+  for (int64_t i = lo; i < hi; ++i) {
+    int c = bin_count[i];
+    sp += c;
+    ss += (c + kMaxSub - 1) / kMaxSub;
+  }
+  s_pts[tid] = sp;
+  s_sub[tid] = ss;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    int a = tid >= off ? s_pts[tid - off] : 0;
+    int b = tid >= off ? s_sub[tid - off] : 0;
+    __syncthreads();
+    s_pts[tid] += a;
+    s_sub[tid] += b;
+    __syncthreads();
+  }
+  int run_p = s_pts[tid] - sp;
+  int run_s = s_sub[tid] - ss;
+  for (int64_t i = lo; i < hi; ++i) {
+    int c = bin_count[i];
+    bin_start[i] = run_p;
+    sub_start[i] = run_s;
+    run_p += c;
+    run_s += (c + kMaxSub - 1) / kMaxSub;
+  }
+  if (tid == 1023) *nsub_total = s_sub[1023];
+}
+
+__global__ void k_sub_fill(const int* bin_count, const int* bin_start, const int* sub_start,
+                           int64_t nbins, int* sub_bin, int* sub_pt, int* sub_n) {
+  int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbins) return;
+  int c = bin_count[b];
+  int s0 = sub_start[b];
+  int p0 = bin_start[b];
+  for (int s = 0; s * kMaxSub < c; ++s) {
+    sub_bin[s0 + s] = (int)b;
+    sub_pt[s0 + s] = p0 + s * kMaxSub;
+    int rem = c - s * kMaxSub;
+    sub_n[s0 + s] = rem < kMaxSub ? rem : kMaxSub;
+  }
+}
+
+template <typename T>
+__global__ void k_bin_scatter(const T* x, const T* y, const T* z, int64_t M, BinGeom g,
+                              const int* pt_bin, const int* pt_pos, const int* bin_start,
+                              int* sidx, T* su0, T* su1, T* su2) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= M) return;
+  double u[3];
+  (void)point_bin(x, y, z, j, g, u);
+  int dst = bin_start[pt_bin[j]] + pt_pos[j];
+  sidx[dst] = (int)j;
+  su0[dst] = (T)u[0];
+  su1[dst] = (T)u[1];
+  su2[dst] = (T)u[2];
+}
+
+// Modes -> zero-padded, deconvolved fine grid (one thread per fine-grid cell).
+template <typename T, bool REAL_IN>
+__global__ void k_deconv_pad(const T* __restrict__ f, Cplx<T>* __restrict__ fine, int64_t ntrans,
+                             int64_t nrun, int N0, int N1, int N2, int nf0, int nf1, int nf2,
+                             const T* __restrict__ ip0, const T* __restrict__ ip1,
+                             const T* __restrict__ ip2) {
+  const int64_t nfine = (int64_t)nf0 * nf1 * nf2;
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nfine * ntrans) return;
+  int64_t t = idx / nfine;
+  int64_t r = idx - t * nfine;
+  int m2 = (int)(r % nf2);
+  int64_t r2 = r / nf2;
+  int m1 = (int)(r2 % nf1);
+  int m0 = (int)(r2 / nf1);
+  // fine index m <-> mode n = m (m < ceil(N/2)) or m - nf (m >= nf - floor(N/2))
+  auto mode_index = [](int m, int N, int nf) -> int {
+    int hp = (N + 1) / 2, hn = N / 2;
+    if (m < hp) return m + hn;
+    if (m >= nf - hn) return m - nf + hn;
+    return -1;
+  };
+  int i0 = mode_index(m0, N0, nf0), i1 = mode_index(m1, N1, nf1), i2 = mode_index(m2, N2, nf2);
+  Cplx<T> v = {(T)0, (T)0};
+  if (t < nrun && i0 >= 0 && i1 >= 0 && i2 >= 0) {
+    T s = ip0[i0] * ip1[i1] * ip2[i2];
+    int64_t src = ((t * N0 + i0) * N1 + i1) * N2 + i2;
+    if (REAL_IN) {
+      v.x = f[src] * s;
+    } else {
+      v.x = f[2 * src] * s;
+      v.y = f[2 * src + 1] * s;
+    }
+  }
+  fine[idx] = v;
+}
+
+// Fine grid -> deconvolved modes (one thread per output mode).
+template <typename T, bool REAL_OUT>
+__global__ void k_deconv_crop(const Cplx<T>* __restrict__ fine, T* __restrict__ f, int64_t ntrans,
+                              int N0, int N1, int N2, int nf0, int nf1, int nf2,
+                              const T* __restrict__ ip0, const T* __restrict__ ip1,
+                              const T* __restrict__ ip2) {
+  const int64_t nm = (int64_t)N0 * N1 * N2;
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nm * ntrans) return;
+  int64_t t = idx / nm;
+  int64_t r = idx - t * nm;
+  int i2 = (int)(r % N2);
+  int64_t r2 = r / N2;
+  int i1 = (int)(r2 % N1);
+  int i0 = (int)(r2 / N1);
+  auto fine_index = [](int i, int N, int nf) -> int {
+    int n = i - N / 2;
+    return n >= 0 ? n : n + nf;
+  };
+  int m0 = fine_index(i0, N0, nf0), m1 = fine_index(i1, N1, nf1), m2 = fine_index(i2, N2, nf2);
+  Cplx<T> v = fine[((t * nf0 + m0) * (int64_t)nf1 + m1) * nf2 + m2];
+  T s = ip0[i0] * ip1[i1] * ip2[i2];
+  if (REAL_OUT) {
+    f[idx] = v.x * s;
+  } else {
+    f[2 * idx] = v.x * s;
+    f[2 * idx + 1] = v.y * s;
+  }
+}
+
+struct SubDesc {
+  const int* sub_bin;
+  const int* sub_pt;
+  const int* sub_n;
+  const int* nsub;
+};
+
+// Maps the flat workgroup index to a subproblem so that each of the 8 XCDs
+// (which receive workgroups round-robin) walks a contiguous run of bins and
+// shares halo cells through its own L2.
+__device__ __forceinline__ int xcd_remap(int bid, int nblk) {
+  if ((nblk & 7) != 0) return bid;
+  return (bid & 7) * (nblk >> 3) + (bid >> 3);
+}
+
+// Loads the haloed tile of bin `bin` for transforms [t0, t0+TC) into LDS,
+// layout [e0][e1][e2][TC].
+template <typename T, int W, int TC>
+__device__ __forceinline__ void load_tile(Cplx<T>* tile, const Cplx<T>* __restrict__ fine, int bin,
+                                          const BinGeom& g, int t0, int ntrans, int E0, int E1,
+                                          int E2, int o0, int o1, int o2) {
+  const int64_t nfine = (int64_t)g.nf[0] * g.nf[1] * g.nf[2];
+  const int ncell = E0 * E1 * E2;
+  for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
+    int e2 = c % E2;
+    int e1 = (c / E2) % E1;
+    int e0 = c / (E1 * E2);
+    int m0 = o0 + e0, m1 = o1 + e1, m2 = o2 + e2;
+    m0 = ((m0 % g.nf[0]) + g.nf[0]) % g.nf[0];
+    m1 = ((m1 % g.nf[1]) + g.nf[1]) % g.nf[1];
+    m2 = ((m2 % g.nf[2]) + g.nf[2]) % g.nf[2];
+    int64_t off = ((int64_t)m0 * g.nf[1] + m1) * g.nf[2] + m2;
+#pragma unroll
+    for (int tt = 0; tt < TC; ++tt) {
+      int t = t0 + tt;
+      Cplx<T> v = {(T)0, (T)0};
+      if (t < ntrans) v = fine[t * nfine + off];
+      tile[c * TC + tt] = v;
+    }
+  }
+}
+
+template <typename T, int W>
+__device__ __forceinline__ void stencil_weights(T u, T c, T beta, int& l0, T* w) {
+  const T halfw = (T)(0.5 * W);
+  int j0 = (int)ceil(u - halfw);
+#pragma unroll
+  for (int a = 0; a < W; ++a) w[a] = es_eval<T>((T)(j0 + a) - u, c, beta);
+  l0 = j0 + W / 2;
+}
+
+// Type-2 interpolation: c[t, j] = sum_taps phi * fine[t, tap].
+template <typename T, int W, int TC>
+__global__ void __launch_bounds__(kBlock)
+    k_interp(const Cplx<T>* __restrict__ fine, Cplx<T>* __restrict__ c, SubDesc sd, BinGeom g,
+             const T* __restrict__ su0, const T* __restrict__ su1, const T* __restrict__ su2,
+             const int* __restrict__ sidx, int64_t M, int ntrans, T kc, T beta) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  Cplx<T>* tile = reinterpret_cast<Cplx<T>*>(smem_raw);
+  const int sub = xcd_remap(blockIdx.x, gridDim.x);
+  if (sub >= *sd.nsub) return;
+  const int t0 = blockIdx.y * TC;
+  const int bin = sd.sub_bin[sub];
+  const int b2 = bin % g.nb[2];
+  const int b1 = (bin / g.nb[2]) % g.nb[1];
+  const int b0 = bin / (g.nb[2] * g.nb[1]);
+  const int E0 = g.bs[0] + W, E1 = g.bs[1] + W, E2 = g.bs[2] + W;
+  load_tile<T, W, TC>(tile, fine, bin, g, t0, ntrans, E0, E1, E2, b0 * g.bs[0] - W / 2,
+                      b1 * g.bs[1] - W / 2, b2 * g.bs[2] - W / 2);
+  __syncthreads();
+  const int p0 = sd.sub_pt[sub];
+  const int np = sd.sub_n[sub];
+  for (int p = p0 + threadIdx.x; p < p0 + np; p += blockDim.x) {
+    T w0[W], w1[W], w2[W];
+    int l0, l1, l2;
+    stencil_weights<T, W>(su0[p], kc, beta, l0, w0);
+    stencil_weights<T, W>(su1[p], kc, beta, l1, w1);
+    stencil_weights<T, W>(su2[p], kc, beta, l2, w2);
+    l0 = min(max(l0, 0), g.bs[0]);
+    l1 = min(max(l1, 0), g.bs[1]);
+    l2 = min(max(l2, 0), g.bs[2]);
+    T ax[TC], ay[TC];
+#pragma unroll
+    for (int tt = 0; tt < TC; ++tt) ax[tt] = ay[tt] = (T)0;
+    for (int a0 = 0; a0 < W; ++a0) {
+#pragma unroll
+      for (int a1 = 0; a1 < W; ++a1) {
+        const T w01 = w0[a0] * w1[a1];
+        const Cplx<T>* row = tile + ((int64_t)((l0 + a0) * E1 + (l1 + a1)) * E2 + l2) * TC;
+#pragma unroll
+        for (int a2 = 0; a2 < W; ++a2) {
+          const T ww = w01 * w2[a2];
+#pragma unroll
+          for (int tt = 0; tt < TC; ++tt) {
+            Cplx<T> v = row[a2 * TC + tt];
+            ax[tt] += ww * v.x;
+            ay[tt] += ww * v.y;
+          }
+        }
+      }
+    }
+    const int64_t j = sidx[p];
+#pragma unroll
+    for (int tt = 0; tt < TC; ++tt) {
+      int t = t0 + tt;
+      if (t < ntrans) c[(int64_t)t * M + j] = Cplx<T>{ax[tt], ay[tt]};
+    }
+  }
+}
+
+// Type-1 spreading: fine[t, tap] += phi * c[t, j], accumulated per bin in LDS.
+template <typename T, int W, int TC>
+__global__ void __launch_bounds__(kBlock)
+    k_spread(const Cplx<T>* __restrict__ c, Cplx<T>* __restrict__ fine, SubDesc sd, BinGeom g,
+             const T* __restrict__ su0, const T* __restrict__ su1, const T* __restrict__ su2,
+             const int* __restrict__ sidx, int64_t M, int ntrans, T kc, T beta) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  Cplx<T>* tile = reinterpret_cast<Cplx<T>*>(smem_raw);
+  const int sub = xcd_remap(blockIdx.x, gridDim.x);
+  if (sub >= *sd.nsub) return;
+  const int t0 = blockIdx.y * TC;
+  const int bin = sd.sub_bin[sub];
+  const int b2 = bin % g.nb[2];
+  const int b1 = (bin / g.nb[2]) % g.nb[1];
+  const int b0 = bin / (g.nb[2] * g.nb[1]);
+  const int E0 = g.bs[0] + W, E1 = g.bs[1] + W, E2 = g.bs[2] + W;
+  const int ncell = E0 * E1 * E2;
+  T* tf = reinterpret_cast<T*>(tile);
+  for (int i = threadIdx.x; i < ncell * TC * 2; i += blockDim.x) tf[i] = (T)0;
+  __syncthreads();
+  const int p0 = sd.sub_pt[sub];
+  const int np = sd.sub_n[sub];
+  for (int p = p0 + threadIdx.x; p < p0 + np; p += blockDim.x) {
+    T w0[W], w1[W], w2[W];
+    int l0, l1, l2;
+    stencil_weights<T, W>(su0[p], kc, beta, l0, w0);
+    stencil_weights<T, W>(su1[p], kc, beta, l1, w1);
+    stencil_weights<T, W>(su2[p], kc, beta, l2, w2);
+    l0 = min(max(l0, 0), g.bs[0]);
+    l1 = min(max(l1, 0), g.bs[1]);
+    l2 = min(max(l2, 0), g.bs[2]);
+    const int64_t j = sidx[p];
+    Cplx<T> cv[TC];
+#pragma unroll
+    for (int tt = 0; tt < TC; ++tt) {
+      int t = t0 + tt;
+      cv[tt] = t < ntrans ? c[(int64_t)t * M + j] : Cplx<T>{(T)0, (T)0};
+    }
+    for (int a0 = 0; a0 < W; ++a0) {
+#pragma unroll
+      for (int a1 = 0; a1 < W; ++a1) {
+        const T w01 = w0[a0] * w1[a1];
+        T* row = tf + ((int64_t)((l0 + a0) * E1 + (l1 + a1)) * E2 + l2) * TC * 2;
+#pragma unroll
+        for (int a2 = 0; a2 < W; ++a2) {
+          const T ww = w01 * w2[a2];
+#pragma unroll
+          for (int tt = 0; tt < TC; ++tt) {
+            atomicAdd(row + (a2 * TC + tt) * 2, ww * cv[tt].x);
+            atomicAdd(row + (a2 * TC + tt) * 2 + 1, ww * cv[tt].y);
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const int64_t nfine = (int64_t)g.nf[0] * g.nf[1] * g.nf[2];
+  const int o0 = b0 * g.bs[0] - W / 2, o1 = b1 * g.bs[1] - W / 2, o2 = b2 * g.bs[2] - W / 2;
+  T* finef = reinterpret_cast<T*>(fine);
+  for (int cidx = threadIdx.x; cidx < ncell; cidx += blockDim.x) {
+    int e2 = cidx % E2;
+    int e1 = (cidx / E2) % E1;
+    int e0 = cidx / (E1 * E2);
+    int m0 = ((o0 + e0) % g.nf[0] + g.nf[0]) % g.nf[0];
+    int m1 = ((o1 + e1) % g.nf[1] + g.nf[1]) % g.nf[1];
+    int m2 = ((o2 + e2) % g.nf[2] + g.nf[2]) % g.nf[2];
+    int64_t off = ((int64_t)m0 * g.nf[1] + m1) * g.nf[2] + m2;
+#pragma unroll
+    for (int tt = 0; tt < TC; ++tt) {
+      int t = t0 + tt;
+      if (t >= ntrans) break;
+      T vx = tf[(cidx * TC + tt) * 2];
+      T vy = tf[(cidx * TC + tt) * 2 + 1];
+      if (vx != (T)0) atomicAdd(finef + 2 * (t * nfine + off), vx);
+      if (vy != (T)0) atomicAdd(finef + 2 * (t * nfine + off) + 1, vy);
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Plan
+// ----------------------------------------------------------------------------
+
+struct Plan {
+  int type = 2;   // 0 = both, 1, 2
+  int dtype = 0;  // 0 = f32, 1 = f64
+  int iflag2 = -1;
+  int64_t N[3] = {0, 0, 0};
+  int64_t nf[3] = {0, 0, 0};
+  int64_t ntrans = 1;
+  int w = 7;
+  int tc = 2;
+  double beta = 0, eps = 1e-6;
+  BinGeom geom;
+  int64_t nbins = 0;
+  void* fine = nullptr;
+  void* invp[3] = {nullptr, nullptr, nullptr};
+  int64_t M = 0, M_cap = 0;
+  int64_t sub_cap = 0;
+  int* bin_count = nullptr;
+  int* bin_start = nullptr;
+  int* sub_start = nullptr;
+  int* sub_bin = nullptr;
+  int* sub_pt = nullptr;
+  int* sub_n = nullptr;
+  int* nsub = nullptr;
+  int* pt_bin = nullptr;
+  int* pt_pos = nullptr;
+  int* sidx = nullptr;
+  void* su[3] = {nullptr, nullptr, nullptr};
+  rocfft_plan fft_fwd = nullptr, fft_bwd = nullptr;
+  rocfft_execution_info info = nullptr;
+  void* fft_work = nullptr;
+  size_t work_size = 0;
+  bool pts_set = false;
+  size_t tile_bytes = 0;
+};
+
+static std::once_flag g_rocfft_once;
+
+static int free_points(Plan* p) {
+  hipFree(p->pt_bin);
+  hipFree(p->pt_pos);
+  hipFree(p->sidx);
+  for (int k = 0; k < 3; ++k) hipFree(p->su[k]);
+  hipFree(p->sub_bin);
+  hipFree(p->sub_pt);
+  hipFree(p->sub_n);
+  p->pt_bin = p->pt_pos = p->sidx = nullptr;
+  p->sub_bin = p->sub_pt = p->sub_n = nullptr;
+  for (int k = 0; k < 3; ++k) p->su[k] = nullptr;
+  p->M_cap = 0;
+  p->sub_cap = 0;
+  return 0;
+}
+
+#define COV3D_FFT(call)                                               \
+  do {                                                                \
+    rocfft_status s_ = (call);                                        \
+    if (s_ != rocfft_status_success) {                                \
+      ::cov3d::set_error("%s failed: rocfft status %d", #call, (int)s_); \
+      return COV3D_ERR_FFT;                                           \
+    }                                                                 \
+  } while (0)
+
+template <typename T, int W, int TC>
+static int launch_interp_spread(Plan* p, bool spread, const void* c_in, void* c_out, int ntrans_run,
+                                hipStream_t st) {
+  const size_t smem = (size_t)(p->geom.bs[0] + W) * (p->geom.bs[1] + W) * (p->geom.bs[2] + W) * TC *
+                      sizeof(Cplx<T>);
+  static bool attr_done[2] = {false, false};
+  if (smem > 65536 && !attr_done[spread ? 1 : 0]) {
+    if (spread)
+      COV3D_HIP(hipFuncSetAttribute((const void*)k_spread<T, W, TC>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    else
+      COV3D_HIP(hipFuncSetAttribute((const void*)k_interp<T, W, TC>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    attr_done[spread ? 1 : 0] = true;
+  }
+  COV3D_CHECK_ARG(smem <= 160 * 1024, "NUFFT tile of %zu bytes exceeds LDS", smem);
+  SubDesc sd{p->sub_bin, p->sub_pt, p->sub_n, p->nsub};
+  dim3 grid((unsigned)p->sub_cap, (unsigned)ceil_div(ntrans_run, TC));
+  const T kc = (T)(4.0 / ((double)W * W));
+  const T beta = (T)p->beta;
+  ScopedTiming tm(spread ? kTSpread : kTInterp, st);
+  if (spread) {
+    hipLaunchKernelGGL((k_spread<T, W, TC>), grid, dim3(kBlock), smem, st,
+                       (const Cplx<T>*)c_in, (Cplx<T>*)p->fine, sd, p->geom, (const T*)p->su[0],
+                       (const T*)p->su[1], (const T*)p->su[2], p->sidx, p->M, ntrans_run, kc, beta);
+  } else {
+    hipLaunchKernelGGL((k_interp<T, W, TC>), grid, dim3(kBlock), smem, st,
+                       (const Cplx<T>*)p->fine, (Cplx<T>*)c_out, sd, p->geom, (const T*)p->su[0],
+                       (const T*)p->su[1], (const T*)p->su[2], p->sidx, p->M, ntrans_run, kc, beta);
+  }
+  COV3D_LAUNCH_CHECK();
+  return 0;
+}
+
+template <typename T, int TC>
+static int dispatch_w(Plan* p, bool spread, const void* c_in, void* c_out, int ntrans_run,
+                      hipStream_t st) {
+  switch (p->w) {
+#define COV3D_W(WW) \
+  case WW:          \
+    return launch_interp_spread<T, WW, TC>(p, spread, c_in, c_out, ntrans_run, st);
+    COV3D_W(2) COV3D_W(3) COV3D_W(4) COV3D_W(5) COV3D_W(6) COV3D_W(7) COV3D_W(8) COV3D_W(9)
+    COV3D_W(10) COV3D_W(11) COV3D_W(12) COV3D_W(13) COV3D_W(14) COV3D_W(15) COV3D_W(16)
+#undef COV3D_W
+    default:
+      set_error("unsupported kernel width %d", p->w);
+      return COV3D_ERR_ARG;
+  }
+}
+
+static int run_interp_spread(Plan* p, bool spread, const void* c_in, void* c_out, int ntrans_run,
+                             hipStream_t st) {
+  if (p->M == 0) return 0;
+  if (p->dtype == 1) return dispatch_w<double, 1>(p, spread, c_in, c_out, ntrans_run, st);
+  if (p->tc == 4) return dispatch_w<float, 4>(p, spread, c_in, c_out, ntrans_run, st);
+  if (p->tc == 1) return dispatch_w<float, 1>(p, spread, c_in, c_out, ntrans_run, st);
+  return dispatch_w<float, 2>(p, spread, c_in, c_out, ntrans_run, st);
+}
+
+static int env_int(const char* name, int dflt) {
+  const char* v = getenv(name);
+  if (!v || !*v) return dflt;
+  return atoi(v);
+}
+
+static int exec_fft(Plan* p, bool forward_sign, hipStream_t st) {
+  COV3D_FFT(rocfft_execution_info_set_stream(p->info, st));
+  ScopedTiming tm(kTFft, st);
+  void* bufs[1] = {p->fine};
+  COV3D_FFT(rocfft_execute(forward_sign ? p->fft_fwd : p->fft_bwd, bufs, nullptr, p->info));
+  return 0;
+}
+
+template <typename T>
+static int exec_type2(Plan* p, const void* f, bool f_real, void* c, int64_t nrun, hipStream_t st) {
+  const int64_t nfine = p->nf[0] * p->nf[1] * p->nf[2];
+  const int64_t total = nfine * p->ntrans;
+  dim3 g1((unsigned)ceil_div(total, kBlock));
+  {
+  ScopedTiming tm(kTDeconvPad, st);
+  if (f_real)
+    hipLaunchKernelGGL((k_deconv_pad<T, true>), g1, dim3(kBlock), 0, st, (const T*)f,
+                       (Cplx<T>*)p->fine, p->ntrans, nrun, (int)p->N[0], (int)p->N[1],
+                       (int)p->N[2], (int)p->nf[0], (int)p->nf[1], (int)p->nf[2],
+                       (const T*)p->invp[0], (const T*)p->invp[1], (const T*)p->invp[2]);
+  else
+    hipLaunchKernelGGL((k_deconv_pad<T, false>), g1, dim3(kBlock), 0, st, (const T*)f,
+                       (Cplx<T>*)p->fine, p->ntrans, nrun, (int)p->N[0], (int)p->N[1],
+                       (int)p->N[2], (int)p->nf[0], (int)p->nf[1], (int)p->nf[2],
+                       (const T*)p->invp[0], (const T*)p->invp[1], (const T*)p->invp[2]);
+  }
+  COV3D_LAUNCH_CHECK();
+  int rc = exec_fft(p, p->iflag2 < 0, st);
+  if (rc) return rc;
+  return run_interp_spread(p, false, nullptr, c, (int)nrun, st);
+}
+
+template <typename T>
+static int exec_type1(Plan* p, const void* c, void* f, bool f_real, int64_t nrun, hipStream_t st) {
+  const int64_t nfine = p->nf[0] * p->nf[1] * p->nf[2];
+  const int64_t total = nfine * p->ntrans * 2;
+  hipLaunchKernelGGL((k_fill_zero<T>), dim3((unsigned)std::min<int64_t>(ceil_div(total, kBlock), 65536)),
+                     dim3(kBlock), 0, st, (T*)p->fine, total);
+  COV3D_LAUNCH_CHECK();
+  int rc = run_interp_spread(p, true, c, nullptr, (int)nrun, st);
+  if (rc) return rc;
+  // type 1 uses the opposite sign of type 2
+  rc = exec_fft(p, !(p->iflag2 < 0), st);
+  if (rc) return rc;
+  const int64_t nm = p->N[0] * p->N[1] * p->N[2];
+  dim3 g1((unsigned)ceil_div(nm * nrun, kBlock));
+  ScopedTiming tm(kTDeconvCrop, st);
+  if (f_real)
+    hipLaunchKernelGGL((k_deconv_crop<T, true>), g1, dim3(kBlock), 0, st, (const Cplx<T>*)p->fine,
+                       (T*)f, nrun, (int)p->N[0], (int)p->N[1], (int)p->N[2], (int)p->nf[0],
+                       (int)p->nf[1], (int)p->nf[2], (const T*)p->invp[0], (const T*)p->invp[1],
+                       (const T*)p->invp[2]);
+  else
+    hipLaunchKernelGGL((k_deconv_crop<T, false>), g1, dim3(kBlock), 0, st, (const Cplx<T>*)p->fine,
+                       (T*)f, nrun, (int)p->N[0], (int)p->N[1], (int)p->N[2], (int)p->nf[0],
+                       (int)p->nf[1], (int)p->nf[2], (const T*)p->invp[0], (const T*)p->invp[1],
+                       (const T*)p->invp[2]);
+  COV3D_LAUNCH_CHECK();
+  return 0;
+}
+
+template <typename T>
+static int setpts_impl(Plan* p, int64_t M, const T* x, const T* y, const T* z, hipStream_t st) {
+  if (M > p->M_cap) {
+    free_points(p);
+    COV3D_HIP(hipMalloc(&p->pt_bin, M * sizeof(int)));
+    COV3D_HIP(hipMalloc(&p->pt_pos, M * sizeof(int)));
+    COV3D_HIP(hipMalloc(&p->sidx, M * sizeof(int)));
+    for (int k = 0; k < 3; ++k) COV3D_HIP(hipMalloc(&p->su[k], M * sizeof(T)));
+    int64_t cap = std::min<int64_t>(p->nbins, M) + ceil_div(M, kMaxSub);
+    cap = ceil_div(cap, 8) * 8;
+    COV3D_HIP(hipMalloc(&p->sub_bin, cap * sizeof(int)));
+    COV3D_HIP(hipMalloc(&p->sub_pt, cap * sizeof(int)));
+    COV3D_HIP(hipMalloc(&p->sub_n, cap * sizeof(int)));
+    p->M_cap = M;
+    p->sub_cap = cap;
+  }
+  p->M = M;
+  if (M == 0) {
+    p->pts_set = true;
+    return 0;
+  }
+  int64_t cap = std::min<int64_t>(p->nbins, M) + ceil_div(M, kMaxSub);
+  p->sub_cap = ceil_div(cap, 8) * 8;
+  ScopedTiming tm(kTSort, st);
+  hipLaunchKernelGGL(k_fill_int, dim3((unsigned)ceil_div(p->nbins, kBlock)), dim3(kBlock), 0, st,
+                     p->bin_count, p->nbins, 0);
+  COV3D_LAUNCH_CHECK();
+  dim3 gm((unsigned)ceil_div(M, kBlock));
+  hipLaunchKernelGGL((k_bin_count<T>), gm, dim3(kBlock), 0, st, x, y, z, M, p->geom, p->bin_count,
+                     p->pt_bin, p->pt_pos);
+  COV3D_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, st, p->bin_count, p->nbins, p->bin_start,
+                     p->sub_start, p->nsub);
+  COV3D_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_sub_fill, dim3((unsigned)ceil_div(p->nbins, kBlock)), dim3(kBlock), 0, st,
+                     p->bin_count, p->bin_start, p->sub_start, p->nbins, p->sub_bin, p->sub_pt,
+                     p->sub_n);
+  COV3D_LAUNCH_CHECK();
+  hipLaunchKernelGGL((k_bin_scatter<T>), gm, dim3(kBlock), 0, st, x, y, z, M, p->geom, p->pt_bin,
+                     p->pt_pos, p->bin_start, p->sidx, (T*)p->su[0], (T*)p->su[1], (T*)p->su[2]);
+  COV3D_LAUNCH_CHECK();
+  p->pts_set = true;
+  return 0;
+}
+
+static int destroy_plan(Plan* p) {
+  if (!p) return 0;
+  free_points(p);
+  hipFree(p->fine);
+  for (int k = 0; k < 3; ++k) hipFree(p->invp[k]);
+  hipFree(p->bin_count);
+  hipFree(p->bin_start);
+  hipFree(p->sub_start);
+  hipFree(p->nsub);
+  hipFree(p->fft_work);
+  if (p->fft_fwd) rocfft_plan_destroy(p->fft_fwd);
+  if (p->fft_bwd) rocfft_plan_destroy(p->fft_bwd);
+  if (p->info) rocfft_execution_info_destroy(p->info);
+  delete p;
+  return 0;
+}
+
+static int make_plan(int type, const int64_t* n, int iflag, int64_t ntrans, double eps, int dtype,
+                     Plan** out) {
+  Plan* p = new Plan();
+  p->type = type;
+  p->dtype = dtype;
+  p->iflag2 = iflag == 0 ? -1 : (type == 1 ? -iflag : iflag);
+  p->ntrans = ntrans;
+  double emin = dtype == 0 ? 10 * 1.1920929e-07 : 10 * 2.220446049250313e-16;
+  p->eps = eps < emin ? emin : eps;
+  p->w = es_width(p->eps);
+  p->beta = es_beta(p->w);
+  int bs_env = env_int("COV3D_NUFFT_BIN", 0);
+  int tc_env = env_int("COV3D_NUFFT_TC", 2);
+  p->tc = dtype == 1 ? 1 : (tc_env == 1 || tc_env == 4 ? tc_env : 2);
+  const size_t csize = dtype == 1 ? 16 : 8;
+  for (int k = 0; k < 3; ++k) {
+    p->N[k] = n[k];
+    p->nf[k] = next235even(std::max<int64_t>(2 * n[k], 2 * p->w));
+  }
+  // Bin edge: largest in [2, 8] whose haloed tile fits 64 KB of LDS (or the override).
+  int bs = 8;
+  while (bs > 2 && (size_t)(bs + p->w) * (bs + p->w) * (bs + p->w) * p->tc * csize > 65536) --bs;
+  if (bs_env > 0) bs = bs_env;
+  for (int k = 0; k < 3; ++k) {
+    p->geom.nf[k] = (int)p->nf[k];
+    p->geom.bs[k] = bs;
+    p->geom.nb[k] = (int)ceil_div(p->nf[k], bs);
+  }
+  p->nbins = (int64_t)p->geom.nb[0] * p->geom.nb[1] * p->geom.nb[2];
+  const int64_t nfine = p->nf[0] * p->nf[1] * p->nf[2];
+  auto fail = [&](int rc) {
+    destroy_plan(p);
+    return rc;
+  };
+  if (hipMalloc(&p->fine, nfine * ntrans * csize) != hipSuccess) {
+    set_error("hipMalloc of %lld-byte fine grid failed", (long long)(nfine * ntrans * csize));
+    return fail(COV3D_ERR_HIP);
+  }
+  for (int k = 0; k < 3; ++k) {
+    std::vector<double> ip = deconv_factors(p->N[k], p->nf[k], p->w, p->beta);
+    if (hipMalloc(&p->invp[k], p->N[k] * (dtype ? 8 : 4)) != hipSuccess) return fail(COV3D_ERR_HIP);
+    if (dtype == 1) {
+      if (hipMemcpy(p->invp[k], ip.data(), p->N[k] * 8, hipMemcpyHostToDevice) != hipSuccess)
+        return fail(COV3D_ERR_HIP);
+    } else {
+      std::vector<float> ipf(ip.begin(), ip.end());
+      if (hipMemcpy(p->invp[k], ipf.data(), p->N[k] * 4, hipMemcpyHostToDevice) != hipSuccess)
+        return fail(COV3D_ERR_HIP);
+    }
+  }
+  if (hipMalloc(&p->bin_count, p->nbins * sizeof(int)) != hipSuccess ||
+      hipMalloc(&p->bin_start, p->nbins * sizeof(int)) != hipSuccess ||
+      hipMalloc(&p->sub_start, p->nbins * sizeof(int)) != hipSuccess ||
+      hipMalloc(&p->nsub, sizeof(int)) != hipSuccess) {
+    set_error("hipMalloc of bin tables failed");
+    return fail(COV3D_ERR_HIP);
+  }
+  std::call_once(g_rocfft_once, []() { rocfft_setup(); });
+  size_t lengths[3] = {(size_t)p->nf[2], (size_t)p->nf[1], (size_t)p->nf[0]};
+  rocfft_precision prec = dtype == 1 ? rocfft_precision_double : rocfft_precision_single;
+  if (rocfft_plan_create(&p->fft_fwd, rocfft_placement_inplace, rocfft_transform_type_complex_forward,
+                         prec, 3, lengths, (size_t)ntrans, nullptr) != rocfft_status_success ||
+      rocfft_plan_create(&p->fft_bwd, rocfft_placement_inplace,
+                         rocfft_transform_type_complex_inverse, prec, 3, lengths, (size_t)ntrans,
+                         nullptr) != rocfft_status_success ||
+      rocfft_execution_info_create(&p->info) != rocfft_status_success) {
+    set_error("rocfft plan creation failed for %lldx%lldx%lld x %lld", (long long)p->nf[0],
+              (long long)p->nf[1], (long long)p->nf[2], (long long)ntrans);
+    return fail(COV3D_ERR_FFT);
+  }
+  size_t w1 = 0, w2 = 0;
+  rocfft_plan_get_work_buffer_size(p->fft_fwd, &w1);
+  rocfft_plan_get_work_buffer_size(p->fft_bwd, &w2);
+  p->work_size = std::max(w1, w2);
+  if (p->work_size) {
+    if (hipMalloc(&p->fft_work, p->work_size) != hipSuccess) return fail(COV3D_ERR_HIP);
+    if (rocfft_execution_info_set_work_buffer(p->info, p->fft_work, p->work_size) !=
+        rocfft_status_success)
+      return fail(COV3D_ERR_FFT);
+  }
+  *out = p;
+  return 0;
+}
+
+}  // namespace cov3d
+
+using cov3d::Plan;
+
+extern "C" int cov3d_nufft_makeplan(int type, int64_t n0, int64_t n1, int64_t n2, int iflag,
+                                    int64_t ntrans, double eps, int dtype, void** plan) {
+  COV3D_CHECK_ARG(plan != nullptr, "plan pointer is NULL");
+  COV3D_CHECK_ARG(type >= 0 && type <= 2, "type must be 0, 1 or 2 (got %d)", type);
+  COV3D_CHECK_ARG(n0 >= 1 && n1 >= 1 && n2 >= 1, "mode counts must be >= 1");
+  COV3D_CHECK_ARG(ntrans >= 1, "ntrans must be >= 1");
+  COV3D_CHECK_ARG(dtype == 0 || dtype == 1, "dtype must be 0 (f32) or 1 (f64)");
+  COV3D_CHECK_ARG(eps > 0, "eps must be positive");
+  int64_t n[3] = {n0, n1, n2};
+  Plan* p = nullptr;
+  int rc = cov3d::make_plan(type, n, iflag, ntrans, eps, dtype, &p);
+  if (rc) return rc;
+  *plan = p;
+  return 0;
+}
+
+extern "C" int cov3d_nufft_setpts(void* plan, int64_t M, const void* x, const void* y,
+                                  const void* z, void* stream) {
+  COV3D_CHECK_ARG(plan != nullptr, "plan is NULL");
+  COV3D_CHECK_ARG(M >= 0 && M < (int64_t)INT32_MAX, "point count %lld out of range", (long long)M);
+  COV3D_CHECK_ARG(M == 0 || (x && y && z), "point arrays must not be NULL");
+  Plan* p = static_cast<Plan*>(plan);
+  hipStream_t st = cov3d::as_stream(stream);
+  if (p->dtype == 1)
+    return cov3d::setpts_impl<double>(p, M, (const double*)x, (const double*)y, (const double*)z, st);
+  return cov3d::setpts_impl<float>(p, M, (const float*)x, (const float*)y, (const float*)z, st);
+}
+
+extern "C" int cov3d_nufft_execute_dir(void* plan, int type, void* c, void* f, int f_is_real,
+                                       int64_t ntrans_run, void* stream) {
+  COV3D_CHECK_ARG(plan != nullptr, "plan is NULL");
+  Plan* p = static_cast<Plan*>(plan);
+  COV3D_CHECK_ARG(type == 1 || type == 2, "direction must be 1 or 2");
+  COV3D_CHECK_ARG(p->type == 0 || p->type == type, "plan of type %d cannot run type %d", p->type,
+                  type);
+  if (!p->pts_set) {
+    cov3d::set_error("execute called before setpts");
+    return COV3D_ERR_STATE;
+  }
+  if (ntrans_run <= 0) ntrans_run = p->ntrans;
+  COV3D_CHECK_ARG(ntrans_run <= p->ntrans, "ntrans_run %lld > plan ntrans %lld",
+                  (long long)ntrans_run, (long long)p->ntrans);
+  COV3D_CHECK_ARG(f != nullptr && (c != nullptr || p->M == 0), "data pointers must not be NULL");
+  hipStream_t st = cov3d::as_stream(stream);
+  if (type == 2) {
+    if (p->dtype == 1) return cov3d::exec_type2<double>(p, f, f_is_real != 0, c, ntrans_run, st);
+    return cov3d::exec_type2<float>(p, f, f_is_real != 0, c, ntrans_run, st);
+  }
+  if (p->dtype == 1) return cov3d::exec_type1<double>(p, c, f, f_is_real != 0, ntrans_run, st);
+  return cov3d::exec_type1<float>(p, c, f, f_is_real != 0, ntrans_run, st);
+}
+
+extern "C" int cov3d_nufft_execute(void* plan, void* c, void* f, int f_is_real, int64_t ntrans_run,
+                                   void* stream) {
+  COV3D_CHECK_ARG(plan != nullptr, "plan is NULL");
+  Plan* p = static_cast<Plan*>(plan);
+  COV3D_CHECK_ARG(p->type != 0, "type-0 plans need cov3d_nufft_execute_dir");
+  return cov3d_nufft_execute_dir(plan, p->type, c, f, f_is_real, ntrans_run, stream);
+}
+
+extern "C" int cov3d_nufft_destroy(void* plan) {
+  return cov3d::destroy_plan(static_cast<Plan*>(plan));
+}
+
+extern "C" int cov3d_nufft_info(void* plan, int64_t* host_nf3, int* host_w, double* host_beta) {
+  COV3D_CHECK_ARG(plan != nullptr, "plan is NULL");
+  Plan* p = static_cast<Plan*>(plan);
+  if (host_nf3)
+    for (int k = 0; k < 3; ++k) host_nf3[k] = p->nf[k];
+  if (host_w) *host_w = p->w;
+  if (host_beta) *host_beta = p->beta;
+  return 0;
+}
+
+// Enables (1) or disables (0) event timing of the NUFFT kernel classes and
+// drops every recorded interval.
+extern "C" int cov3d_timing_enable(int on) {
+  for (int k = 0; k < cov3d::kTCount; ++k) {
+    for (auto& e : cov3d::g_timing.ev[k]) {
+      hipEventDestroy(e.first);
+      hipEventDestroy(e.second);
+    }
+    cov3d::g_timing.ev[k].clear();
+  }
+  cov3d::g_timing.enabled = on != 0;
+  return 0;
+}
+
+// Sums the recorded intervals of one kernel class (0 interp, 1 spread,
+// 2 deconv/pad, 3 deconv/crop, 4 rocFFT, 5 bin sort); waits for them.
+extern "C" int cov3d_timing_read(int kind, double* host_total_ms, int64_t* host_count) {
+  COV3D_CHECK_ARG(kind >= 0 && kind < cov3d::kTCount, "bad timing kind %d", kind);
+  double tot = 0;
+  for (auto& e : cov3d::g_timing.ev[kind]) {
+    COV3D_HIP(hipEventSynchronize(e.second));
+    float ms = 0;
+    COV3D_HIP(hipEventElapsedTime(&ms, e.first, e.second));
+    tot += ms;
+  }
+  if (host_total_ms) *host_total_ms = tot;
+  if (host_count) *host_count = (int64_t)cov3d::g_timing.ev[kind].size();
+  return 0;
+}

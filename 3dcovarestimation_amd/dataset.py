@@ -1,0 +1,307 @@
+"""Device-resident particle dataset, batch iterator and synthetic generator.
+
+``CovarDataset`` keeps the interface the trainers use from the reference's
+cov3d/dataset.py:32-515 (``images``, ``filters``, ``noise_var``,
+``resolution``, ``__getitem__`` -> (images, pts_rot, filters, idx),
+``to_fourier_domain`` / ``to_spatial_domain``, ``half_split``).  Instead of
+materialising pts_rot (N, 3, L^2) on the host (dataset.py:146-164: 19.7 GB at
+L=128, N=100k) it stores one rotation matrix per image in HBM and generates
+the rotated slice frequencies per batch with ``cov3d_pts_rot``; CTFs can be
+stored or generated per batch with ``cov3d_ctf_eval``.
+
+``create_dataloader`` yields contiguous, unshuffled batches like
+BatchIndexSampler(shuffle=False) (dataset.py:798-895).
+``make_synthetic`` builds the benchmark workloads (Gaussian-blob mean and
+eigenvolumes, Haar-uniform SO(3) poses, radial CTF, white noise at an SNR).
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from . import _native
+from .projection_funcs import centered_fft2, centered_ifft2
+
+
+def pts_rot_from_rotations(rots: torch.Tensor, L: int) -> torch.Tensor:
+    """(B, 3, 3) float32 rotations on a HIP device -> (B, 3, L*L) wrapped zyx frequencies."""
+    _native.require_device(rots, "rotations")
+    rots = rots.to(torch.float32).contiguous()
+    out = torch.empty((rots.shape[0], 3, L * L), dtype=torch.float32, device=rots.device)
+    _native.check(_native.load().cov3d_pts_rot(rots.data_ptr(), out.data_ptr(), rots.shape[0], L,
+                                                _native.stream_ptr(rots.device)))
+    return out
+
+
+def ctf_from_params(params: torch.Tensor, L: int) -> torch.Tensor:
+    """(B, 7) [dU, dV, ang, kV, Cs, amp, pixel] on a HIP device -> (B, L, L) CTF."""
+    _native.require_device(params, "ctf params")
+    params = params.to(torch.float32).contiguous()
+    out = torch.empty((params.shape[0], L, L), dtype=torch.float32, device=params.device)
+    _native.check(_native.load().cov3d_ctf_eval(params.data_ptr(), out.data_ptr(), params.shape[0], L,
+                                                 _native.stream_ptr(params.device)))
+    return out
+
+
+def rotvec_to_rotmat(rotvecs: torch.Tensor) -> torch.Tensor:
+    """Rodrigues' formula (poses.py:44-67)."""
+    theta = torch.norm(rotvecs, dim=-1, keepdim=True)
+    k = rotvecs / (theta + 1e-6)
+    K = torch.zeros((rotvecs.shape[0], 3, 3), device=rotvecs.device, dtype=rotvecs.dtype)
+    K[:, 0, 1], K[:, 0, 2] = -k[:, 2], k[:, 1]
+    K[:, 1, 0], K[:, 1, 2] = k[:, 2], -k[:, 0]
+    K[:, 2, 0], K[:, 2, 1] = -k[:, 1], k[:, 0]
+    eye = torch.eye(3, device=rotvecs.device, dtype=rotvecs.dtype).unsqueeze(0)
+    return eye + torch.sin(theta).unsqueeze(-1) * K + (1 - torch.cos(theta).unsqueeze(-1)) * (K @ K)
+
+
+def haar_rotations(n: int, generator: torch.Generator, dtype=torch.float32) -> torch.Tensor:
+    """Haar-uniform SO(3) rotations from normalised Gaussian quaternions."""
+    q = torch.randn((n, 4), generator=generator, dtype=torch.float64)
+    q = q / q.norm(dim=1, keepdim=True)
+    w, x, y, z = q.unbind(1)
+    R = torch.stack(
+        [
+            1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w),
+            2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w),
+            2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y),
+        ],
+        dim=1,
+    ).reshape(n, 3, 3)
+    return R.to(dtype)
+
+
+class CovarDataset:
+    """Particle stack with per-image rotations and CTFs, resident on one device."""
+
+    def __init__(
+        self,
+        images: torch.Tensor,
+        rotations: torch.Tensor,
+        noise_var: float,
+        filters: Optional[torch.Tensor] = None,
+        ctf_params: Optional[torch.Tensor] = None,
+        offsets: Optional[torch.Tensor] = None,
+        mask: Optional[torch.Tensor] = None,
+    ) -> None:
+        if filters is None and ctf_params is None:
+            raise ValueError("need filters or ctf_params")
+        self.images = images
+        self.rotations = rotations
+        self.filters = filters
+        self.ctf_params = ctf_params
+        self.noise_var = float(noise_var)
+        self.resolution = images.shape[-1]
+        self.offsets = offsets if offsets is not None else torch.zeros((len(images), 2), device=images.device)
+        self.contrasts = torch.ones(len(images), device=images.device)
+        self.mask = mask
+        self._in_spatial_domain = not images.is_complex()
+        self.dtype = images.real.dtype if images.is_complex() else images.dtype
+
+    def __len__(self) -> int:
+        return self.images.shape[0]
+
+    def get_filters(self, idx) -> torch.Tensor:
+        if self.filters is not None:
+            return self.filters[idx]
+        p = self.ctf_params[idx]
+        return ctf_from_params(p.reshape(-1, 7), self.resolution).reshape(*p.shape[:-1], self.resolution,
+                                                                            self.resolution)
+
+    def __getitem__(self, idx):
+        rots = self.rotations[idx]
+        single = rots.ndim == 2
+        pts = pts_rot_from_rotations(rots.reshape(-1, 3, 3), self.resolution)
+        if single:
+            pts = pts[0]
+        return self.images[idx], pts, self.get_filters(idx), idx
+
+    def to_fourier_domain(self) -> None:
+        """images <- fft2c(images), noise_var *= L^2 (dataset.py:426-433)."""
+        if self._in_spatial_domain:
+            self.images = centered_fft2(self.images)
+            self.noise_var *= self.resolution**2
+            self._in_spatial_domain = False
+
+    def to_spatial_domain(self) -> None:
+        """Inverse of to_fourier_domain (dataset.py:435-439)."""
+        if not self._in_spatial_domain:
+            self.images = centered_ifft2(self.images).real
+            self.noise_var /= self.resolution**2
+            self._in_spatial_domain = True
+
+    def _subset(self, idx: torch.Tensor) -> "CovarDataset":
+        sub = CovarDataset(
+            self.images[idx], self.rotations[idx], self.noise_var,
+            filters=None if self.filters is None else self.filters[idx],
+            ctf_params=None if self.ctf_params is None else self.ctf_params[idx],
+            offsets=self.offsets[idx], mask=self.mask,
+        )
+        sub._in_spatial_domain = self._in_spatial_domain
+        return sub
+
+    def half_split(self) -> Tuple["CovarDataset", "CovarDataset"]:
+        """Even/odd split into two half-sets."""
+        n = len(self)
+        dev = self.images.device
+        return self._subset(torch.arange(0, n, 2, device=dev)), self._subset(torch.arange(1, n, 2, device=dev))
+
+    def filter_gain(self, batch_size: int = 4096) -> torch.Tensor:
+        """Mean CTF^2 over the stack, (L, L)."""
+        acc = None
+        for i in range(0, len(self), batch_size):
+            f = self.get_filters(slice(i, min(i + batch_size, len(self))))
+            s = (f.float() ** 2).sum(dim=0)
+            acc = s if acc is None else acc + s
+        return acc / len(self)
+
+
+class BatchIterator:
+    """Contiguous unshuffled batches of a dataset (BatchIndexSampler(shuffle=False))."""
+
+    def __init__(self, dataset, batch_size: int, idx: Optional[torch.Tensor] = None) -> None:
+        self.dataset = dataset
+        self.batch_size = int(batch_size)
+        self.idx = idx
+
+    def __len__(self) -> int:
+        n = len(self.dataset) if self.idx is None else len(self.idx)
+        return math.ceil(n / self.batch_size)
+
+    def __iter__(self):
+        n = len(self.dataset) if self.idx is None else len(self.idx)
+        for i in range(0, n, self.batch_size):
+            if self.idx is None:
+                yield self.dataset[i : min(i + self.batch_size, n)]
+            else:
+                yield self.dataset[self.idx[i : i + self.batch_size]]
+
+
+def create_dataloader(dataset, batch_size: int, idx: Optional[torch.Tensor] = None, **unused) -> BatchIterator:
+    """Batch iterator over a device-resident dataset (dataset.py:850-895)."""
+    return BatchIterator(dataset, batch_size, idx)
+
+
+# ---------------------------------------------------------------------------
+# Synthetic workloads
+# ---------------------------------------------------------------------------
+
+
+def gaussian_blob_volume(L: int, n_blobs: int, gen: torch.Generator, radius_frac: float = 0.6,
+                         sigma_range=(0.04, 0.08)) -> torch.Tensor:
+    """Sum of isotropic Gaussian blobs with centres uniform in a ball of radius radius_frac*L/2."""
+    ax = torch.arange(L, dtype=torch.float64) - L // 2
+    zz, yy, xx = torch.meshgrid(ax, ax, ax, indexing="ij")
+    vol = torch.zeros((L, L, L), dtype=torch.float64)
+    R = radius_frac * L / 2
+    for _ in range(n_blobs):
+        d = torch.randn(3, generator=gen, dtype=torch.float64)
+        d = d / d.norm()
+        rad = R * torch.rand(1, generator=gen, dtype=torch.float64) ** (1 / 3)
+        c = d * rad
+        s = (sigma_range[0] + (sigma_range[1] - sigma_range[0]) * torch.rand(1, generator=gen,
+                                                                            dtype=torch.float64)) * L
+        vol += torch.exp(-((zz - c[0]) ** 2 + (yy - c[1]) ** 2 + (xx - c[2]) ** 2) / (2 * s**2))
+    return vol.to(torch.float32)
+
+
+@dataclass
+class SyntheticConfig:
+    L: int
+    n_images: int
+    rank: int
+    eigenvalues: Sequence[float]
+    pixel_size: float
+    snr: float
+    defocus_um: Tuple[float, float]
+    shift_px: float = 0.0
+    seed: int = 1000
+    voltage_kv: float = 300.0
+    cs_mm: float = 2.7
+    amplitude_contrast: float = 0.1
+
+
+def config_by_name(name: str) -> SyntheticConfig:
+    """The benchmark workloads: L32 (CPU-sized), L64, L128 (SGD), L128-mean."""
+    if name == "L32":
+        return SyntheticConfig(32, 2000, 4, [4, 2, 1, 0.5], 3.0, 0.1, (1.0, 2.5), 0.0, 1001)
+    if name == "L64":
+        return SyntheticConfig(64, 20000, 10, list(np.geomspace(10, 0.1, 10)), 3.0, 0.05, (0.8, 3.0), 2.0, 1002)
+    if name == "L128":
+        return SyntheticConfig(128, 100000, 20, list(np.geomspace(10, 0.1, 20)), 1.5, 0.02, (0.5, 3.0), 0.0, 1003)
+    if name == "L128-mean":
+        return SyntheticConfig(128, 100000, 0, [], 1.5, 0.02, (0.5, 3.0), 0.0, 1004)
+    raise KeyError(name)
+
+
+@dataclass
+class SyntheticData:
+    dataset: CovarDataset
+    mean: torch.Tensor
+    eigenvecs: torch.Tensor  # (r, L, L, L) orthonormal
+    eigenvals: torch.Tensor
+    latents: Optional[torch.Tensor] = None
+    extra: dict = field(default_factory=dict)
+
+
+def make_synthetic(cfg: SyntheticConfig, device, n_images: Optional[int] = None, subtract_mean: bool = True,
+                   batch: int = 1024, store_filters: bool = True) -> SyntheticData:
+    """Generates images y_i = CTF_i * A_i(mean + sum_k z_ik u_k) + noise on the device.
+
+    Projections use the gfx950 type-2 NUFFT (vol_forward on a NufftPlan).
+    With ``subtract_mean`` the projected mean is removed, as the reference's
+    preprocessing does before covariance estimation.
+    """
+    from .nufft_plan import NufftPlan
+    from .projection_funcs import vol_forward
+
+    dev = torch.device(device)
+    n = cfg.n_images if n_images is None else n_images
+    L, r = cfg.L, cfg.rank
+    gen = torch.Generator().manual_seed(cfg.seed)
+    mean = gaussian_blob_volume(L, 8, gen)
+    if r > 0:
+        pert = torch.stack([gaussian_blob_volume(L, 4, gen) for _ in range(r)]).reshape(r, -1)
+        q, _ = torch.linalg.qr(pert.T.double())
+        eigvecs = q.T.reshape(r, L, L, L).to(torch.float32)
+        eigvals = torch.tensor(cfg.eigenvalues, dtype=torch.float32)
+    else:
+        eigvecs = torch.zeros((0, L, L, L))
+        eigvals = torch.zeros(0)
+    rots = haar_rotations(n, gen).to(dev)
+    du = torch.empty(n, dtype=torch.float64).uniform_(cfg.defocus_um[0], cfg.defocus_um[1], generator=gen) * 1e4
+    ctf_params = torch.stack(
+        [du, du, torch.zeros(n, dtype=torch.float64), torch.full((n,), cfg.voltage_kv, dtype=torch.float64),
+         torch.full((n,), cfg.cs_mm, dtype=torch.float64), torch.full((n,), cfg.amplitude_contrast, dtype=torch.float64),
+         torch.full((n,), cfg.pixel_size, dtype=torch.float64)], dim=1).to(torch.float32).to(dev)
+    latents = (torch.randn((n, r), generator=gen) * eigvals.sqrt()).to(dev) if r > 0 else None
+    vols = torch.cat([mean.unsqueeze(0), eigvecs], dim=0).to(dev)
+    plan = NufftPlan((L,) * 3, batch_size=r + 1, device=dev)
+    images = torch.empty((n, L, L), dtype=torch.float32, device=dev)
+    filters = torch.empty((n, L, L), dtype=torch.float32, device=dev) if store_filters else None
+    for i in range(0, n, batch):
+        j = min(i + batch, n)
+        plan.setpts(pts_rot_from_rotations(rots[i:j], L))
+        f = ctf_from_params(ctf_params[i:j], L)
+        proj = vol_forward(vols, plan, f)  # (b, r+1, L, L)
+        if proj.ndim == 3:
+            proj = proj.unsqueeze(0)
+        img = proj[:, 1:].mul(latents[i:j, :, None, None]).sum(1) if r > 0 else torch.zeros_like(proj[:, 0])
+        if not subtract_mean:
+            img = img + proj[:, 0]
+        images[i:j] = img
+        if filters is not None:
+            filters[i:j] = f
+    signal_var = float(images.var())
+    noise_var = signal_var / cfg.snr if signal_var > 0 else 1.0
+    g = torch.Generator(device=dev).manual_seed(cfg.seed + 7)
+    for i in range(0, n, 8 * batch):
+        j = min(i + 8 * batch, n)
+        images[i:j] += math.sqrt(noise_var) * torch.randn((j - i, L, L), device=dev, generator=g)
+    ds = CovarDataset(images, rots, noise_var, filters=filters, ctf_params=ctf_params)
+    return SyntheticData(ds, mean.to(dev), eigvecs.to(dev), eigvals.to(dev), latents)

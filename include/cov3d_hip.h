@@ -1,0 +1,204 @@
+/*
+ * cov3d_hip.h -- C ABI of the MI355X (gfx950) COV3D hot-path library
+ * (lib cov3d_hip.so, built from 3dcovarestimation_amd/csrc/).
+ *
+ * Conventions shared by every entry point:
+ *   - every pointer argument is a DEVICE pointer unless named host_*;
+ *     buffers are owned by the caller, contiguous, and stay valid until the
+ *     work queued on `stream` has finished;
+ *   - `stream` is a hipStream_t passed as void* (NULL = the null stream);
+ *   - the return value is 0 on success and a nonzero status otherwise;
+ *     cov3d_last_error() then returns a thread-local message;
+ *   - complex data is interleaved (re, im) pairs of the plan's real type.
+ *
+ * Citations name the reference interface (RoeyYadgar/3DCovarEstimation)
+ * that each entry point replaces.
+ */
+#ifndef COV3D_HIP_H
+#define COV3D_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define COV3D_OK 0
+#define COV3D_ERR_ARG 1      /* bad shape, null pointer or unsupported option */
+#define COV3D_ERR_HIP 2      /* a HIP runtime call failed                    */
+#define COV3D_ERR_FFT 3      /* a rocFFT call failed                          */
+#define COV3D_ERR_STATE 4    /* call out of order (e.g. execute before setpts) */
+
+/* Thread-local description of the last failure on this thread. */
+const char* cov3d_last_error(void);
+
+/* Library version string. */
+const char* cov3d_version(void);
+
+/* ------------------------------------------------------------------------
+ * 3D NUFFT plans (exponential-of-semicircle kernel, oversampling 2).
+ * Replaces the finufft/cufinufft Plan used by cov3d/nufft_plan.py:263-392
+ * (NufftPlan.__init__ builds one type-2 plan for execute_forward and one
+ * type-1 plan for execute_adjoint; setpts feeds both, :318-336).
+ *
+ *   type 2 (forward):  c[t, j] = sum_n f[t, n] exp(iflag * i * n . p_j)
+ *   type 1 (adjoint):  f[t, n] = sum_j c[t, j] exp(iflag * i * n . p_j)
+ *   n_k in [-floor(N_k/2), ceil(N_k/2)-1] stored at index n_k + floor(N_k/2)
+ *   (C order, axis 0 slowest); axis k pairs with point coordinate k.
+ *   f has shape (ntrans, N0, N1, N2), c has shape (ntrans, M).
+ *
+ * type = 1 or 2 gives a plan for that direction; type = 0 gives a plan that
+ * executes both directions over one sorted point set (the pair NufftPlan
+ * holds).  iflag = 0 selects the reference defaults (-1 for type 2, +1 for
+ * type 1); for type 0 a nonzero iflag is the type-2 sign and the type-1
+ * sign is its negation.  dtype: 0 = float32/complex64, 1 = float64/complex128.
+ * ------------------------------------------------------------------------ */
+int cov3d_nufft_makeplan(int type, int64_t n0, int64_t n1, int64_t n2,
+                         int iflag, int64_t ntrans, double eps, int dtype,
+                         void** plan);
+
+/* Points: three arrays of M coordinates in radians (any real value; they are
+ * wrapped to [-pi, pi) like nufft_plan.py:327).  Same real type as the plan.
+ * Sorts the points into fine-grid bins on `stream`.  The plan keeps no
+ * pointer to x/y/z after the call returns its queued work. */
+int cov3d_nufft_setpts(void* plan, int64_t M, const void* x, const void* y,
+                       const void* z, void* stream);
+
+/* Executes the plan's single direction (type 1 or 2 plans):
+ *   type 2: f (input, real or complex per f_is_real) -> c (output, complex)
+ *   type 1: c (input, complex) -> f (output; real part only if f_is_real)
+ * ntrans_run <= ntrans transforms are run (0 = all). */
+int cov3d_nufft_execute(void* plan, void* c, void* f, int f_is_real,
+                        int64_t ntrans_run, void* stream);
+
+/* Executes one direction of a type-0 (both-direction) plan. */
+int cov3d_nufft_execute_dir(void* plan, int type, void* c, void* f,
+                            int f_is_real, int64_t ntrans_run, void* stream);
+
+/* Frees the plan's device buffers and rocFFT plans. */
+int cov3d_nufft_destroy(void* plan);
+
+/* Query: fine-grid size per axis and kernel width chosen for eps. */
+int cov3d_nufft_info(void* plan, int64_t* host_nf3, int* host_w,
+                     double* host_beta);
+
+/* ------------------------------------------------------------------------
+ * Discretised Fourier-slice extraction on an upsampled centred-FFT volume.
+ * Replaces torch.nn.functional.grid_sample (3-D, align_corners=True, zero
+ * padding) in NufftPlanDiscretized.execute_forward (nufft_plan.py:120-169)
+ * and its autograd adjoint in execute_adjoint / execute_adjoint_unaggregated
+ * (nufft_plan.py:171-242).
+ *   vol:  (ntrans, D, H, W) complex, grid: (npts, 3) real in grid_sample's
+ *   normalised xyz convention (already scaled by NufftPlanDiscretized.setpts,
+ *   nufft_plan.py:94-118), out: (ntrans, npts) complex.
+ *   mode: 0 = trilinear ("bilinear"), 1 = nearest.
+ * Adjoint: out (ntrans, D, H, W) is accumulated into (caller zeroes it);
+ *   aggregated = 1 sums all points into one volume per trans (execute_adjoint);
+ *   aggregated = 0 writes volume k from the points [k*pts_per_vol,
+ *   (k+1)*pts_per_vol) (execute_adjoint_unaggregated), ntrans must be 1.
+ * ------------------------------------------------------------------------ */
+int cov3d_slice_forward(const void* vol, const void* grid, void* out,
+                        int64_t ntrans, int64_t D, int64_t H, int64_t W,
+                        int64_t npts, int mode, int dtype, void* stream);
+int cov3d_slice_adjoint(const void* sig, const void* grid, void* vol_out,
+                        int64_t ntrans, int64_t D, int64_t H, int64_t W,
+                        int64_t npts, int mode, int aggregated,
+                        int64_t pts_per_vol, int dtype, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Maximum-likelihood contractions for rank-r projected eigenvectors.
+ * Replaces cov3d/wiener_coords.py:168-203 (compute_latentMAP_batch: Gram,
+ * P^H y, normalised solve) together with the solve/logdet of
+ * cov3d/covar_sgd.py:1238-1286 (cost_maximum_liklihood) and its autograd.
+ *
+ * Inputs per image i (B images, d = L*L pixels, centred Fourier layout):
+ *   c     (r, B, d) complex64: raw type-2 NUFFT values of the r volumes
+ *   ctf   (B, d) float32 or NULL
+ *   yhat  (B, d) complex64: centred 2-D FFT of the spatial image
+ * The kernel forms X_t = mask * ctf * c_t / L (the Fourier-domain output of
+ * vol_forward, projection_funcs.py:321-370; mask zeroes row/column 0 for
+ * even L), Q_t = X_t / L and y~ = yhat / L, so that real inner products of
+ * (Q, y~) equal the spatial-domain inner products of vol_forward's images.
+ *   M = Q Q^T / noise_var + lam_inv  (lam_inv (r, r) float32, NULL = I)
+ *   b = Q y~ / noise_var,  z = M^{-1} b  (FP64 Cholesky),
+ *   cost_i = 0.5 * (-b.z + logdet M) [+ |y|^2 / noise_var if mean_const]
+ * precision: 0 = FP32 FMA Gram, 1 = BF16 MFMA with split-BF16 (hi/lo)
+ * compensation and FP32 accumulation.
+ * Outputs (any may be NULL): cost (B) f32, z (B, r) f32, m_out (B, r, r) f32,
+ *   minv (B, r, r) f32, b_out (B, r) f32.
+ * r must be in [1, 31].
+ * ------------------------------------------------------------------------ */
+int cov3d_ml_forward(const void* c, const void* ctf, const void* yhat,
+                     const void* lam_inv, int64_t B, int64_t r, int64_t L,
+                     double noise_var, int mean_const, int precision,
+                     void* cost, void* z, void* m_out, void* minv,
+                     void* b_out, void* stream);
+
+/* Hand-written adjoint of the ML cost with respect to the raw type-2 values:
+ *   g_t = mask * ctf * (Minv Q + z (Q^T z - y~))_t * scale
+ * with scale = grad_scale / (L*L * noise_var) (grad_scale = 1/B for the
+ * batch-mean cost).  g (r, B, d) complex64 may alias c (in place).  Its
+ * type-1 NUFFT (real part) is the eigenvolume gradient that autograd
+ * through TorchNufftForward.backward (nufft_plan.py:441-465) produces. */
+int cov3d_ml_grad_images(const void* c, const void* ctf, const void* yhat,
+                         const void* z, const void* minv, void* g,
+                         int64_t B, int64_t r, int64_t L, double noise_var,
+                         double grad_scale, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Helpers on the data feed and the mean-volume solver.
+ * ------------------------------------------------------------------------ */
+
+/* Rotated central-slice frequencies, replaces CovarDataset.compute_pts_rot
+ * (dataset.py:146-164) / PoseModule.forward (poses.py:237-269):
+ *   pts[i, :, k] = wrap(flip_zyx(R_i . pi*[gx_k, gy_k, 0])), R (B, 3, 3) f32,
+ *   out (B, 3, L*L) f32, grid from aspire grid_2d(L, "yx"). */
+int cov3d_pts_rot(const void* rot, void* pts, int64_t B, int64_t L,
+                  void* stream);
+
+/* Radial CTF on the centred L x L grid (textbook formula, the one aspire's
+ * RadialCTFFilter / CTFFilter.evaluate_grid uses, dataset.py:117-125):
+ *   params (B, 7) f32 = [defocus_u A, defocus_v A, defocus_ang rad,
+ *                        voltage kV, Cs mm, amplitude contrast, pixel A]
+ *   out (B, L, L) f32. */
+int cov3d_ctf_eval(const void* params, void* out, int64_t B, int64_t L,
+                   void* stream);
+
+/* out = sum_k conj(x_k) y_k accumulated in FP64 (torch.vdot in
+ * complex_conjugate_gradient, mean.py:43-93).  x, y complex64 of length n,
+ * out: 2 doubles (re, im) in device memory. */
+int cov3d_vdot_c64_f64(const void* x, const void* y, int64_t n, void* out,
+                       void* stream);
+
+/* Toeplitz normal operator of the mean CG (mean.py:118-123) in one call:
+ *   y = crop_L(ifft3c(lhs * fft3c(pad_{nf}(x))))  for x (L^3) complex64,
+ *   lhs (nf^3) complex64 with nf = us * L.  Uses rocFFT; workspace owned by
+ *   a cached per-(L, nf) context. */
+int cov3d_toeplitz_apply(const void* x, const void* lhs, void* y, int64_t L,
+                         int64_t nf, void* stream);
+
+/* Learned diagonal preconditioner MLP (in -> 64 -> 64 -> 64 -> 1, GELU):
+ * forward over n rows; weights packed [W1 b1 W2 b2 W3 b3 W4 b4] f32. */
+int cov3d_mlp_forward(const void* params, const void* x, void* y,
+                      int64_t n, int64_t n_in, void* stream);
+/* Backward: given dL/dy (n), accumulates dL/dparams into grad_params. */
+int cov3d_mlp_backward(const void* params, const void* x, const void* gy,
+                       void* grad_params, int64_t n, int64_t n_in,
+                       void* stream);
+
+/* ------------------------------------------------------------------------
+ * Measurement: HIP-event intervals around every launch of a NUFFT kernel
+ * class, recorded on the stream the kernel runs on (no reference
+ * counterpart; used by the benchmark's roofline figures).
+ *   kind: 0 interpolation, 1 spreading, 2 deconvolve+pad, 3 deconvolve+crop,
+ *         4 rocFFT, 5 bin sort.
+ * ------------------------------------------------------------------------ */
+int cov3d_timing_enable(int on);
+int cov3d_timing_read(int kind, double* host_total_ms, int64_t* host_count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* COV3D_HIP_H */
