@@ -154,41 +154,50 @@ __global__ void __launch_bounds__(kMlBlock)
     }
     __syncthreads();
   } else {
-    // ---- FP32 Gram: stage 32 pixels of all rows in LDS, one (t, s) pair per thread.
+    // ---- FP32 Gram: stage 64 pixels of all rows in LDS; thread k owns the
+    // upper-triangle pairs k, k + 256, k + 512 ((r+1)(r+2)/2 <= 528).
     __shared__ float2 s_rows[kRmax][64];
-    const int np = (r + 1) * (r + 2) / 2;
-    int pt = -1, ps = -1;
-    {
-      int q = tid, t = 0;
-      while (t <= r && q >= (r + 1 - t)) {
-        q -= (r + 1 - t);
-        ++t;
-      }
-      if (tid < np) {
-        pt = t;
-        ps = t + q;
+    const int npair = (r + 1) * (r + 2) / 2;
+    int pt[3], ps[3];
+    float accp[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      int q = tid + k * kMlBlock, t = 0;
+      pt[k] = ps[k] = -1;
+      if (q < npair) {
+        while (q >= (r + 1 - t)) {
+          q -= (r + 1 - t);
+          ++t;
+        }
+        pt[k] = t;
+        ps[k] = t + q;
       }
     }
-    float accp = 0.f;
     for (int64_t k0 = 0; k0 < d; k0 += 64) {
       for (int e = tid; e < (r + 1) * 64; e += blockDim.x) {
         int m = e / 64, kk = e % 64;
         s_rows[m][kk] = gram_row_value(c, yhat, ctf, i, m, r, B, d, k0 + kk, L, inv_l, inv_l2);
       }
       __syncthreads();
-      if (pt >= 0) {
-        for (int kk = 0; kk < 64; ++kk) {
-          float2 a = s_rows[pt][kk], b = s_rows[ps][kk];
-          accp += a.x * b.x + a.y * b.y;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        if (pt[k] >= 0) {
+          for (int kk = 0; kk < 64; ++kk) {
+            float2 a = s_rows[pt[k]][kk], bb = s_rows[ps[k]][kk];
+            accp[k] += a.x * bb.x + a.y * bb.y;
+          }
         }
       }
       __syncthreads();
     }
     for (int e = tid; e < 4 * kRmax * kRmax; e += blockDim.x) (&s_part[0][0])[e] = 0.f;
     __syncthreads();
-    if (pt >= 0) {
-      s_part[0][pt * kRmax + ps] = accp;
-      s_part[0][ps * kRmax + pt] = accp;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      if (pt[k] >= 0) {
+        s_part[0][pt[k] * kRmax + ps[k]] = accp[k];
+        s_part[0][ps[k] * kRmax + pt[k]] = accp[k];
+      }
     }
     __syncthreads();
   }

@@ -158,16 +158,40 @@ __global__ void k_fill_zero(T* p, int64_t n) {
   for (; i < n; i += stride) p[i] = (T)0;
 }
 
+
+// Binning geometry.  Bins are 4^3 fine-grid cells; 2x2x2 bins form an 8^3
+// super-bin (the interpolation tile).  Bins are numbered super-bin-major so
+// the points of one super-bin are contiguous after sorting.  nf is a multiple
+// of 8 on every axis, so bins tile the periodic grid exactly.
+constexpr int kBinEdge = 4;
+constexpr int kSuperEdge = 8;
+
 struct BinGeom {
   int nf[3];
-  int bs[3];
-  int nb[3];
+  int nb[3];   // bins per axis (nf / 4)
+  int nsb[3];  // super-bins per axis (nf / 8)
 };
 
-// Wrapped fine-grid coordinate (double) of one point, its bin and local offset.
+__host__ __device__ __forceinline__ int bin_index(const BinGeom& g, int b0, int b1, int b2) {
+  int sb = ((b0 >> 1) * g.nsb[1] + (b1 >> 1)) * g.nsb[2] + (b2 >> 1);
+  return sb * 8 + ((b0 & 1) << 2) + ((b1 & 1) << 1) + (b2 & 1);
+}
+
+__device__ __forceinline__ void bin_coords(const BinGeom& g, int bin, int& b0, int& b1, int& b2) {
+  int sb = bin >> 3, k = bin & 7;
+  int s2 = sb % g.nsb[2];
+  int s1 = (sb / g.nsb[2]) % g.nsb[1];
+  int s0 = sb / (g.nsb[2] * g.nsb[1]);
+  b0 = 2 * s0 + ((k >> 2) & 1);
+  b1 = 2 * s1 + ((k >> 1) & 1);
+  b2 = 2 * s2 + (k & 1);
+}
+
+// Wrapped fine-grid coordinate X in [0, nf) (double), its bin, and the
+// coordinate relative to the bin's super-bin origin, v = X - 8*floor(b/2).
 template <typename T>
 __device__ __forceinline__ int point_bin(const T* x, const T* y, const T* z, int64_t j,
-                                         const BinGeom& g, double u[3]) {
+                                         const BinGeom& g, double v[3]) {
   const double inv2pi = 0.15915494309189533577;
   double p[3] = {(double)x[j], (double)y[j], (double)z[j]};
   int b[3];
@@ -177,12 +201,12 @@ __device__ __forceinline__ int point_bin(const T* x, const T* y, const T* z, int
     X -= nf * floor(X / nf);
     if (X >= nf) X -= nf;
     if (X < 0) X = 0;
-    int bi = (int)floor(X) / g.bs[k];
+    int bi = (int)floor(X) / kBinEdge;
     if (bi > g.nb[k] - 1) bi = g.nb[k] - 1;
     b[k] = bi;
-    u[k] = X - (double)bi * g.bs[k];
+    v[k] = X - (double)((bi >> 1) * kSuperEdge);
   }
-  return (b[0] * g.nb[1] + b[1]) * g.nb[2] + b[2];
+  return bin_index(g, b[0], b[1], b[2]);
 }
 
 template <typename T>
@@ -190,25 +214,27 @@ __global__ void k_bin_count(const T* x, const T* y, const T* z, int64_t M, BinGe
                             int* bin_count, int* pt_bin, int* pt_pos) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= M) return;
-  double u[3];
-  int bin = point_bin(x, y, z, j, g, u);
+  double v[3];
+  int bin = point_bin(x, y, z, j, g, v);
   pt_bin[j] = bin;
   pt_pos[j] = atomicAdd(&bin_count[bin], 1);
 }
 
-// Single-workgroup exclusive scans of bin counts and subproblem counts.
-__global__ void __launch_bounds__(1024) k_bin_scan(const int* bin_count, int64_t nbins,
+// Single-workgroup scans: bin_start (exclusive over bins) and, per super-bin,
+// the number of interpolation subproblems (ceil(points / kMaxSub)).
+__global__ void __launch_bounds__(1024) k_bin_scan(const int* bin_count, int64_t nsuper,
                                                    int* bin_start, int* sub_start,
                                                    int* nsub_total) {
   __shared__ int s_pts[1024];
   __shared__ int s_sub[1024];
   const int tid = threadIdx.x;
-  const int64_t chunk = (nbins + 1023) / 1024;
+  const int64_t chunk = (nsuper + 1023) / 1024;
   const int64_t lo = tid * chunk;
-  const int64_t hi = lo + chunk < nbins ? lo + chunk : nbins;
+  const int64_t hi = lo + chunk < nsuper ? lo + chunk : nsuper;
   int sp = 0, ss = 0;
-  for (int64_t i = lo; i < hi; ++i) {
-    int c = bin_count[i];
+  for (int64_t s = lo; s < hi; ++s) {
+    int c = 0;
+    for (int k = 0; k < 8; ++k) c += bin_count[8 * s + k];
     sp += c;
     ss += (c + kMaxSub - 1) / kMaxSub;
   }
@@ -225,10 +251,13 @@ __global__ void __launch_bounds__(1024) k_bin_scan(const int* bin_count, int64_t
   }
   int run_p = s_pts[tid] - sp;
   int run_s = s_sub[tid] - ss;
-  for (int64_t i = lo; i < hi; ++i) {
-    int c = bin_count[i];
-    bin_start[i] = run_p;
-    sub_start[i] = run_s;
+  for (int64_t s = lo; s < hi; ++s) {
+    int c = 0;
+    for (int k = 0; k < 8; ++k) {
+      bin_start[8 * s + k] = run_p + c;
+      c += bin_count[8 * s + k];
+    }
+    sub_start[s] = run_s;
     run_p += c;
     run_s += (c + kMaxSub - 1) / kMaxSub;
   }
@@ -236,14 +265,15 @@ __global__ void __launch_bounds__(1024) k_bin_scan(const int* bin_count, int64_t
 }
 
 __global__ void k_sub_fill(const int* bin_count, const int* bin_start, const int* sub_start,
-                           int64_t nbins, int* sub_bin, int* sub_pt, int* sub_n) {
-  int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nbins) return;
-  int c = bin_count[b];
-  int s0 = sub_start[b];
-  int p0 = bin_start[b];
+                           int64_t nsuper, int* sub_sb, int* sub_pt, int* sub_n) {
+  int64_t sb = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (sb >= nsuper) return;
+  int c = 0;
+  for (int k = 0; k < 8; ++k) c += bin_count[8 * sb + k];
+  int s0 = sub_start[sb];
+  int p0 = bin_start[8 * sb];
   for (int s = 0; s * kMaxSub < c; ++s) {
-    sub_bin[s0 + s] = (int)b;
+    sub_sb[s0 + s] = (int)sb;
     sub_pt[s0 + s] = p0 + s * kMaxSub;
     int rem = c - s * kMaxSub;
     sub_n[s0 + s] = rem < kMaxSub ? rem : kMaxSub;
@@ -253,16 +283,16 @@ __global__ void k_sub_fill(const int* bin_count, const int* bin_start, const int
 template <typename T>
 __global__ void k_bin_scatter(const T* x, const T* y, const T* z, int64_t M, BinGeom g,
                               const int* pt_bin, const int* pt_pos, const int* bin_start,
-                              int* sidx, T* su0, T* su1, T* su2) {
+                              int* sidx, T* sv0, T* sv1, T* sv2) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= M) return;
-  double u[3];
-  (void)point_bin(x, y, z, j, g, u);
+  double v[3];
+  (void)point_bin(x, y, z, j, g, v);
   int dst = bin_start[pt_bin[j]] + pt_pos[j];
   sidx[dst] = (int)j;
-  su0[dst] = (T)u[0];
-  su1[dst] = (T)u[1];
-  su2[dst] = (T)u[2];
+  sv0[dst] = (T)v[0];
+  sv1[dst] = (T)v[1];
+  sv2[dst] = (T)v[2];
 }
 
 // Modes -> zero-padded, deconvolved fine grid (one thread per fine-grid cell).
@@ -332,46 +362,20 @@ __global__ void k_deconv_crop(const Cplx<T>* __restrict__ fine, T* __restrict__ 
   }
 }
 
+
 struct SubDesc {
-  const int* sub_bin;
+  const int* sub_sb;
   const int* sub_pt;
   const int* sub_n;
   const int* nsub;
 };
 
-// Maps the flat workgroup index to a subproblem so that each of the 8 XCDs
-// (which receive workgroups round-robin) walks a contiguous run of bins and
-// shares halo cells through its own L2.
+// Maps the flat workgroup index so that each of the 8 XCDs (which receive
+// workgroups round-robin) walks a contiguous run of bins and shares halo
+// data through its own L2.
 __device__ __forceinline__ int xcd_remap(int bid, int nblk) {
   if ((nblk & 7) != 0) return bid;
   return (bid & 7) * (nblk >> 3) + (bid >> 3);
-}
-
-// Loads the haloed tile of bin `bin` for transforms [t0, t0+TC) into LDS,
-// layout [e0][e1][e2][TC].
-template <typename T, int W, int TC>
-__device__ __forceinline__ void load_tile(Cplx<T>* tile, const Cplx<T>* __restrict__ fine, int bin,
-                                          const BinGeom& g, int t0, int ntrans, int E0, int E1,
-                                          int E2, int o0, int o1, int o2) {
-  const int64_t nfine = (int64_t)g.nf[0] * g.nf[1] * g.nf[2];
-  const int ncell = E0 * E1 * E2;
-  for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
-    int e2 = c % E2;
-    int e1 = (c / E2) % E1;
-    int e0 = c / (E1 * E2);
-    int m0 = o0 + e0, m1 = o1 + e1, m2 = o2 + e2;
-    m0 = ((m0 % g.nf[0]) + g.nf[0]) % g.nf[0];
-    m1 = ((m1 % g.nf[1]) + g.nf[1]) % g.nf[1];
-    m2 = ((m2 % g.nf[2]) + g.nf[2]) % g.nf[2];
-    int64_t off = ((int64_t)m0 * g.nf[1] + m1) * g.nf[2] + m2;
-#pragma unroll
-    for (int tt = 0; tt < TC; ++tt) {
-      int t = t0 + tt;
-      Cplx<T> v = {(T)0, (T)0};
-      if (t < ntrans) v = fine[t * nfine + off];
-      tile[c * TC + tt] = v;
-    }
-  }
 }
 
 template <typename T, int W>
@@ -384,35 +388,56 @@ __device__ __forceinline__ void stencil_weights(T u, T c, T beta, int& l0, T* w)
 }
 
 // Type-2 interpolation: c[t, j] = sum_taps phi * fine[t, tap].
+// One workgroup per (super-bin subproblem, transform chunk).  The 8^3 tile
+// plus a W-cell halo is staged in LDS as [cell][TC/2] 16-byte words holding
+// two complex transforms each, so one tap of one point is TC/2 ds_read_b128.
+// TC = 1 (FP64 plans) stores one complex per cell.
 template <typename T, int W, int TC>
 __global__ void __launch_bounds__(kBlock)
     k_interp(const Cplx<T>* __restrict__ fine, Cplx<T>* __restrict__ c, SubDesc sd, BinGeom g,
-             const T* __restrict__ su0, const T* __restrict__ su1, const T* __restrict__ su2,
+             const T* __restrict__ sv0, const T* __restrict__ sv1, const T* __restrict__ sv2,
              const int* __restrict__ sidx, int64_t M, int ntrans, T kc, T beta) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Cplx<T>* tile = reinterpret_cast<Cplx<T>*>(smem_raw);
   const int sub = xcd_remap(blockIdx.x, gridDim.x);
   if (sub >= *sd.nsub) return;
   const int t0 = blockIdx.y * TC;
-  const int bin = sd.sub_bin[sub];
-  const int b2 = bin % g.nb[2];
-  const int b1 = (bin / g.nb[2]) % g.nb[1];
-  const int b0 = bin / (g.nb[2] * g.nb[1]);
-  const int E0 = g.bs[0] + W, E1 = g.bs[1] + W, E2 = g.bs[2] + W;
-  load_tile<T, W, TC>(tile, fine, bin, g, t0, ntrans, E0, E1, E2, b0 * g.bs[0] - W / 2,
-                      b1 * g.bs[1] - W / 2, b2 * g.bs[2] - W / 2);
+  const int sb = sd.sub_sb[sub];
+  const int s2 = sb % g.nsb[2];
+  const int s1 = (sb / g.nsb[2]) % g.nsb[1];
+  const int s0 = sb / (g.nsb[2] * g.nsb[1]);
+  constexpr int E = kSuperEdge + W;
+  constexpr int ncell = E * E * E;
+  const int o0 = s0 * kSuperEdge - W / 2, o1 = s1 * kSuperEdge - W / 2, o2 = s2 * kSuperEdge - W / 2;
+  const int64_t nfine = (int64_t)g.nf[0] * g.nf[1] * g.nf[2];
+  for (int cidx = threadIdx.x; cidx < ncell; cidx += blockDim.x) {
+    int e2 = cidx % E;
+    int e1 = (cidx / E) % E;
+    int e0 = cidx / (E * E);
+    int m0 = ((o0 + e0) % g.nf[0] + g.nf[0]) % g.nf[0];
+    int m1 = ((o1 + e1) % g.nf[1] + g.nf[1]) % g.nf[1];
+    int m2 = ((o2 + e2) % g.nf[2] + g.nf[2]) % g.nf[2];
+    int64_t off = ((int64_t)m0 * g.nf[1] + m1) * g.nf[2] + m2;
+#pragma unroll
+    for (int tt = 0; tt < TC; ++tt) {
+      int t = t0 + tt;
+      Cplx<T> v = {(T)0, (T)0};
+      if (t < ntrans) v = fine[t * nfine + off];
+      tile[cidx * TC + tt] = v;
+    }
+  }
   __syncthreads();
   const int p0 = sd.sub_pt[sub];
   const int np = sd.sub_n[sub];
   for (int p = p0 + threadIdx.x; p < p0 + np; p += blockDim.x) {
     T w0[W], w1[W], w2[W];
     int l0, l1, l2;
-    stencil_weights<T, W>(su0[p], kc, beta, l0, w0);
-    stencil_weights<T, W>(su1[p], kc, beta, l1, w1);
-    stencil_weights<T, W>(su2[p], kc, beta, l2, w2);
-    l0 = min(max(l0, 0), g.bs[0]);
-    l1 = min(max(l1, 0), g.bs[1]);
-    l2 = min(max(l2, 0), g.bs[2]);
+    stencil_weights<T, W>(sv0[p], kc, beta, l0, w0);
+    stencil_weights<T, W>(sv1[p], kc, beta, l1, w1);
+    stencil_weights<T, W>(sv2[p], kc, beta, l2, w2);
+    l0 = min(max(l0, 0), kSuperEdge);
+    l1 = min(max(l1, 0), kSuperEdge);
+    l2 = min(max(l2, 0), kSuperEdge);
     T ax[TC], ay[TC];
 #pragma unroll
     for (int tt = 0; tt < TC; ++tt) ax[tt] = ay[tt] = (T)0;
@@ -420,15 +445,27 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
       for (int a1 = 0; a1 < W; ++a1) {
         const T w01 = w0[a0] * w1[a1];
-        const Cplx<T>* row = tile + ((int64_t)((l0 + a0) * E1 + (l1 + a1)) * E2 + l2) * TC;
+        const int row = ((l0 + a0) * E + (l1 + a1)) * E + l2;
 #pragma unroll
         for (int a2 = 0; a2 < W; ++a2) {
           const T ww = w01 * w2[a2];
+          if constexpr (TC % 2 == 0 && sizeof(T) == 4) {
+            const float4* cell = reinterpret_cast<const float4*>(tile + (row + a2) * TC);
 #pragma unroll
-          for (int tt = 0; tt < TC; ++tt) {
-            Cplx<T> v = row[a2 * TC + tt];
-            ax[tt] += ww * v.x;
-            ay[tt] += ww * v.y;
+            for (int h = 0; h < TC / 2; ++h) {
+              const float4 v = cell[h];
+              ax[2 * h] += ww * v.x;
+              ay[2 * h] += ww * v.y;
+              ax[2 * h + 1] += ww * v.z;
+              ay[2 * h + 1] += ww * v.w;
+            }
+          } else {
+#pragma unroll
+            for (int tt = 0; tt < TC; ++tt) {
+              const Cplx<T> v = tile[(row + a2) * TC + tt];
+              ax[tt] += ww * v.x;
+              ay[tt] += ww * v.y;
+            }
           }
         }
       }
@@ -442,82 +479,98 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-// Type-1 spreading: fine[t, tap] += phi * c[t, j], accumulated per bin in LDS.
-template <typename T, int W, int TC>
-__global__ void __launch_bounds__(kBlock)
-    k_spread(const Cplx<T>* __restrict__ c, Cplx<T>* __restrict__ fine, SubDesc sd, BinGeom g,
-             const T* __restrict__ su0, const T* __restrict__ su1, const T* __restrict__ su2,
-             const int* __restrict__ sidx, int64_t M, int ntrans, T kc, T beta) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  Cplx<T>* tile = reinterpret_cast<Cplx<T>*>(smem_raw);
-  const int sub = xcd_remap(blockIdx.x, gridDim.x);
-  if (sub >= *sd.nsub) return;
+// Type-1 spreading as an output-driven gather: one 64-lane workgroup owns one
+// 4^3 bin of fine-grid cells (one cell per lane, TC transforms in registers)
+// and walks the points of the 27 neighbouring bins.  For each batch of points
+// the lanes evaluate the 3W kernel weights once and stage them with the
+// point values in LDS; every lane then adds its cell's contribution when the
+// cell lies inside the point's W^3 support.  Each cell is written exactly
+// once (no atomics, deterministic summation order).
+constexpr int kGatherBlock = 64;
+constexpr int kMaxBatch = 16;  // points per staging round
+
+template <typename T, int TC>
+__global__ void __launch_bounds__(kGatherBlock)
+    k_spread_gather(const Cplx<T>* __restrict__ c, Cplx<T>* __restrict__ fine, BinGeom g,
+                    const int* __restrict__ bin_start, const int* __restrict__ bin_count,
+                    const T* __restrict__ sv0, const T* __restrict__ sv1, const T* __restrict__ sv2,
+                    const int* __restrict__ sidx, int64_t M, int ntrans, int W, T kc, T beta) {
+  __shared__ T s_w[kMaxBatch][3][16];
+  __shared__ int s_j0[kMaxBatch][3];
+  __shared__ Cplx<T> s_c[kMaxBatch][TC];
+  const int bin = xcd_remap(blockIdx.x, gridDim.x);
   const int t0 = blockIdx.y * TC;
-  const int bin = sd.sub_bin[sub];
-  const int b2 = bin % g.nb[2];
-  const int b1 = (bin / g.nb[2]) % g.nb[1];
-  const int b0 = bin / (g.nb[2] * g.nb[1]);
-  const int E0 = g.bs[0] + W, E1 = g.bs[1] + W, E2 = g.bs[2] + W;
-  const int ncell = E0 * E1 * E2;
-  T* tf = reinterpret_cast<T*>(tile);
-  for (int i = threadIdx.x; i < ncell * TC * 2; i += blockDim.x) tf[i] = (T)0;
-  __syncthreads();
-  const int p0 = sd.sub_pt[sub];
-  const int np = sd.sub_n[sub];
-  for (int p = p0 + threadIdx.x; p < p0 + np; p += blockDim.x) {
-    T w0[W], w1[W], w2[W];
-    int l0, l1, l2;
-    stencil_weights<T, W>(su0[p], kc, beta, l0, w0);
-    stencil_weights<T, W>(su1[p], kc, beta, l1, w1);
-    stencil_weights<T, W>(su2[p], kc, beta, l2, w2);
-    l0 = min(max(l0, 0), g.bs[0]);
-    l1 = min(max(l1, 0), g.bs[1]);
-    l2 = min(max(l2, 0), g.bs[2]);
-    const int64_t j = sidx[p];
-    Cplx<T> cv[TC];
+  const int lane = threadIdx.x;
+  int b0, b1, b2;
+  bin_coords(g, bin, b0, b1, b2);
+  const int g0 = kBinEdge * b0 + (lane >> 4);
+  const int g1 = kBinEdge * b1 + ((lane >> 2) & 3);
+  const int g2 = kBinEdge * b2 + (lane & 3);
+  T ax[TC], ay[TC];
 #pragma unroll
-    for (int tt = 0; tt < TC; ++tt) {
-      int t = t0 + tt;
-      cv[tt] = t < ntrans ? c[(int64_t)t * M + j] : Cplx<T>{(T)0, (T)0};
-    }
-    for (int a0 = 0; a0 < W; ++a0) {
+  for (int tt = 0; tt < TC; ++tt) ax[tt] = ay[tt] = (T)0;
+  const T halfw = (T)(0.5 * W);
+  // neighbour bins within reach of a W-wide support: ceil((W/2) / 4)
+  const int R = (W + 2 * kBinEdge - 1) / (2 * kBinEdge);
+  for (int d0 = -R; d0 <= R; ++d0) {
+    int n0 = b0 + d0, sh0 = 0;
+    if (n0 < 0) { n0 += g.nb[0]; sh0 = -g.nf[0]; }
+    if (n0 >= g.nb[0]) { n0 -= g.nb[0]; sh0 = g.nf[0]; }
+    for (int d1 = -R; d1 <= R; ++d1) {
+      int n1 = b1 + d1, sh1 = 0;
+      if (n1 < 0) { n1 += g.nb[1]; sh1 = -g.nf[1]; }
+      if (n1 >= g.nb[1]) { n1 -= g.nb[1]; sh1 = g.nf[1]; }
+      for (int d2 = -R; d2 <= R; ++d2) {
+        int n2 = b2 + d2, sh2 = 0;
+        if (n2 < 0) { n2 += g.nb[2]; sh2 = -g.nf[2]; }
+        if (n2 >= g.nb[2]) { n2 -= g.nb[2]; sh2 = g.nf[2]; }
+        const int nbin = bin_index(g, n0, n1, n2);
+        const int start = bin_start[nbin];
+        const int cnt = bin_count[nbin];
+        // this lane's cell relative to the neighbour's super-bin origin (unwrapped frame)
+        const int r0 = g0 - (n0 >> 1) * kSuperEdge - sh0;
+        const int r1 = g1 - (n1 >> 1) * kSuperEdge - sh1;
+        const int r2 = g2 - (n2 >> 1) * kSuperEdge - sh2;
+        for (int p = start; p < start + cnt; p += kMaxBatch) {
+          const int nb = min(kMaxBatch, start + cnt - p);
+          __syncthreads();  // previous batch fully consumed
+          for (int e = lane; e < nb * 3 * W; e += kGatherBlock) {
+            const int q = e / (3 * W), j = e % (3 * W);
+            const int axis = j / W, k = j % W;
+            const T v = axis == 0 ? sv0[p + q] : (axis == 1 ? sv1[p + q] : sv2[p + q]);
+            const int j0 = (int)ceil(v - halfw);
+            s_w[q][axis][k] = es_eval<T>((T)(j0 + k) - v, kc, beta);
+            if (k == 0) s_j0[q][axis] = j0;
+          }
+          for (int e = lane; e < nb * TC; e += kGatherBlock) {
+            const int q = e / TC, tt = e % TC;
+            const int t = t0 + tt;
+            s_c[q][tt] = t < ntrans ? c[(int64_t)t * M + sidx[p + q]] : Cplx<T>{(T)0, (T)0};
+          }
+          __syncthreads();
+          for (int q = 0; q < nb; ++q) {
+            const int x0 = r0 - s_j0[q][0], x1 = r1 - s_j0[q][1], x2 = r2 - s_j0[q][2];
+            const bool in = x0 >= 0 && x0 < W && x1 >= 0 && x1 < W && x2 >= 0 && x2 < W;
+            const T w = in ? s_w[q][0][min(max(x0, 0), W - 1)] * s_w[q][1][min(max(x1, 0), W - 1)] *
+                                 s_w[q][2][min(max(x2, 0), W - 1)]
+                           : (T)0;
 #pragma unroll
-      for (int a1 = 0; a1 < W; ++a1) {
-        const T w01 = w0[a0] * w1[a1];
-        T* row = tf + ((int64_t)((l0 + a0) * E1 + (l1 + a1)) * E2 + l2) * TC * 2;
-#pragma unroll
-        for (int a2 = 0; a2 < W; ++a2) {
-          const T ww = w01 * w2[a2];
-#pragma unroll
-          for (int tt = 0; tt < TC; ++tt) {
-            atomicAdd(row + (a2 * TC + tt) * 2, ww * cv[tt].x);
-            atomicAdd(row + (a2 * TC + tt) * 2 + 1, ww * cv[tt].y);
+            for (int tt = 0; tt < TC; ++tt) {
+              const Cplx<T> cv = s_c[q][tt];
+              ax[tt] += w * cv.x;
+              ay[tt] += w * cv.y;
+            }
           }
         }
       }
     }
   }
-  __syncthreads();
   const int64_t nfine = (int64_t)g.nf[0] * g.nf[1] * g.nf[2];
-  const int o0 = b0 * g.bs[0] - W / 2, o1 = b1 * g.bs[1] - W / 2, o2 = b2 * g.bs[2] - W / 2;
-  T* finef = reinterpret_cast<T*>(fine);
-  for (int cidx = threadIdx.x; cidx < ncell; cidx += blockDim.x) {
-    int e2 = cidx % E2;
-    int e1 = (cidx / E2) % E1;
-    int e0 = cidx / (E1 * E2);
-    int m0 = ((o0 + e0) % g.nf[0] + g.nf[0]) % g.nf[0];
-    int m1 = ((o1 + e1) % g.nf[1] + g.nf[1]) % g.nf[1];
-    int m2 = ((o2 + e2) % g.nf[2] + g.nf[2]) % g.nf[2];
-    int64_t off = ((int64_t)m0 * g.nf[1] + m1) * g.nf[2] + m2;
+  const int64_t off = ((int64_t)g0 * g.nf[1] + g1) * g.nf[2] + g2;
 #pragma unroll
-    for (int tt = 0; tt < TC; ++tt) {
-      int t = t0 + tt;
-      if (t >= ntrans) break;
-      T vx = tf[(cidx * TC + tt) * 2];
-      T vy = tf[(cidx * TC + tt) * 2 + 1];
-      if (vx != (T)0) atomicAdd(finef + 2 * (t * nfine + off), vx);
-      if (vy != (T)0) atomicAdd(finef + 2 * (t * nfine + off) + 1, vy);
-    }
+  for (int tt = 0; tt < TC; ++tt) {
+    const int t = t0 + tt;
+    if (t < ntrans) fine[t * nfine + off] = Cplx<T>{ax[tt], ay[tt]};
   }
 }
 
@@ -533,31 +586,30 @@ struct Plan {
   int64_t nf[3] = {0, 0, 0};
   int64_t ntrans = 1;
   int w = 7;
-  int tc = 2;
+  int tc = 2;  // transforms per interpolation workgroup
   double beta = 0, eps = 1e-6;
   BinGeom geom;
-  int64_t nbins = 0;
+  int64_t nbins = 0, nsuper = 0;
   void* fine = nullptr;
   void* invp[3] = {nullptr, nullptr, nullptr};
   int64_t M = 0, M_cap = 0;
-  int64_t sub_cap = 0;
+  int64_t sub_cap = 0, sub_alloc = 0;
   int* bin_count = nullptr;
   int* bin_start = nullptr;
   int* sub_start = nullptr;
-  int* sub_bin = nullptr;
+  int* sub_sb = nullptr;
   int* sub_pt = nullptr;
   int* sub_n = nullptr;
   int* nsub = nullptr;
   int* pt_bin = nullptr;
   int* pt_pos = nullptr;
   int* sidx = nullptr;
-  void* su[3] = {nullptr, nullptr, nullptr};
+  void* sv[3] = {nullptr, nullptr, nullptr};
   rocfft_plan fft_fwd = nullptr, fft_bwd = nullptr;
   rocfft_execution_info info = nullptr;
   void* fft_work = nullptr;
   size_t work_size = 0;
   bool pts_set = false;
-  size_t tile_bytes = 0;
 };
 
 static std::once_flag g_rocfft_once;
@@ -566,68 +618,54 @@ static int free_points(Plan* p) {
   hipFree(p->pt_bin);
   hipFree(p->pt_pos);
   hipFree(p->sidx);
-  for (int k = 0; k < 3; ++k) hipFree(p->su[k]);
-  hipFree(p->sub_bin);
+  for (int k = 0; k < 3; ++k) hipFree(p->sv[k]);
+  hipFree(p->sub_sb);
   hipFree(p->sub_pt);
   hipFree(p->sub_n);
   p->pt_bin = p->pt_pos = p->sidx = nullptr;
-  p->sub_bin = p->sub_pt = p->sub_n = nullptr;
-  for (int k = 0; k < 3; ++k) p->su[k] = nullptr;
+  p->sub_sb = p->sub_pt = p->sub_n = nullptr;
+  for (int k = 0; k < 3; ++k) p->sv[k] = nullptr;
   p->M_cap = 0;
-  p->sub_cap = 0;
+  p->sub_alloc = 0;
   return 0;
 }
 
-#define COV3D_FFT(call)                                               \
-  do {                                                                \
-    rocfft_status s_ = (call);                                        \
-    if (s_ != rocfft_status_success) {                                \
+#define COV3D_FFT(call)                                                  \
+  do {                                                                   \
+    rocfft_status s_ = (call);                                           \
+    if (s_ != rocfft_status_success) {                                   \
       ::cov3d::set_error("%s failed: rocfft status %d", #call, (int)s_); \
-      return COV3D_ERR_FFT;                                           \
-    }                                                                 \
+      return COV3D_ERR_FFT;                                              \
+    }                                                                    \
   } while (0)
 
 template <typename T, int W, int TC>
-static int launch_interp_spread(Plan* p, bool spread, const void* c_in, void* c_out, int ntrans_run,
-                                hipStream_t st) {
-  const size_t smem = (size_t)(p->geom.bs[0] + W) * (p->geom.bs[1] + W) * (p->geom.bs[2] + W) * TC *
-                      sizeof(Cplx<T>);
-  static bool attr_done[2] = {false, false};
-  if (smem > 65536 && !attr_done[spread ? 1 : 0]) {
-    if (spread)
-      COV3D_HIP(hipFuncSetAttribute((const void*)k_spread<T, W, TC>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    else
-      COV3D_HIP(hipFuncSetAttribute((const void*)k_interp<T, W, TC>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    attr_done[spread ? 1 : 0] = true;
-  }
+static int launch_interp(Plan* p, void* c_out, int ntrans_run, hipStream_t st) {
+  const size_t smem = (size_t)(kSuperEdge + W) * (kSuperEdge + W) * (kSuperEdge + W) * TC * sizeof(Cplx<T>);
   COV3D_CHECK_ARG(smem <= 160 * 1024, "NUFFT tile of %zu bytes exceeds LDS", smem);
-  SubDesc sd{p->sub_bin, p->sub_pt, p->sub_n, p->nsub};
+  static bool attr_done = false;
+  if (smem > 65536 && !attr_done) {
+    COV3D_HIP(hipFuncSetAttribute((const void*)k_interp<T, W, TC>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    attr_done = true;
+  }
+  SubDesc sd{p->sub_sb, p->sub_pt, p->sub_n, p->nsub};
   dim3 grid((unsigned)p->sub_cap, (unsigned)ceil_div(ntrans_run, TC));
   const T kc = (T)(4.0 / ((double)W * W));
-  const T beta = (T)p->beta;
-  ScopedTiming tm(spread ? kTSpread : kTInterp, st);
-  if (spread) {
-    hipLaunchKernelGGL((k_spread<T, W, TC>), grid, dim3(kBlock), smem, st,
-                       (const Cplx<T>*)c_in, (Cplx<T>*)p->fine, sd, p->geom, (const T*)p->su[0],
-                       (const T*)p->su[1], (const T*)p->su[2], p->sidx, p->M, ntrans_run, kc, beta);
-  } else {
-    hipLaunchKernelGGL((k_interp<T, W, TC>), grid, dim3(kBlock), smem, st,
-                       (const Cplx<T>*)p->fine, (Cplx<T>*)c_out, sd, p->geom, (const T*)p->su[0],
-                       (const T*)p->su[1], (const T*)p->su[2], p->sidx, p->M, ntrans_run, kc, beta);
-  }
+  ScopedTiming tm(kTInterp, st);
+  hipLaunchKernelGGL((k_interp<T, W, TC>), grid, dim3(kBlock), smem, st, (const Cplx<T>*)p->fine,
+                     (Cplx<T>*)c_out, sd, p->geom, (const T*)p->sv[0], (const T*)p->sv[1],
+                     (const T*)p->sv[2], p->sidx, p->M, ntrans_run, kc, (T)p->beta);
   COV3D_LAUNCH_CHECK();
   return 0;
 }
 
 template <typename T, int TC>
-static int dispatch_w(Plan* p, bool spread, const void* c_in, void* c_out, int ntrans_run,
-                      hipStream_t st) {
+static int dispatch_interp_w(Plan* p, void* c_out, int ntrans_run, hipStream_t st) {
   switch (p->w) {
 #define COV3D_W(WW) \
   case WW:          \
-    return launch_interp_spread<T, WW, TC>(p, spread, c_in, c_out, ntrans_run, st);
+    return launch_interp<T, WW, TC>(p, c_out, ntrans_run, st);
     COV3D_W(2) COV3D_W(3) COV3D_W(4) COV3D_W(5) COV3D_W(6) COV3D_W(7) COV3D_W(8) COV3D_W(9)
     COV3D_W(10) COV3D_W(11) COV3D_W(12) COV3D_W(13) COV3D_W(14) COV3D_W(15) COV3D_W(16)
 #undef COV3D_W
@@ -637,13 +675,39 @@ static int dispatch_w(Plan* p, bool spread, const void* c_in, void* c_out, int n
   }
 }
 
-static int run_interp_spread(Plan* p, bool spread, const void* c_in, void* c_out, int ntrans_run,
-                             hipStream_t st) {
+static int run_interp(Plan* p, void* c_out, int ntrans_run, hipStream_t st) {
   if (p->M == 0) return 0;
-  if (p->dtype == 1) return dispatch_w<double, 1>(p, spread, c_in, c_out, ntrans_run, st);
-  if (p->tc == 4) return dispatch_w<float, 4>(p, spread, c_in, c_out, ntrans_run, st);
-  if (p->tc == 1) return dispatch_w<float, 1>(p, spread, c_in, c_out, ntrans_run, st);
-  return dispatch_w<float, 2>(p, spread, c_in, c_out, ntrans_run, st);
+  if (p->dtype == 1) return dispatch_interp_w<double, 1>(p, c_out, ntrans_run, st);
+  if (p->tc == 4) return dispatch_interp_w<float, 4>(p, c_out, ntrans_run, st);
+  if (p->tc == 1) return dispatch_interp_w<float, 1>(p, c_out, ntrans_run, st);
+  return dispatch_interp_w<float, 2>(p, c_out, ntrans_run, st);
+}
+
+template <typename T, int TC>
+static int launch_spread(Plan* p, const void* c_in, int ntrans_run, hipStream_t st) {
+  dim3 grid((unsigned)p->nbins, (unsigned)ceil_div(ntrans_run, TC));
+  const T kc = (T)(4.0 / ((double)p->w * p->w));
+  ScopedTiming tm(kTSpread, st);
+  hipLaunchKernelGGL((k_spread_gather<T, TC>), grid, dim3(kGatherBlock), 0, st, (const Cplx<T>*)c_in,
+                     (Cplx<T>*)p->fine, p->geom, p->bin_start, p->bin_count, (const T*)p->sv[0],
+                     (const T*)p->sv[1], (const T*)p->sv[2], p->sidx, p->M, ntrans_run, p->w, kc,
+                     (T)p->beta);
+  COV3D_LAUNCH_CHECK();
+  return 0;
+}
+
+// Transforms per gather workgroup: at most 20 in registers, split evenly.
+template <typename T>
+static int run_spread(Plan* p, const void* c_in, int ntrans_run, hipStream_t st) {
+  const int chunks = (int)ceil_div(ntrans_run, 20);
+  const int need = (int)ceil_div(ntrans_run, chunks);
+  if (need <= 1) return launch_spread<T, 1>(p, c_in, ntrans_run, st);
+  if (need <= 2) return launch_spread<T, 2>(p, c_in, ntrans_run, st);
+  if (need <= 4) return launch_spread<T, 4>(p, c_in, ntrans_run, st);
+  if (need <= 8) return launch_spread<T, 8>(p, c_in, ntrans_run, st);
+  if (need <= 10) return launch_spread<T, 10>(p, c_in, ntrans_run, st);
+  if (need <= 16) return launch_spread<T, 16>(p, c_in, ntrans_run, st);
+  return launch_spread<T, 20>(p, c_in, ntrans_run, st);
 }
 
 static int env_int(const char* name, int dflt) {
@@ -666,35 +730,38 @@ static int exec_type2(Plan* p, const void* f, bool f_real, void* c, int64_t nrun
   const int64_t total = nfine * p->ntrans;
   dim3 g1((unsigned)ceil_div(total, kBlock));
   {
-  ScopedTiming tm(kTDeconvPad, st);
-  if (f_real)
-    hipLaunchKernelGGL((k_deconv_pad<T, true>), g1, dim3(kBlock), 0, st, (const T*)f,
-                       (Cplx<T>*)p->fine, p->ntrans, nrun, (int)p->N[0], (int)p->N[1],
-                       (int)p->N[2], (int)p->nf[0], (int)p->nf[1], (int)p->nf[2],
-                       (const T*)p->invp[0], (const T*)p->invp[1], (const T*)p->invp[2]);
-  else
-    hipLaunchKernelGGL((k_deconv_pad<T, false>), g1, dim3(kBlock), 0, st, (const T*)f,
-                       (Cplx<T>*)p->fine, p->ntrans, nrun, (int)p->N[0], (int)p->N[1],
-                       (int)p->N[2], (int)p->nf[0], (int)p->nf[1], (int)p->nf[2],
-                       (const T*)p->invp[0], (const T*)p->invp[1], (const T*)p->invp[2]);
+    ScopedTiming tm(kTDeconvPad, st);
+    if (f_real)
+      hipLaunchKernelGGL((k_deconv_pad<T, true>), g1, dim3(kBlock), 0, st, (const T*)f,
+                         (Cplx<T>*)p->fine, p->ntrans, nrun, (int)p->N[0], (int)p->N[1],
+                         (int)p->N[2], (int)p->nf[0], (int)p->nf[1], (int)p->nf[2],
+                         (const T*)p->invp[0], (const T*)p->invp[1], (const T*)p->invp[2]);
+    else
+      hipLaunchKernelGGL((k_deconv_pad<T, false>), g1, dim3(kBlock), 0, st, (const T*)f,
+                         (Cplx<T>*)p->fine, p->ntrans, nrun, (int)p->N[0], (int)p->N[1],
+                         (int)p->N[2], (int)p->nf[0], (int)p->nf[1], (int)p->nf[2],
+                         (const T*)p->invp[0], (const T*)p->invp[1], (const T*)p->invp[2]);
   }
   COV3D_LAUNCH_CHECK();
   int rc = exec_fft(p, p->iflag2 < 0, st);
   if (rc) return rc;
-  return run_interp_spread(p, false, nullptr, c, (int)nrun, st);
+  return run_interp(p, c, (int)nrun, st);
 }
 
 template <typename T>
 static int exec_type1(Plan* p, const void* c, void* f, bool f_real, int64_t nrun, hipStream_t st) {
-  const int64_t nfine = p->nf[0] * p->nf[1] * p->nf[2];
-  const int64_t total = nfine * p->ntrans * 2;
-  hipLaunchKernelGGL((k_fill_zero<T>), dim3((unsigned)std::min<int64_t>(ceil_div(total, kBlock), 65536)),
-                     dim3(kBlock), 0, st, (T*)p->fine, total);
-  COV3D_LAUNCH_CHECK();
-  int rc = run_interp_spread(p, true, c, nullptr, (int)nrun, st);
-  if (rc) return rc;
+  if (p->M == 0) {
+    // no points: the spread grid is all zeros
+    const int64_t total = p->nf[0] * p->nf[1] * p->nf[2] * p->ntrans * 2;
+    hipLaunchKernelGGL((k_fill_zero<T>), dim3((unsigned)std::min<int64_t>(ceil_div(total, kBlock), 65536)),
+                       dim3(kBlock), 0, st, (T*)p->fine, total);
+    COV3D_LAUNCH_CHECK();
+  } else {
+    int rc = run_spread<T>(p, c, (int)nrun, st);
+    if (rc) return rc;
+  }
   // type 1 uses the opposite sign of type 2
-  rc = exec_fft(p, !(p->iflag2 < 0), st);
+  int rc = exec_fft(p, !(p->iflag2 < 0), st);
   if (rc) return rc;
   const int64_t nm = p->N[0] * p->N[1] * p->N[2];
   dim3 g1((unsigned)ceil_div(nm * nrun, kBlock));
@@ -715,45 +782,47 @@ static int exec_type1(Plan* p, const void* c, void* f, bool f_real, int64_t nrun
 
 template <typename T>
 static int setpts_impl(Plan* p, int64_t M, const T* x, const T* y, const T* z, hipStream_t st) {
-  if (M > p->M_cap) {
+  // subproblems: at most one per non-empty super-bin plus one per kMaxSub points
+  const int64_t cap = ceil_div(std::min<int64_t>(p->nsuper, M) + ceil_div(M, kMaxSub), 8) * 8;
+  if (M > p->M_cap || cap > p->sub_alloc) {
     free_points(p);
-    COV3D_HIP(hipMalloc(&p->pt_bin, M * sizeof(int)));
-    COV3D_HIP(hipMalloc(&p->pt_pos, M * sizeof(int)));
-    COV3D_HIP(hipMalloc(&p->sidx, M * sizeof(int)));
-    for (int k = 0; k < 3; ++k) COV3D_HIP(hipMalloc(&p->su[k], M * sizeof(T)));
-    int64_t cap = std::min<int64_t>(p->nbins, M) + ceil_div(M, kMaxSub);
-    cap = ceil_div(cap, 8) * 8;
-    COV3D_HIP(hipMalloc(&p->sub_bin, cap * sizeof(int)));
-    COV3D_HIP(hipMalloc(&p->sub_pt, cap * sizeof(int)));
-    COV3D_HIP(hipMalloc(&p->sub_n, cap * sizeof(int)));
+    const int64_t Ma = M > 0 ? M : 1;
+    COV3D_HIP(hipMalloc(&p->pt_bin, Ma * sizeof(int)));
+    COV3D_HIP(hipMalloc(&p->pt_pos, Ma * sizeof(int)));
+    COV3D_HIP(hipMalloc(&p->sidx, Ma * sizeof(int)));
+    for (int k = 0; k < 3; ++k) COV3D_HIP(hipMalloc(&p->sv[k], Ma * sizeof(T)));
+    const int64_t ca = cap > 8 ? cap : 8;
+    COV3D_HIP(hipMalloc(&p->sub_sb, ca * sizeof(int)));
+    COV3D_HIP(hipMalloc(&p->sub_pt, ca * sizeof(int)));
+    COV3D_HIP(hipMalloc(&p->sub_n, ca * sizeof(int)));
     p->M_cap = M;
-    p->sub_cap = cap;
+    p->sub_alloc = ca;
   }
   p->M = M;
-  if (M == 0) {
-    p->pts_set = true;
-    return 0;
-  }
-  int64_t cap = std::min<int64_t>(p->nbins, M) + ceil_div(M, kMaxSub);
-  p->sub_cap = ceil_div(cap, 8) * 8;
+  p->sub_cap = cap;
   ScopedTiming tm(kTSort, st);
   hipLaunchKernelGGL(k_fill_int, dim3((unsigned)ceil_div(p->nbins, kBlock)), dim3(kBlock), 0, st,
                      p->bin_count, p->nbins, 0);
   COV3D_LAUNCH_CHECK();
-  dim3 gm((unsigned)ceil_div(M, kBlock));
-  hipLaunchKernelGGL((k_bin_count<T>), gm, dim3(kBlock), 0, st, x, y, z, M, p->geom, p->bin_count,
-                     p->pt_bin, p->pt_pos);
-  COV3D_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, st, p->bin_count, p->nbins, p->bin_start,
+  if (M > 0) {
+    dim3 gm((unsigned)ceil_div(M, kBlock));
+    hipLaunchKernelGGL((k_bin_count<T>), gm, dim3(kBlock), 0, st, x, y, z, M, p->geom, p->bin_count,
+                       p->pt_bin, p->pt_pos);
+    COV3D_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, st, p->bin_count, p->nsuper, p->bin_start,
                      p->sub_start, p->nsub);
   COV3D_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_sub_fill, dim3((unsigned)ceil_div(p->nbins, kBlock)), dim3(kBlock), 0, st,
-                     p->bin_count, p->bin_start, p->sub_start, p->nbins, p->sub_bin, p->sub_pt,
-                     p->sub_n);
-  COV3D_LAUNCH_CHECK();
-  hipLaunchKernelGGL((k_bin_scatter<T>), gm, dim3(kBlock), 0, st, x, y, z, M, p->geom, p->pt_bin,
-                     p->pt_pos, p->bin_start, p->sidx, (T*)p->su[0], (T*)p->su[1], (T*)p->su[2]);
-  COV3D_LAUNCH_CHECK();
+  if (M > 0) {
+    hipLaunchKernelGGL(k_sub_fill, dim3((unsigned)ceil_div(p->nsuper, kBlock)), dim3(kBlock), 0, st,
+                       p->bin_count, p->bin_start, p->sub_start, p->nsuper, p->sub_sb, p->sub_pt,
+                       p->sub_n);
+    COV3D_LAUNCH_CHECK();
+    dim3 gm((unsigned)ceil_div(M, kBlock));
+    hipLaunchKernelGGL((k_bin_scatter<T>), gm, dim3(kBlock), 0, st, x, y, z, M, p->geom, p->pt_bin,
+                       p->pt_pos, p->bin_start, p->sidx, (T*)p->sv[0], (T*)p->sv[1], (T*)p->sv[2]);
+    COV3D_LAUNCH_CHECK();
+  }
   p->pts_set = true;
   return 0;
 }
@@ -775,6 +844,13 @@ static int destroy_plan(Plan* p) {
   return 0;
 }
 
+// Smallest integer >= n that is a multiple of 8 with only 2, 3, 5 as prime factors.
+static int64_t next_fine_size(int64_t n) {
+  int64_t m = next235even(n);
+  while (m % kSuperEdge) m = next235even(m + 2);
+  return m;
+}
+
 static int make_plan(int type, const int64_t* n, int iflag, int64_t ntrans, double eps, int dtype,
                      Plan** out) {
   Plan* p = new Plan();
@@ -786,24 +862,18 @@ static int make_plan(int type, const int64_t* n, int iflag, int64_t ntrans, doub
   p->eps = eps < emin ? emin : eps;
   p->w = es_width(p->eps);
   p->beta = es_beta(p->w);
-  int bs_env = env_int("COV3D_NUFFT_BIN", 0);
-  int tc_env = env_int("COV3D_NUFFT_TC", 2);
+  const int tc_env = env_int("COV3D_NUFFT_TC", 2);
   p->tc = dtype == 1 ? 1 : (tc_env == 1 || tc_env == 4 ? tc_env : 2);
   const size_t csize = dtype == 1 ? 16 : 8;
   for (int k = 0; k < 3; ++k) {
     p->N[k] = n[k];
-    p->nf[k] = next235even(std::max<int64_t>(2 * n[k], 2 * p->w));
-  }
-  // Bin edge: largest in [2, 8] whose haloed tile fits 64 KB of LDS (or the override).
-  int bs = 8;
-  while (bs > 2 && (size_t)(bs + p->w) * (bs + p->w) * (bs + p->w) * p->tc * csize > 65536) --bs;
-  if (bs_env > 0) bs = bs_env;
-  for (int k = 0; k < 3; ++k) {
+    p->nf[k] = next_fine_size(std::max<int64_t>(2 * n[k], 2 * p->w));
     p->geom.nf[k] = (int)p->nf[k];
-    p->geom.bs[k] = bs;
-    p->geom.nb[k] = (int)ceil_div(p->nf[k], bs);
+    p->geom.nb[k] = (int)(p->nf[k] / kBinEdge);
+    p->geom.nsb[k] = (int)(p->nf[k] / kSuperEdge);
   }
-  p->nbins = (int64_t)p->geom.nb[0] * p->geom.nb[1] * p->geom.nb[2];
+  p->nsuper = (int64_t)p->geom.nsb[0] * p->geom.nsb[1] * p->geom.nsb[2];
+  p->nbins = 8 * p->nsuper;
   const int64_t nfine = p->nf[0] * p->nf[1] * p->nf[2];
   auto fail = [&](int rc) {
     destroy_plan(p);
@@ -827,7 +897,7 @@ static int make_plan(int type, const int64_t* n, int iflag, int64_t ntrans, doub
   }
   if (hipMalloc(&p->bin_count, p->nbins * sizeof(int)) != hipSuccess ||
       hipMalloc(&p->bin_start, p->nbins * sizeof(int)) != hipSuccess ||
-      hipMalloc(&p->sub_start, p->nbins * sizeof(int)) != hipSuccess ||
+      hipMalloc(&p->sub_start, p->nsuper * sizeof(int)) != hipSuccess ||
       hipMalloc(&p->nsub, sizeof(int)) != hipSuccess) {
     set_error("hipMalloc of bin tables failed");
     return fail(COV3D_ERR_HIP);

@@ -28,15 +28,15 @@ def test_slice_forward_adjoint_vs_oracle(dev, mode, L, us):
     out = plan.execute_forward(torch.tensor(vol, dtype=torch.complex64, device=dev))
     grid = plan._grid_flat.double().cpu().numpy()
     ref = o.slice_forward(vol, grid, mode).reshape(3, 5, L, L)
-    assert rel_l2(out.cpu().numpy(), ref) < 1e-6
+    assert rel_l2(out.cpu().numpy(), ref) < 5e-6  # FP32 weights and sums vs the FP64 oracle
     sig = rng.standard_normal((5, L, L)) + 1j * rng.standard_normal((5, L, L))
     adj = plan.execute_adjoint(torch.tensor(sig, dtype=torch.complex64, device=dev))
     ref_adj = o.slice_adjoint(sig.reshape(1, -1), grid, (vL,) * 3, mode)
     assert adj.shape == (1, vL, vL, vL)
-    assert rel_l2(adj.cpu().numpy(), ref_adj) < 1e-6
+    assert rel_l2(adj.cpu().numpy(), ref_adj) < 5e-6
     un = plan.execute_adjoint_unaggregated(torch.tensor(sig, dtype=torch.complex64, device=dev))
     assert un.shape == (5, vL, vL, vL)
-    assert rel_l2(un.sum(0).cpu().numpy(), ref_adj[0]) < 1e-6
+    assert rel_l2(un.sum(0).cpu().numpy(), ref_adj[0]) < 5e-6
 
 
 def test_vol_forward_im_backward_adjointness_discretized(dev):
