@@ -33,8 +33,10 @@ _SIGNATURES = {
     "cov3d_nufft_info": [c_vp, ctypes.POINTER(c_i64), ctypes.POINTER(c_int), ctypes.POINTER(c_dbl)],
     "cov3d_slice_forward": [c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_i64, c_i64, c_int, c_int, c_vp],
     "cov3d_slice_adjoint": [c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_i64, c_i64, c_int, c_int, c_i64, c_int, c_vp],
-    "cov3d_ml_forward": [c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_dbl, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
-    "cov3d_ml_grad_images": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_dbl, c_dbl, c_vp],
+    "cov3d_ml_forward": [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_i64, c_dbl, c_int, c_int, c_vp, c_vp,
+                         c_vp, c_vp, c_vp, c_vp],
+    "cov3d_ml_grad_images": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_i64, c_i64, c_i64, c_dbl, c_dbl,
+                             c_vp],
     "cov3d_pts_rot": [c_vp, c_vp, c_i64, c_i64, c_vp],
     "cov3d_ctf_eval": [c_vp, c_vp, c_i64, c_i64, c_vp],
     "cov3d_vdot_c64_f64": [c_vp, c_vp, c_i64, c_vp, c_vp],
@@ -97,7 +99,7 @@ def ptr(t: torch.Tensor | None) -> int | None:
     return None if t is None else t.data_ptr()
 
 
-TIMING_KINDS = {"interp": 0, "spread": 1, "deconv_pad": 2, "deconv_crop": 3, "fft": 4, "sort": 5}
+TIMING_KINDS = {"interp": 0, "spread": 1, "deconv_pad": 2, "deconv_crop": 3, "fft": 4, "sort": 5, "permute": 6}
 
 
 def timing_enable(on: bool) -> None:

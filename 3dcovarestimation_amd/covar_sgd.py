@@ -96,7 +96,7 @@ def _cost_ml_autograd(vols, images, nufft_plans, filters, noise_var, apply_mean_
 
 def cost_maximum_liklihood(vols, images, nufft_plans, filters, noise_var, reg_scale=0, fourier_reg=None,
                            apply_mean_const_term=False, mean_aggregate=True, form=None,
-                           precision=ml_ops.PRECISION_BF16X3):
+                           precision=ml_ops.PRECISION_BF16X3, half_plane=True):
     """Maximum-likelihood objective, spatial domain (covar_sgd.py:1238-1286)."""
     form = form or _DEFAULT_FORM
     fused_ok = (
@@ -107,7 +107,7 @@ def cost_maximum_liklihood(vols, images, nufft_plans, filters, noise_var, reg_sc
         L = vols.shape[-1]
         yhat = centered_fft2(images.reshape(-1, L, L).to(torch.float32))
         val = ml_ops.MLCostNufft.apply(vols, yhat, nufft_plans, filters, float(noise_var),
-                                       bool(apply_mean_const_term), int(precision))
+                                       bool(apply_mean_const_term), int(precision), bool(half_plane))
     else:
         val = _cost_ml_autograd(vols, images, nufft_plans, filters, noise_var, apply_mean_const_term, mean_aggregate)
     if fourier_reg is not None and reg_scale != 0:

@@ -44,19 +44,39 @@ __device__ __forceinline__ float pixel_scale(const float* __restrict__ ctf, int6
   return s;
 }
 
-// Row value (complex) of pixel k for Gram row m of image i.
+// Pixel of entry k of image i's point list: the identity (full plane) or a
+// map onto the centred L x L grid (Hermitian half plane); mult is the number
+// of grid pixels the entry stands for (2 for a +-k pair, 1 for the centre).
+__device__ __forceinline__ int64_t entry_pixel(const int* __restrict__ pix_map, int64_t k, int L,
+                                               float& mult) {
+  if (!pix_map) {
+    mult = 1.f;
+    return k;
+  }
+  const int64_t px = pix_map[k];
+  mult = px == (int64_t)(L / 2) * L + L / 2 ? 1.f : 2.f;
+  return px;
+}
+
+// Row value (complex) of entry k for Gram row m of image i, scaled by
+// sqrt(mult) so that the plain Gram of the rows is the full-plane sum.
 __device__ __forceinline__ float2 gram_row_value(const float2* __restrict__ c,
                                                  const float2* __restrict__ yhat,
-                                                 const float* __restrict__ ctf, int64_t i, int m,
-                                                 int r, int64_t B, int64_t d, int64_t k, int L,
+                                                 const float* __restrict__ ctf,
+                                                 const int* __restrict__ pix_map, int64_t i, int m,
+                                                 int r, int64_t B, int64_t np, int64_t k, int L,
                                                  float inv_l, float inv_l2) {
-  if (k >= d || m > r) return make_float2(0.f, 0.f);
+  if (k >= np || m > r) return make_float2(0.f, 0.f);
+  float mult;
+  const int64_t px = entry_pixel(pix_map, k, L, mult);
+  const float sm = mult == 1.f ? 1.f : 1.41421356237309515f;
+  const int64_t d = (int64_t)L * L;
   if (m == r) {
-    float2 v = yhat[i * d + k];
-    return make_float2(v.x * inv_l, v.y * inv_l);
+    float2 v = yhat[i * d + px];
+    return make_float2(v.x * inv_l * sm, v.y * inv_l * sm);
   }
-  float s = pixel_scale(ctf, i, k, L, inv_l2);
-  float2 v = c[((int64_t)m * B + i) * d + k];
+  float s = pixel_scale(ctf, i, px, L, inv_l2) * sm;
+  float2 v = c[((int64_t)m * B + i) * np + k];
   return make_float2(v.x * s, v.y * s);
 }
 
@@ -96,7 +116,8 @@ __device__ void chol_solve(const double* A, double* x, int r) {
 template <int PREC>
 __global__ void __launch_bounds__(kMlBlock)
     k_ml_forward(const float2* __restrict__ c, const float* __restrict__ ctf,
-                 const float2* __restrict__ yhat, const float* __restrict__ lam_inv, int64_t B,
+                 const float2* __restrict__ yhat, const float* __restrict__ lam_inv,
+                 const int* __restrict__ pix_map, int64_t np, int64_t B,
                  int r, int L, float noise_var, int mean_const, float* __restrict__ cost,
                  float* __restrict__ z_out, float* __restrict__ m_out,
                  float* __restrict__ minv_out, float* __restrict__ b_out) {
@@ -120,13 +141,13 @@ __global__ void __launch_bounds__(kMlBlock)
     f32x16 acc;
 #pragma unroll
     for (int v = 0; v < 16; ++v) acc[v] = 0.f;
-    const int64_t nchunk = (d + 31) / 32;
+    const int64_t nchunk = (np + 31) / 32;
     for (int64_t ch = wave; ch < nchunk; ch += 4) {
       const int64_t k0 = ch * 32 + 16 * h;
       float2 vals[16];
 #pragma unroll
       for (int q = 0; q < 16; ++q)
-        vals[q] = gram_row_value(c, yhat, ctf, i, m, r, B, d, k0 + q, L, inv_l, inv_l2);
+        vals[q] = gram_row_value(c, yhat, ctf, pix_map, i, m, r, B, np, k0 + q, L, inv_l, inv_l2);
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         u16x8 hi, lo;
@@ -173,10 +194,10 @@ __global__ void __launch_bounds__(kMlBlock)
         ps[k] = t + q;
       }
     }
-    for (int64_t k0 = 0; k0 < d; k0 += 64) {
+    for (int64_t k0 = 0; k0 < np; k0 += 64) {
       for (int e = tid; e < (r + 1) * 64; e += blockDim.x) {
         int m = e / 64, kk = e % 64;
-        s_rows[m][kk] = gram_row_value(c, yhat, ctf, i, m, r, B, d, k0 + kk, L, inv_l, inv_l2);
+        s_rows[m][kk] = gram_row_value(c, yhat, ctf, pix_map, i, m, r, B, np, k0 + kk, L, inv_l, inv_l2);
       }
       __syncthreads();
 #pragma unroll
@@ -252,8 +273,8 @@ template <int RM>
 __global__ void __launch_bounds__(kMlBlock)
     k_ml_grad(const float2* __restrict__ c, const float* __restrict__ ctf,
               const float2* __restrict__ yhat, const float* __restrict__ z,
-              const float* __restrict__ minv, float2* __restrict__ g, int64_t B, int r, int L,
-              float scale) {
+              const float* __restrict__ minv, const int* __restrict__ pix_map, int64_t np,
+              float2* __restrict__ g, int64_t B, int r, int L, float scale) {
   __shared__ float s_minv[RM * RM];
   __shared__ float s_z[RM];
   const int64_t i = blockIdx.y;
@@ -265,23 +286,25 @@ __global__ void __launch_bounds__(kMlBlock)
   if (threadIdx.x < RM) s_z[threadIdx.x] = (int)threadIdx.x < r ? z[i * r + threadIdx.x] : 0.f;
   __syncthreads();
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= d) return;
+  if (k >= np) return;
+  float mult;
+  const int64_t px = entry_pixel(pix_map, k, L, mult);
   const float inv_l = 1.f / (float)L;
-  const float ps = pixel_scale(ctf, i, k, L, inv_l * inv_l);
+  const float ps = pixel_scale(ctf, i, px, L, inv_l * inv_l);
   float qx[RM], qy[RM];
   float ex = 0.f, ey = 0.f;
 #pragma unroll
   for (int s = 0; s < RM; ++s) {
-    float2 v = s < r ? c[((int64_t)s * B + i) * d + k] : make_float2(0.f, 0.f);
+    float2 v = s < r ? c[((int64_t)s * B + i) * np + k] : make_float2(0.f, 0.f);
     qx[s] = v.x * ps;
     qy[s] = v.y * ps;
     ex += s_z[s] * qx[s];
     ey += s_z[s] * qy[s];
   }
-  float2 yv = yhat[i * d + k];
+  float2 yv = yhat[i * d + px];
   ex -= yv.x * inv_l;
   ey -= yv.y * inv_l;
-  const float gs = ps * scale * (float)(L * L);  // mask*ctf*scale (ps carries 1/L^2)
+  const float gs = ps * scale * (float)(L * L) * mult;  // mult*mask*ctf*scale (ps carries 1/L^2)
   for (int t = 0; t < r; ++t) {
     float ax = s_z[t] * ex, ay = s_z[t] * ey;
 #pragma unroll
@@ -290,55 +313,61 @@ __global__ void __launch_bounds__(kMlBlock)
       ax += mv * qx[s];
       ay += mv * qy[s];
     }
-    g[((int64_t)t * B + i) * d + k] = make_float2(ax * gs, ay * gs);
+    g[((int64_t)t * B + i) * np + k] = make_float2(ax * gs, ay * gs);
   }
 }
 
 }  // namespace cov3d
 
 extern "C" int cov3d_ml_forward(const void* c, const void* ctf, const void* yhat,
-                                const void* lam_inv, int64_t B, int64_t r, int64_t L,
-                                double noise_var, int mean_const, int precision, void* cost,
-                                void* z, void* m_out, void* minv, void* b_out, void* stream) {
+                                const void* lam_inv, const int* pix_map, int64_t npix, int64_t B,
+                                int64_t r, int64_t L, double noise_var, int mean_const,
+                                int precision, void* cost, void* z, void* m_out, void* minv,
+                                void* b_out, void* stream) {
   COV3D_CHECK_ARG(r >= 1 && r <= 31, "rank r=%lld must be in [1, 31]", (long long)r);
   COV3D_CHECK_ARG(B >= 0 && L >= 1, "bad batch/size");
   COV3D_CHECK_ARG(noise_var > 0, "noise_var must be positive");
   COV3D_CHECK_ARG(precision == 0 || precision == 1, "precision must be 0 or 1");
   if (B == 0) return 0;
   COV3D_CHECK_ARG(c && yhat, "c and yhat must not be NULL");
+  COV3D_CHECK_ARG(pix_map ? (npix >= 1 && npix <= L * L) : (npix == 0 || npix == L * L),
+                  "npix=%lld invalid for L=%lld", (long long)npix, (long long)L);
+  const int64_t np = pix_map ? npix : L * L;
   hipStream_t st = cov3d::as_stream(stream);
   if (precision == 1)
     hipLaunchKernelGGL((cov3d::k_ml_forward<1>), dim3((unsigned)B), dim3(cov3d::kMlBlock), 0, st,
                        (const float2*)c, (const float*)ctf, (const float2*)yhat,
-                       (const float*)lam_inv, B, (int)r, (int)L, (float)noise_var, mean_const,
+                       (const float*)lam_inv, pix_map, np, B, (int)r, (int)L, (float)noise_var, mean_const,
                        (float*)cost, (float*)z, (float*)m_out, (float*)minv, (float*)b_out);
   else
     hipLaunchKernelGGL((cov3d::k_ml_forward<0>), dim3((unsigned)B), dim3(cov3d::kMlBlock), 0, st,
                        (const float2*)c, (const float*)ctf, (const float2*)yhat,
-                       (const float*)lam_inv, B, (int)r, (int)L, (float)noise_var, mean_const,
+                       (const float*)lam_inv, pix_map, np, B, (int)r, (int)L, (float)noise_var, mean_const,
                        (float*)cost, (float*)z, (float*)m_out, (float*)minv, (float*)b_out);
   COV3D_LAUNCH_CHECK();
   return 0;
 }
 
 extern "C" int cov3d_ml_grad_images(const void* c, const void* ctf, const void* yhat,
-                                    const void* z, const void* minv, void* g, int64_t B,
-                                    int64_t r, int64_t L, double noise_var, double grad_scale,
-                                    void* stream) {
+                                    const void* z, const void* minv, const int* pix_map,
+                                    int64_t npix, void* g, int64_t B, int64_t r, int64_t L,
+                                    double noise_var, double grad_scale, void* stream) {
   COV3D_CHECK_ARG(r >= 1 && r <= 31, "rank r=%lld must be in [1, 31]", (long long)r);
   COV3D_CHECK_ARG(B >= 0 && L >= 1, "bad batch/size");
   COV3D_CHECK_ARG(noise_var > 0, "noise_var must be positive");
   if (B == 0) return 0;
   COV3D_CHECK_ARG(c && yhat && z && minv && g, "NULL pointer");
   COV3D_CHECK_ARG(B < 65536, "batch too large for one launch (B=%lld)", (long long)B);
+  COV3D_CHECK_ARG(pix_map ? (npix >= 1 && npix <= L * L) : (npix == 0 || npix == L * L),
+                  "npix=%lld invalid for L=%lld", (long long)npix, (long long)L);
   hipStream_t st = cov3d::as_stream(stream);
-  const int64_t d = L * L;
+  const int64_t np = pix_map ? npix : L * L;
   const float scale = (float)(grad_scale / ((double)(L * L) * noise_var));
-  dim3 grid((unsigned)cov3d::ceil_div(d, cov3d::kMlBlock), (unsigned)B);
+  dim3 grid((unsigned)cov3d::ceil_div(np, cov3d::kMlBlock), (unsigned)B);
 #define COV3D_GRAD(RM)                                                                            \
   hipLaunchKernelGGL(cov3d::k_ml_grad<RM>, grid, dim3(cov3d::kMlBlock), 0, st, (const float2*)c, \
                      (const float*)ctf, (const float2*)yhat, (const float*)z, (const float*)minv, \
-                     (float2*)g, B, (int)r, (int)L, scale)
+                     pix_map, np, (float2*)g, B, (int)r, (int)L, scale)
   if (r <= 4)
     COV3D_GRAD(4);
   else if (r <= 8)

@@ -114,7 +114,16 @@ constexpr int kMaxSub = 1024;  // points per subproblem
 // ----------------------------------------------------------------------------
 // Optional HIP-event timing of kernel classes on the stream they run on.
 // ----------------------------------------------------------------------------
-enum TimedKernel { kTInterp = 0, kTSpread = 1, kTDeconvPad = 2, kTDeconvCrop = 3, kTFft = 4, kTSort = 5, kTCount };
+enum TimedKernel {
+  kTInterp = 0,
+  kTSpread = 1,
+  kTDeconvPad = 2,
+  kTDeconvCrop = 3,
+  kTFft = 4,
+  kTSort = 5,
+  kTPermute = 6,
+  kTCount
+};
 
 struct TimingRegistry {
   bool enabled = false;
@@ -283,13 +292,14 @@ __global__ void k_sub_fill(const int* bin_count, const int* bin_start, const int
 template <typename T>
 __global__ void k_bin_scatter(const T* x, const T* y, const T* z, int64_t M, BinGeom g,
                               const int* pt_bin, const int* pt_pos, const int* bin_start,
-                              int* sidx, T* sv0, T* sv1, T* sv2) {
+                              int* sidx, int* spos, T* sv0, T* sv1, T* sv2) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= M) return;
   double v[3];
   (void)point_bin(x, y, z, j, g, v);
   int dst = bin_start[pt_bin[j]] + pt_pos[j];
   sidx[dst] = (int)j;
+  spos[j] = dst;
   sv0[dst] = (T)v[0];
   sv1[dst] = (T)v[1];
   sv2[dst] = (T)v[2];
@@ -362,6 +372,33 @@ __global__ void k_deconv_crop(const Cplx<T>* __restrict__ fine, T* __restrict__ 
   }
 }
 
+
+// Point-order permutations between the caller's layout c[t][j] (j in input
+// order) and the sorted layouts the bin kernels use.  Both walk j in input
+// order so that the c[t][j] side is coalesced; the sorted side is a
+// contiguous run per point.
+//   to_sorted:   cs[spos[j]][t]        = c[t][j]      (spreading input)
+//   from_sorted: c[t][j] = cs[t / TC][spos[j]][t % TC] (interpolation output)
+template <typename T>
+__global__ void k_to_sorted(const Cplx<T>* __restrict__ c, Cplx<T>* __restrict__ cs,
+                            const int* __restrict__ spos, int64_t M, int ntrans) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= M) return;
+  Cplx<T>* dst = cs + (int64_t)spos[j] * ntrans;
+  for (int t = 0; t < ntrans; ++t) dst[t] = c[(int64_t)t * M + j];
+}
+
+template <typename T>
+__global__ void k_from_sorted(const Cplx<T>* __restrict__ cs, Cplx<T>* __restrict__ c,
+                              const int* __restrict__ spos, int64_t M, int ntrans, int tc) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= M) return;
+  const int64_t sp = spos[j];
+  for (int t = 0; t < ntrans; ++t) {
+    const int h = t / tc, tt = t - h * tc;
+    c[(int64_t)t * M + j] = cs[((int64_t)h * M + sp) * tc + tt];
+  }
+}
 
 struct SubDesc {
   const int* sub_sb;
@@ -470,107 +507,388 @@ __global__ void __launch_bounds__(kBlock)
         }
       }
     }
-    const int64_t j = sidx[p];
+    // sorted, chunk-major output: cs[chunk][p][TC]
+    Cplx<T>* dst = c + ((int64_t)blockIdx.y * M + p) * TC;
+#pragma unroll
+    for (int tt = 0; tt < TC; ++tt) dst[tt] = Cplx<T>{ax[tt], ay[tt]};
+  }
+}
+
+// Type-2 interpolation, tap-parallel: same LDS tile as k_interp, but each
+// wavefront takes one point at a time and lane q < W^2 owns the stencil row
+// (a0, a1) = (q / W, q % W), walking a2 = 0..W-1.  The lanes of one LDS read
+// touch W^2 distinct, regularly strided cells (few bank conflicts); the
+// per-row partial sums are then reduced across the wavefront with xor
+// shuffles.  The 3W kernel weights are evaluated once per point (one per
+// lane) and broadcast.
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
+  return v;
+}
+
+template <typename T, int W, int TC>
+__global__ void __launch_bounds__(kBlock)
+    k_interp_tp(const Cplx<T>* __restrict__ fine, Cplx<T>* __restrict__ c, SubDesc sd, BinGeom g,
+                const T* __restrict__ sv0, const T* __restrict__ sv1, const T* __restrict__ sv2,
+                int64_t M, int ntrans, T kc, T beta) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  Cplx<T>* tile = reinterpret_cast<Cplx<T>*>(smem_raw);
+  const int sub = xcd_remap(blockIdx.x, gridDim.x);
+  if (sub >= *sd.nsub) return;
+  const int t0 = blockIdx.y * TC;
+  const int sb = sd.sub_sb[sub];
+  const int s2 = sb % g.nsb[2];
+  const int s1 = (sb / g.nsb[2]) % g.nsb[1];
+  const int s0 = sb / (g.nsb[2] * g.nsb[1]);
+  constexpr int E = kSuperEdge + W;
+  constexpr int ncell = E * E * E;
+  const int o0 = s0 * kSuperEdge - W / 2, o1 = s1 * kSuperEdge - W / 2, o2 = s2 * kSuperEdge - W / 2;
+  const int64_t nfine = (int64_t)g.nf[0] * g.nf[1] * g.nf[2];
+  for (int cidx = threadIdx.x; cidx < ncell; cidx += blockDim.x) {
+    int e2 = cidx % E;
+    int e1 = (cidx / E) % E;
+    int e0 = cidx / (E * E);
+    int m0 = ((o0 + e0) % g.nf[0] + g.nf[0]) % g.nf[0];
+    int m1 = ((o1 + e1) % g.nf[1] + g.nf[1]) % g.nf[1];
+    int m2 = ((o2 + e2) % g.nf[2] + g.nf[2]) % g.nf[2];
+    int64_t off = ((int64_t)m0 * g.nf[1] + m1) * g.nf[2] + m2;
 #pragma unroll
     for (int tt = 0; tt < TC; ++tt) {
       int t = t0 + tt;
-      if (t < ntrans) c[(int64_t)t * M + j] = Cplx<T>{ax[tt], ay[tt]};
+      Cplx<T> v = {(T)0, (T)0};
+      if (t < ntrans) v = fine[t * nfine + off];
+      tile[cidx * TC + tt] = v;
+    }
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int nwave = blockDim.x >> 6;
+  const int p0 = sd.sub_pt[sub];
+  const int np = sd.sub_n[sub];
+  const T halfw = (T)(0.5 * W);
+  constexpr int NQ = (W * W + 63) / 64;
+  for (int p = p0 + wave; p < p0 + np; p += nwave) {
+    const T v0 = sv0[p], v1 = sv1[p], v2 = sv2[p];
+    const int j00 = (int)ceil(v0 - halfw), j01 = (int)ceil(v1 - halfw), j02 = (int)ceil(v2 - halfw);
+    T wv = (T)0;
+    if (lane < 3 * W) {
+      const int axis = lane / W, k = lane - axis * W;
+      const T v = axis == 0 ? v0 : (axis == 1 ? v1 : v2);
+      const int j0 = axis == 0 ? j00 : (axis == 1 ? j01 : j02);
+      wv = es_eval<T>((T)(j0 + k) - v, kc, beta);
+    }
+    const int l0 = min(max(j00 + W / 2, 0), kSuperEdge);
+    const int l1 = min(max(j01 + W / 2, 0), kSuperEdge);
+    const int l2 = min(max(j02 + W / 2, 0), kSuperEdge);
+    T wrow[NQ];
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+      const int q = lane + 64 * k;
+      const int a0 = min(q / W, W - 1), a1 = q % W;
+      wrow[k] = __shfl(wv, a0) * __shfl(wv, W + a1);
+    }
+    T w2[W];
+#pragma unroll
+    for (int a2 = 0; a2 < W; ++a2) w2[a2] = __shfl(wv, 2 * W + a2);
+    T ax[TC], ay[TC];
+#pragma unroll
+    for (int tt = 0; tt < TC; ++tt) ax[tt] = ay[tt] = (T)0;
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+      const int q = lane + 64 * k;
+      if (q < W * W) {
+        const int a0 = q / W, a1 = q % W;
+        const int base = ((l0 + a0) * E + (l1 + a1)) * E + l2;
+#pragma unroll
+        for (int a2 = 0; a2 < W; ++a2) {
+          const T w = wrow[k] * w2[a2];
+          if constexpr (TC % 2 == 0 && sizeof(T) == 4) {
+            const float4* cell = reinterpret_cast<const float4*>(tile + (base + a2) * TC);
+#pragma unroll
+            for (int h = 0; h < TC / 2; ++h) {
+              const float4 v = cell[h];
+              ax[2 * h] += w * v.x;
+              ay[2 * h] += w * v.y;
+              ax[2 * h + 1] += w * v.z;
+              ay[2 * h + 1] += w * v.w;
+            }
+          } else {
+#pragma unroll
+            for (int tt = 0; tt < TC; ++tt) {
+              const Cplx<T> v = tile[(base + a2) * TC + tt];
+              ax[tt] += w * v.x;
+              ay[tt] += w * v.y;
+            }
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int tt = 0; tt < TC; ++tt) {
+      ax[tt] = wave_sum(ax[tt]);
+      ay[tt] = wave_sum(ay[tt]);
+    }
+    if (lane == 0) {
+      Cplx<T>* dst = c + ((int64_t)blockIdx.y * M + p) * TC;
+#pragma unroll
+      for (int tt = 0; tt < TC; ++tt) dst[tt] = Cplx<T>{ax[tt], ay[tt]};
     }
   }
 }
 
 // Type-1 spreading as an output-driven gather: one 64-lane workgroup owns one
 // 4^3 bin of fine-grid cells (one cell per lane, TC transforms in registers)
-// and walks the points of the 27 neighbouring bins.  For each batch of points
-// the lanes evaluate the 3W kernel weights once and stage them with the
-// point values in LDS; every lane then adds its cell's contribution when the
-// cell lies inside the point's W^3 support.  Each cell is written exactly
-// once (no atomics, deterministic summation order).
+// and walks the points of the neighbouring bins in batches of up to 64.  A
+// batch is prefetched into registers (one point per lane) while the previous
+// batch is consumed, then staged in LDS with its 3W kernel weights (each lane
+// evaluates the weights of its own point).  Points whose W^3 support misses
+// the block are skipped wave-uniformly; inside, every lane adds its cell's
+// contribution.  Each cell is written exactly once: no atomics, deterministic
+// summation order.
 constexpr int kGatherBlock = 64;
-constexpr int kMaxBatch = 16;  // points per staging round
+constexpr int kGatherBatch = 32;
 
 template <typename T, int TC>
 __global__ void __launch_bounds__(kGatherBlock)
-    k_spread_gather(const Cplx<T>* __restrict__ c, Cplx<T>* __restrict__ fine, BinGeom g,
+    k_spread_gather(const Cplx<T>* __restrict__ cs, Cplx<T>* __restrict__ fine, BinGeom g,
                     const int* __restrict__ bin_start, const int* __restrict__ bin_count,
                     const T* __restrict__ sv0, const T* __restrict__ sv1, const T* __restrict__ sv2,
-                    const int* __restrict__ sidx, int64_t M, int ntrans, int W, T kc, T beta) {
-  __shared__ T s_w[kMaxBatch][3][16];
-  __shared__ int s_j0[kMaxBatch][3];
-  __shared__ Cplx<T> s_c[kMaxBatch][TC];
+                    int64_t M, int ntrans, int W, T kc, T beta) {
+  __shared__ T s_w[kGatherBatch][3][16];
+  __shared__ int s_j0[kGatherBatch][3];
+  __shared__ __align__(16) Cplx<T> s_c[kGatherBatch][TC];
   const int bin = xcd_remap(blockIdx.x, gridDim.x);
   const int t0 = blockIdx.y * TC;
   const int lane = threadIdx.x;
   int b0, b1, b2;
   bin_coords(g, bin, b0, b1, b2);
-  const int g0 = kBinEdge * b0 + (lane >> 4);
-  const int g1 = kBinEdge * b1 + ((lane >> 2) & 3);
-  const int g2 = kBinEdge * b2 + (lane & 3);
-  T ax[TC], ay[TC];
-#pragma unroll
-  for (int tt = 0; tt < TC; ++tt) ax[tt] = ay[tt] = (T)0;
+  const int c0 = lane >> 4, c1 = (lane >> 2) & 3, c2 = lane & 3;
   const T halfw = (T)(0.5 * W);
   // neighbour bins within reach of a W-wide support: ceil((W/2) / 4)
   const int R = (W + 2 * kBinEdge - 1) / (2 * kBinEdge);
-  for (int d0 = -R; d0 <= R; ++d0) {
-    int n0 = b0 + d0, sh0 = 0;
-    if (n0 < 0) { n0 += g.nb[0]; sh0 = -g.nf[0]; }
-    if (n0 >= g.nb[0]) { n0 -= g.nb[0]; sh0 = g.nf[0]; }
-    for (int d1 = -R; d1 <= R; ++d1) {
-      int n1 = b1 + d1, sh1 = 0;
+  const int nside = 2 * R + 1;
+  const int nnb = nside * nside * nside;
+
+  // iteration state over (neighbour bin, batch)
+  int nbi = 0, pnext = 0, pend = 0, nq0 = 0, nq1 = 0, nq2 = 0;
+  auto next_chunk = [&](int& cp, int& cn, int& cq0, int& cq1, int& cq2) -> bool {
+    while (pnext >= pend) {
+      if (nbi >= nnb) return false;
+      const int d0 = nbi / (nside * nside) - R, d1 = (nbi / nside) % nside - R, d2 = nbi % nside - R;
+      ++nbi;
+      int n0 = b0 + d0, n1 = b1 + d1, n2 = b2 + d2, sh0 = 0, sh1 = 0, sh2 = 0;
+      if (n0 < 0) { n0 += g.nb[0]; sh0 = -g.nf[0]; }
+      if (n0 >= g.nb[0]) { n0 -= g.nb[0]; sh0 = g.nf[0]; }
       if (n1 < 0) { n1 += g.nb[1]; sh1 = -g.nf[1]; }
       if (n1 >= g.nb[1]) { n1 -= g.nb[1]; sh1 = g.nf[1]; }
-      for (int d2 = -R; d2 <= R; ++d2) {
-        int n2 = b2 + d2, sh2 = 0;
-        if (n2 < 0) { n2 += g.nb[2]; sh2 = -g.nf[2]; }
-        if (n2 >= g.nb[2]) { n2 -= g.nb[2]; sh2 = g.nf[2]; }
-        const int nbin = bin_index(g, n0, n1, n2);
-        const int start = bin_start[nbin];
-        const int cnt = bin_count[nbin];
-        // this lane's cell relative to the neighbour's super-bin origin (unwrapped frame)
-        const int r0 = g0 - (n0 >> 1) * kSuperEdge - sh0;
-        const int r1 = g1 - (n1 >> 1) * kSuperEdge - sh1;
-        const int r2 = g2 - (n2 >> 1) * kSuperEdge - sh2;
-        for (int p = start; p < start + cnt; p += kMaxBatch) {
-          const int nb = min(kMaxBatch, start + cnt - p);
-          __syncthreads();  // previous batch fully consumed
-          for (int e = lane; e < nb * 3 * W; e += kGatherBlock) {
-            const int q = e / (3 * W), j = e % (3 * W);
-            const int axis = j / W, k = j % W;
-            const T v = axis == 0 ? sv0[p + q] : (axis == 1 ? sv1[p + q] : sv2[p + q]);
-            const int j0 = (int)ceil(v - halfw);
-            s_w[q][axis][k] = es_eval<T>((T)(j0 + k) - v, kc, beta);
-            if (k == 0) s_j0[q][axis] = j0;
-          }
-          for (int e = lane; e < nb * TC; e += kGatherBlock) {
-            const int q = e / TC, tt = e % TC;
-            const int t = t0 + tt;
-            s_c[q][tt] = t < ntrans ? c[(int64_t)t * M + sidx[p + q]] : Cplx<T>{(T)0, (T)0};
-          }
-          __syncthreads();
-          for (int q = 0; q < nb; ++q) {
-            const int x0 = r0 - s_j0[q][0], x1 = r1 - s_j0[q][1], x2 = r2 - s_j0[q][2];
-            const bool in = x0 >= 0 && x0 < W && x1 >= 0 && x1 < W && x2 >= 0 && x2 < W;
-            const T w = in ? s_w[q][0][min(max(x0, 0), W - 1)] * s_w[q][1][min(max(x1, 0), W - 1)] *
-                                 s_w[q][2][min(max(x2, 0), W - 1)]
-                           : (T)0;
+      if (n2 < 0) { n2 += g.nb[2]; sh2 = -g.nf[2]; }
+      if (n2 >= g.nb[2]) { n2 -= g.nb[2]; sh2 = g.nf[2]; }
+      const int nbin = bin_index(g, n0, n1, n2);
+      pnext = bin_start[nbin];
+      pend = pnext + bin_count[nbin];
+      // first cell of this block relative to the neighbour's super-bin origin (unwrapped frame)
+      nq0 = kBinEdge * b0 - (n0 >> 1) * kSuperEdge - sh0;
+      nq1 = kBinEdge * b1 - (n1 >> 1) * kSuperEdge - sh1;
+      nq2 = kBinEdge * b2 - (n2 >> 1) * kSuperEdge - sh2;
+    }
+    cp = pnext;
+    cn = min(kGatherBatch, pend - pnext);
+    cq0 = nq0;
+    cq1 = nq1;
+    cq2 = nq2;
+    pnext += cn;
+    return true;
+  };
+
+  T ax[TC], ay[TC];
 #pragma unroll
-            for (int tt = 0; tt < TC; ++tt) {
-              const Cplx<T> cv = s_c[q][tt];
-              ax[tt] += w * cv.x;
-              ay[tt] += w * cv.y;
-            }
-          }
+  for (int tt = 0; tt < TC; ++tt) ax[tt] = ay[tt] = (T)0;
+  T rv0 = 0, rv1 = 0, rv2 = 0;
+  Cplx<T> rc[TC];
+  auto prefetch = [&](int cp, int cn) {
+    if (lane < cn) {
+      rv0 = sv0[cp + lane];
+      rv1 = sv1[cp + lane];
+      rv2 = sv2[cp + lane];
+      const Cplx<T>* src = cs + (int64_t)(cp + lane) * ntrans + t0;
+#pragma unroll
+      for (int tt = 0; tt < TC; ++tt) rc[tt] = (t0 + tt < ntrans) ? src[tt] : Cplx<T>{(T)0, (T)0};
+    }
+  };
+  int cp, cn, cq0, cq1, cq2;
+  bool have = next_chunk(cp, cn, cq0, cq1, cq2);
+  if (have) prefetch(cp, cn);
+  while (have) {
+    __syncthreads();  // previous batch fully consumed
+    if (lane < cn) {
+      const T vv[3] = {rv0, rv1, rv2};
+#pragma unroll
+      for (int axis = 0; axis < 3; ++axis) {
+        const int j0 = (int)ceil(vv[axis] - halfw);
+        s_j0[lane][axis] = j0;
+        for (int k = 0; k < W; ++k) s_w[lane][axis][k] = es_eval<T>((T)(j0 + k) - vv[axis], kc, beta);
+      }
+#pragma unroll
+      for (int tt = 0; tt < TC; ++tt) s_c[lane][tt] = rc[tt];
+    }
+    __syncthreads();
+    int np_, nn_, nq0_, nq1_, nq2_;
+    const bool have_next = next_chunk(np_, nn_, nq0_, nq1_, nq2_);
+    if (have_next) prefetch(np_, nn_);
+    for (int q = 0; q < cn; ++q) {
+      const int j00 = s_j0[q][0], j01 = s_j0[q][1], j02 = s_j0[q][2];
+      // wave-uniform skip: the point's support misses this 4^3 block
+      if (cq0 > j00 + W - 1 || cq0 + 3 < j00 || cq1 > j01 + W - 1 || cq1 + 3 < j01 || cq2 > j02 + W - 1 ||
+          cq2 + 3 < j02)
+        continue;
+      const int x0 = cq0 + c0 - j00, x1 = cq1 + c1 - j01, x2 = cq2 + c2 - j02;
+      const bool in = x0 >= 0 && x0 < W && x1 >= 0 && x1 < W && x2 >= 0 && x2 < W;
+      const T w = in ? s_w[q][0][min(max(x0, 0), W - 1)] * s_w[q][1][min(max(x1, 0), W - 1)] *
+                           s_w[q][2][min(max(x2, 0), W - 1)]
+                     : (T)0;
+      if constexpr (TC % 2 == 0 && sizeof(T) == 4) {
+        const float4* cv4 = reinterpret_cast<const float4*>(&s_c[q][0]);
+#pragma unroll
+        for (int h = 0; h < TC / 2; ++h) {
+          const float4 v = cv4[h];
+          ax[2 * h] += w * v.x;
+          ay[2 * h] += w * v.y;
+          ax[2 * h + 1] += w * v.z;
+          ay[2 * h + 1] += w * v.w;
+        }
+      } else {
+#pragma unroll
+        for (int tt = 0; tt < TC; ++tt) {
+          const Cplx<T> cv = s_c[q][tt];
+          ax[tt] += w * cv.x;
+          ay[tt] += w * cv.y;
         }
       }
     }
+    have = have_next;
+    cp = np_;
+    cn = nn_;
+    cq0 = nq0_;
+    cq1 = nq1_;
+    cq2 = nq2_;
   }
+  const int g0 = kBinEdge * b0 + c0, g1 = kBinEdge * b1 + c1, g2 = kBinEdge * b2 + c2;
   const int64_t nfine = (int64_t)g.nf[0] * g.nf[1] * g.nf[2];
   const int64_t off = ((int64_t)g0 * g.nf[1] + g1) * g.nf[2] + g2;
 #pragma unroll
   for (int tt = 0; tt < TC; ++tt) {
     const int t = t0 + tt;
     if (t < ntrans) fine[t * nfine + off] = Cplx<T>{ax[tt], ay[tt]};
+  }
+}
+
+// Type-1 spreading, point-driven ("tap-parallel"): one workgroup per
+// (super-bin subproblem, transform chunk) accumulates into an LDS tile
+// [TC][re|im][cell] of the 8^3 super-bin plus halo.  Each wavefront takes one
+// point at a time; lane q < W^2 owns the stencil row (a0, a1) = (q / W, q % W)
+// and walks a2 = 0..W-1, so the W^2 lanes of one ds_add_f32 hit W^2 distinct
+// cells (no same-address serialisation).  The 3W kernel weights are computed
+// once per point by 3W lanes and broadcast with readlane / bpermute.  The
+// tile is then added to the periodic fine grid with float atomics (tiles of
+// neighbouring super-bins overlap in their halos).
+template <typename T, int W, int TC>
+__global__ void __launch_bounds__(kBlock)
+    k_spread_tiles(const Cplx<T>* __restrict__ cs, Cplx<T>* __restrict__ fine, SubDesc sd, BinGeom g,
+                   const T* __restrict__ sv0, const T* __restrict__ sv1, const T* __restrict__ sv2,
+                   int64_t M, int ntrans, T kc, T beta) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* tile = reinterpret_cast<T*>(smem_raw);
+  const int sub = xcd_remap(blockIdx.x, gridDim.x);
+  if (sub >= *sd.nsub) return;
+  const int t0 = blockIdx.y * TC;
+  const int sb = sd.sub_sb[sub];
+  const int s2 = sb % g.nsb[2];
+  const int s1 = (sb / g.nsb[2]) % g.nsb[1];
+  const int s0 = sb / (g.nsb[2] * g.nsb[1]);
+  constexpr int E = kSuperEdge + W;
+  constexpr int ncell = E * E * E;
+  for (int i = threadIdx.x; i < ncell * TC * 2; i += blockDim.x) tile[i] = (T)0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int nwave = blockDim.x >> 6;
+  const int p0 = sd.sub_pt[sub];
+  const int np = sd.sub_n[sub];
+  const T halfw = (T)(0.5 * W);
+  for (int p = p0 + wave; p < p0 + np; p += nwave) {
+    // weights: lane j < 3W evaluates axis j / W, tap j % W
+    const T v0 = sv0[p], v1 = sv1[p], v2 = sv2[p];
+    const int j00 = (int)ceil(v0 - halfw), j01 = (int)ceil(v1 - halfw), j02 = (int)ceil(v2 - halfw);
+    T wv = (T)0;
+    if (lane < 3 * W) {
+      const int axis = lane / W, k = lane - axis * W;
+      const T v = axis == 0 ? v0 : (axis == 1 ? v1 : v2);
+      const int j0 = axis == 0 ? j00 : (axis == 1 ? j01 : j02);
+      wv = es_eval<T>((T)(j0 + k) - v, kc, beta);
+    }
+    const int l0 = min(max(j00 + W / 2, 0), kSuperEdge);
+    const int l1 = min(max(j01 + W / 2, 0), kSuperEdge);
+    const int l2 = min(max(j02 + W / 2, 0), kSuperEdge);
+    Cplx<T> cv[TC];
+#pragma unroll
+    for (int tt = 0; tt < TC; ++tt) {
+      const int t = t0 + tt;
+      cv[tt] = t < ntrans ? cs[(int64_t)p * ntrans + t] : Cplx<T>{(T)0, (T)0};
+    }
+    // broadcasts with the whole wave active: the (a0, a1) row weight of each of
+    // this lane's stencil rows and the W weights along axis 2
+    constexpr int NQ = (W * W + 63) / 64;
+    T wrow[NQ];
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+      const int q = lane + 64 * k;
+      const int a0 = min(q / W, W - 1), a1 = q % W;
+      wrow[k] = __shfl(wv, a0) * __shfl(wv, W + a1);
+    }
+    T w2[W];
+#pragma unroll
+    for (int a2 = 0; a2 < W; ++a2) w2[a2] = __shfl(wv, 2 * W + a2);
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+      const int q = lane + 64 * k;
+      if (q < W * W) {
+        const int a0 = q / W, a1 = q % W;
+        const int base = ((l0 + a0) * E + (l1 + a1)) * E + l2;
+#pragma unroll
+        for (int a2 = 0; a2 < W; ++a2) {
+          const T w = wrow[k] * w2[a2];
+#pragma unroll
+          for (int tt = 0; tt < TC; ++tt) {
+            atomicAdd(tile + (2 * tt) * ncell + base + a2, w * cv[tt].x);
+            atomicAdd(tile + (2 * tt + 1) * ncell + base + a2, w * cv[tt].y);
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const int64_t nfine = (int64_t)g.nf[0] * g.nf[1] * g.nf[2];
+  const int o0 = s0 * kSuperEdge - W / 2, o1 = s1 * kSuperEdge - W / 2, o2 = s2 * kSuperEdge - W / 2;
+  T* finef = reinterpret_cast<T*>(fine);
+  for (int cidx = threadIdx.x; cidx < ncell; cidx += blockDim.x) {
+    const int e2 = cidx % E;
+    const int e1 = (cidx / E) % E;
+    const int e0 = cidx / (E * E);
+    const int m0 = ((o0 + e0) % g.nf[0] + g.nf[0]) % g.nf[0];
+    const int m1 = ((o1 + e1) % g.nf[1] + g.nf[1]) % g.nf[1];
+    const int m2 = ((o2 + e2) % g.nf[2] + g.nf[2]) % g.nf[2];
+    const int64_t off = ((int64_t)m0 * g.nf[1] + m1) * g.nf[2] + m2;
+#pragma unroll
+    for (int tt = 0; tt < TC; ++tt) {
+      const int t = t0 + tt;
+      if (t >= ntrans) break;
+      const T vx = tile[(2 * tt) * ncell + cidx], vy = tile[(2 * tt + 1) * ncell + cidx];
+      if (vx != (T)0) atomicAdd(finef + 2 * (t * nfine + off), vx);
+      if (vy != (T)0) atomicAdd(finef + 2 * (t * nfine + off) + 1, vy);
+    }
   }
 }
 
@@ -587,6 +905,10 @@ struct Plan {
   int64_t ntrans = 1;
   int w = 7;
   int tc = 2;  // transforms per interpolation workgroup
+  int spread_mode = 0;  // 0 = output-driven gather, 1 = point-driven LDS tiles
+  int interp_mode = 0;  // 0 = thread per point, 1 = wavefront per point (tap-parallel)
+  int max_tc_gather = 20;  // transforms per gather workgroup
+  int tc_spread = 2;    // transforms per spreading workgroup (tile mode)
   double beta = 0, eps = 1e-6;
   BinGeom geom;
   int64_t nbins = 0, nsuper = 0;
@@ -604,6 +926,8 @@ struct Plan {
   int* pt_bin = nullptr;
   int* pt_pos = nullptr;
   int* sidx = nullptr;
+  int* spos = nullptr;
+  void* csort = nullptr;  // sorted-order staging of point values (M * ntrans complex)
   void* sv[3] = {nullptr, nullptr, nullptr};
   rocfft_plan fft_fwd = nullptr, fft_bwd = nullptr;
   rocfft_execution_info info = nullptr;
@@ -618,11 +942,14 @@ static int free_points(Plan* p) {
   hipFree(p->pt_bin);
   hipFree(p->pt_pos);
   hipFree(p->sidx);
+  hipFree(p->spos);
+  hipFree(p->csort);
   for (int k = 0; k < 3; ++k) hipFree(p->sv[k]);
   hipFree(p->sub_sb);
   hipFree(p->sub_pt);
   hipFree(p->sub_n);
-  p->pt_bin = p->pt_pos = p->sidx = nullptr;
+  p->pt_bin = p->pt_pos = p->sidx = p->spos = nullptr;
+  p->csort = nullptr;
   p->sub_sb = p->sub_pt = p->sub_n = nullptr;
   for (int k = 0; k < 3; ++k) p->sv[k] = nullptr;
   p->M_cap = 0;
@@ -647,15 +974,28 @@ static int launch_interp(Plan* p, void* c_out, int ntrans_run, hipStream_t st) {
   if (smem > 65536 && !attr_done) {
     COV3D_HIP(hipFuncSetAttribute((const void*)k_interp<T, W, TC>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    COV3D_HIP(hipFuncSetAttribute((const void*)k_interp_tp<T, W, TC>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     attr_done = true;
   }
   SubDesc sd{p->sub_sb, p->sub_pt, p->sub_n, p->nsub};
   dim3 grid((unsigned)p->sub_cap, (unsigned)ceil_div(ntrans_run, TC));
   const T kc = (T)(4.0 / ((double)W * W));
-  ScopedTiming tm(kTInterp, st);
-  hipLaunchKernelGGL((k_interp<T, W, TC>), grid, dim3(kBlock), smem, st, (const Cplx<T>*)p->fine,
-                     (Cplx<T>*)c_out, sd, p->geom, (const T*)p->sv[0], (const T*)p->sv[1],
-                     (const T*)p->sv[2], p->sidx, p->M, ntrans_run, kc, (T)p->beta);
+  {
+    ScopedTiming tm(kTInterp, st);
+    if (p->interp_mode == 1)
+      hipLaunchKernelGGL((k_interp_tp<T, W, TC>), grid, dim3(kBlock), smem, st, (const Cplx<T>*)p->fine,
+                         (Cplx<T>*)p->csort, sd, p->geom, (const T*)p->sv[0], (const T*)p->sv[1],
+                         (const T*)p->sv[2], p->M, ntrans_run, kc, (T)p->beta);
+    else
+      hipLaunchKernelGGL((k_interp<T, W, TC>), grid, dim3(kBlock), smem, st, (const Cplx<T>*)p->fine,
+                         (Cplx<T>*)p->csort, sd, p->geom, (const T*)p->sv[0], (const T*)p->sv[1],
+                         (const T*)p->sv[2], p->sidx, p->M, ntrans_run, kc, (T)p->beta);
+  }
+  COV3D_LAUNCH_CHECK();
+  ScopedTiming tm2(kTPermute, st);
+  hipLaunchKernelGGL((k_from_sorted<T>), dim3((unsigned)ceil_div(p->M, kBlock)), dim3(kBlock), 0, st,
+                     (const Cplx<T>*)p->csort, (Cplx<T>*)c_out, p->spos, p->M, ntrans_run, TC);
   COV3D_LAUNCH_CHECK();
   return 0;
 }
@@ -685,21 +1025,82 @@ static int run_interp(Plan* p, void* c_out, int ntrans_run, hipStream_t st) {
 
 template <typename T, int TC>
 static int launch_spread(Plan* p, const void* c_in, int ntrans_run, hipStream_t st) {
+  {
+    ScopedTiming tm(kTPermute, st);
+    hipLaunchKernelGGL((k_to_sorted<T>), dim3((unsigned)ceil_div(p->M, kBlock)), dim3(kBlock), 0, st,
+                       (const Cplx<T>*)c_in, (Cplx<T>*)p->csort, p->spos, p->M, ntrans_run);
+    COV3D_LAUNCH_CHECK();
+  }
   dim3 grid((unsigned)p->nbins, (unsigned)ceil_div(ntrans_run, TC));
   const T kc = (T)(4.0 / ((double)p->w * p->w));
   ScopedTiming tm(kTSpread, st);
-  hipLaunchKernelGGL((k_spread_gather<T, TC>), grid, dim3(kGatherBlock), 0, st, (const Cplx<T>*)c_in,
+  hipLaunchKernelGGL((k_spread_gather<T, TC>), grid, dim3(kGatherBlock), 0, st, (const Cplx<T>*)p->csort,
                      (Cplx<T>*)p->fine, p->geom, p->bin_start, p->bin_count, (const T*)p->sv[0],
-                     (const T*)p->sv[1], (const T*)p->sv[2], p->sidx, p->M, ntrans_run, p->w, kc,
+                     (const T*)p->sv[1], (const T*)p->sv[2], p->M, ntrans_run, p->w, kc,
                      (T)p->beta);
   COV3D_LAUNCH_CHECK();
   return 0;
 }
 
-// Transforms per gather workgroup: at most 20 in registers, split evenly.
+template <typename T, int W, int TC>
+static int launch_spread_tiles(Plan* p, int ntrans_run, hipStream_t st) {
+  const size_t smem = (size_t)(kSuperEdge + W) * (kSuperEdge + W) * (kSuperEdge + W) * TC * 2 * sizeof(T);
+  COV3D_CHECK_ARG(smem <= 160 * 1024, "spread tile of %zu bytes exceeds LDS", smem);
+  static bool attr_done = false;
+  if (smem > 65536 && !attr_done) {
+    COV3D_HIP(hipFuncSetAttribute((const void*)k_spread_tiles<T, W, TC>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    attr_done = true;
+  }
+  const int64_t total = p->nf[0] * p->nf[1] * p->nf[2] * (int64_t)ntrans_run * 2;
+  hipLaunchKernelGGL((k_fill_zero<T>), dim3((unsigned)std::min<int64_t>(ceil_div(total, kBlock), 65536)),
+                     dim3(kBlock), 0, st, (T*)p->fine, total);
+  COV3D_LAUNCH_CHECK();
+  SubDesc sd{p->sub_sb, p->sub_pt, p->sub_n, p->nsub};
+  dim3 grid((unsigned)p->sub_cap, (unsigned)ceil_div(ntrans_run, TC));
+  const T kc = (T)(4.0 / ((double)W * W));
+  ScopedTiming tm(kTSpread, st);
+  hipLaunchKernelGGL((k_spread_tiles<T, W, TC>), grid, dim3(kBlock), smem, st, (const Cplx<T>*)p->csort,
+                     (Cplx<T>*)p->fine, sd, p->geom, (const T*)p->sv[0], (const T*)p->sv[1],
+                     (const T*)p->sv[2], p->M, ntrans_run, kc, (T)p->beta);
+  COV3D_LAUNCH_CHECK();
+  return 0;
+}
+
+template <typename T, int TC>
+static int dispatch_spread_tiles_w(Plan* p, int ntrans_run, hipStream_t st) {
+  switch (p->w) {
+#define COV3D_W(WW) \
+  case WW:          \
+    return launch_spread_tiles<T, WW, TC>(p, ntrans_run, st);
+    COV3D_W(2) COV3D_W(3) COV3D_W(4) COV3D_W(5) COV3D_W(6) COV3D_W(7) COV3D_W(8) COV3D_W(9)
+    COV3D_W(10) COV3D_W(11) COV3D_W(12) COV3D_W(13) COV3D_W(14) COV3D_W(15) COV3D_W(16)
+#undef COV3D_W
+    default:
+      set_error("unsupported kernel width %d", p->w);
+      return COV3D_ERR_ARG;
+  }
+}
+
+// Transforms per gather workgroup: at most p->max_tc_gather (<= 20) in
+// registers, split evenly.
 template <typename T>
 static int run_spread(Plan* p, const void* c_in, int ntrans_run, hipStream_t st) {
-  const int chunks = (int)ceil_div(ntrans_run, 20);
+  if (p->spread_mode == 1) {
+    {
+      ScopedTiming tm(kTPermute, st);
+      hipLaunchKernelGGL((k_to_sorted<T>), dim3((unsigned)ceil_div(p->M, kBlock)), dim3(kBlock), 0, st,
+                         (const Cplx<T>*)c_in, (Cplx<T>*)p->csort, p->spos, p->M, ntrans_run);
+      COV3D_LAUNCH_CHECK();
+    }
+    if (p->dtype == 1 || p->tc_spread == 1) return dispatch_spread_tiles_w<T, 1>(p, ntrans_run, st);
+    if constexpr (sizeof(T) == 4) {
+      if (p->tc_spread == 4) return dispatch_spread_tiles_w<T, 4>(p, ntrans_run, st);
+      return dispatch_spread_tiles_w<T, 2>(p, ntrans_run, st);
+    }
+    return dispatch_spread_tiles_w<T, 1>(p, ntrans_run, st);
+  }
+  const int chunks = (int)ceil_div(ntrans_run, p->max_tc_gather);
   const int need = (int)ceil_div(ntrans_run, chunks);
   if (need <= 1) return launch_spread<T, 1>(p, c_in, ntrans_run, st);
   if (need <= 2) return launch_spread<T, 2>(p, c_in, ntrans_run, st);
@@ -790,6 +1191,9 @@ static int setpts_impl(Plan* p, int64_t M, const T* x, const T* y, const T* z, h
     COV3D_HIP(hipMalloc(&p->pt_bin, Ma * sizeof(int)));
     COV3D_HIP(hipMalloc(&p->pt_pos, Ma * sizeof(int)));
     COV3D_HIP(hipMalloc(&p->sidx, Ma * sizeof(int)));
+    COV3D_HIP(hipMalloc(&p->spos, Ma * sizeof(int)));
+    // interpolation writes whole chunks of up to 4 transforms: round ntrans up to 4
+    COV3D_HIP(hipMalloc(&p->csort, Ma * ceil_div(p->ntrans, 4) * 4 * (p->dtype == 1 ? 16 : 8)));
     for (int k = 0; k < 3; ++k) COV3D_HIP(hipMalloc(&p->sv[k], Ma * sizeof(T)));
     const int64_t ca = cap > 8 ? cap : 8;
     COV3D_HIP(hipMalloc(&p->sub_sb, ca * sizeof(int)));
@@ -820,7 +1224,8 @@ static int setpts_impl(Plan* p, int64_t M, const T* x, const T* y, const T* z, h
     COV3D_LAUNCH_CHECK();
     dim3 gm((unsigned)ceil_div(M, kBlock));
     hipLaunchKernelGGL((k_bin_scatter<T>), gm, dim3(kBlock), 0, st, x, y, z, M, p->geom, p->pt_bin,
-                       p->pt_pos, p->bin_start, p->sidx, (T*)p->sv[0], (T*)p->sv[1], (T*)p->sv[2]);
+                       p->pt_pos, p->bin_start, p->sidx, p->spos, (T*)p->sv[0], (T*)p->sv[1],
+                       (T*)p->sv[2]);
     COV3D_LAUNCH_CHECK();
   }
   p->pts_set = true;
@@ -864,6 +1269,11 @@ static int make_plan(int type, const int64_t* n, int iflag, int64_t ntrans, doub
   p->beta = es_beta(p->w);
   const int tc_env = env_int("COV3D_NUFFT_TC", 2);
   p->tc = dtype == 1 ? 1 : (tc_env == 1 || tc_env == 4 ? tc_env : 2);
+  p->spread_mode = env_int("COV3D_SPREAD_MODE", 0) == 1 ? 1 : 0;
+  p->interp_mode = env_int("COV3D_INTERP_MODE", 0) == 1 ? 1 : 0;
+  p->max_tc_gather = std::min(std::max(env_int("COV3D_GATHER_TC", 20), 1), 20);
+  const int tcs_env = env_int("COV3D_SPREAD_TC", 2);
+  p->tc_spread = dtype == 1 ? 1 : (tcs_env == 1 || tcs_env == 4 ? tcs_env : 2);
   const size_t csize = dtype == 1 ? 16 : 8;
   for (int k = 0; k < 3; ++k) {
     p->N[k] = n[k];
@@ -1022,7 +1432,7 @@ extern "C" int cov3d_timing_enable(int on) {
 }
 
 // Sums the recorded intervals of one kernel class (0 interp, 1 spread,
-// 2 deconv/pad, 3 deconv/crop, 4 rocFFT, 5 bin sort); waits for them.
+// 2 deconv/pad, 3 deconv/crop, 4 rocFFT, 5 bin sort, 6 point permutes); waits for them.
 extern "C" int cov3d_timing_read(int kind, double* host_total_ms, int64_t* host_count) {
   COV3D_CHECK_ARG(kind >= 0 && kind < cov3d::kTCount, "bad timing kind %d", kind);
   double tot = 0;

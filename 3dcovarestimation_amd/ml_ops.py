@@ -28,6 +28,27 @@ from . import _native
 PRECISION_FP32 = 0
 PRECISION_BF16X3 = 1
 
+_HALF_CACHE = {}
+
+
+def half_plane_pixels(L: int, device) -> torch.Tensor:
+    """Row-major indices (row * L + col) of one pixel of every +-k pair, plus the centre.
+
+    Even L: rows/columns 0 are the zeroed Nyquist lines, pixel (i, j) pairs with
+    (L - i, L - j); kept: rows > L/2 and row L/2 from column L/2 on.
+    Odd L: (i, j) pairs with (L-1-i, L-1-j); kept: rows > c and row c from c on.
+    """
+    key = (L, str(device))
+    if key not in _HALF_CACHE:
+        c = L // 2
+        rows = []
+        lo = 1 if L % 2 == 0 else 0
+        for i in range(c, L):
+            j0 = c if i == c else lo
+            rows.append(torch.arange(j0, L) + i * L)
+        _HALF_CACHE[key] = torch.cat(rows).to(torch.int32).to(device)
+    return _HALF_CACHE[key]
+
 
 def ml_forward(
     c: torch.Tensor,
@@ -41,11 +62,13 @@ def ml_forward(
     want_m: bool = False,
     want_minv: bool = True,
     want_b: bool = False,
+    pix_map: Optional[torch.Tensor] = None,
 ):
     """Per-image ML terms from raw type-2 values.
 
-    c: (r, B, L*L) complex64 raw NUFFT values; yhat: (B, L, L) complex64
-    centred FFT of the images; ctf: (B, L, L) float32 or None.
+    c: (r, B, L*L) complex64 raw NUFFT values (or (r, B, npix) on the pixels of
+    ``pix_map``); yhat: (B, L, L) complex64 centred FFT of the images; ctf:
+    (B, L, L) float32 or None.
     Returns dict with cost (B,), z (B, r) and optionally m (B, r, r),
     minv (B, r, r), b (B, r).
     """
@@ -69,7 +92,8 @@ def ml_forward(
     lib = _native.load()
     _native.check(
         lib.cov3d_ml_forward(
-            c.data_ptr(), _native.ptr(ctf_t), yhat.data_ptr(), _native.ptr(lam_t), B, r, L,
+            c.data_ptr(), _native.ptr(ctf_t), yhat.data_ptr(), _native.ptr(lam_t), _native.ptr(pix_map),
+            0 if pix_map is None else pix_map.numel(), B, r, L,
             float(noise_var), 1 if mean_const else 0, int(precision),
             out["cost"].data_ptr(), out["z"].data_ptr(), _native.ptr(out.get("m")),
             _native.ptr(out.get("minv")), _native.ptr(out.get("b")), _native.stream_ptr(dev),
@@ -88,6 +112,7 @@ def ml_grad_images(
     grad_scale: float,
     ctf: Optional[torch.Tensor] = None,
     out: Optional[torch.Tensor] = None,
+    pix_map: Optional[torch.Tensor] = None,
 ) -> torch.Tensor:
     """Gradient of the summed ML cost with respect to the raw type-2 values, times grad_scale."""
     r, B = c.shape[0], c.shape[1]
@@ -97,43 +122,70 @@ def ml_grad_images(
     _native.check(
         lib.cov3d_ml_grad_images(
             c.data_ptr(), _native.ptr(ctf_t), yhat.contiguous().data_ptr(), z.contiguous().data_ptr(),
-            minv.contiguous().data_ptr(), g.data_ptr(), B, r, L, float(noise_var), float(grad_scale),
+            minv.contiguous().data_ptr(), _native.ptr(pix_map), 0 if pix_map is None else pix_map.numel(),
+            g.data_ptr(), B, r, L, float(noise_var), float(grad_scale),
             _native.stream_ptr(c.device),
         )
     )
     return g
 
 
+def _half_plan(plan):
+    """A companion plan of ``plan`` for the half-plane points (created once, kept on the plan)."""
+    hp = getattr(plan, "_half_plan", None)
+    if hp is None:
+        from .nufft_plan import NufftPlan
+
+        hp = NufftPlan(plan.sz, batch_size=plan.batch_size, eps=plan.eps, dtype=plan.dtype, device=plan.device)
+        plan._half_plan = hp
+    return hp
+
+
 class MLCostNufft(torch.autograd.Function):
     """Batch-mean spatial ML cost of rank-r real eigenvolumes on a NufftPlan.
 
     forward(vols (r, L, L, L) f32, yhat (B, L, L) c64, plan, ctf (B, L, L) or
-    None, noise_var, mean_const, precision) -> scalar cost
+    None, noise_var, mean_const, precision, half) -> scalar cost
     = 0.5 * mean_i(-b_i.z_i + logdet M_i) [+ |y_i|^2/noise_var].
-    The plan's points must stay set until backward has run.
+    With ``half`` the NUFFTs run on one pixel of every +-k pair (the
+    projections of real volumes are Hermitian), through a companion plan
+    whose points are taken from ``plan.points``.  The plan's points must stay
+    set until backward has run.
     """
 
     @staticmethod
-    def forward(ctx, vols, yhat, plan, ctf, noise_var, mean_const, precision):
+    def forward(ctx, vols, yhat, plan, ctf, noise_var, mean_const, precision, half=True):
         r = vols.shape[0]
         L = vols.shape[-1]
         B = yhat.shape[0]
-        c = plan.execute_forward(vols.detach())  # (batch_size, B, L, L) complex
-        c = c[:r].reshape(r, B, L * L)
-        res = ml_forward(c, yhat, L, noise_var, ctf=ctf, mean_const=mean_const, precision=precision)
-        ctx.plan = plan
+        if half:
+            pix = half_plane_pixels(L, vols.device)
+            hp = _half_plan(plan)
+            pts = plan.points.reshape(3, B, L * L)[:, :, pix.long()].reshape(3, -1)
+            hp.set_points_flat(pts, B)
+            run_plan = hp
+            c = hp.execute_forward_flat(vols.detach())[:r].reshape(r, B, pix.numel())
+        else:
+            pix = None
+            run_plan = plan
+            c = plan.execute_forward(vols.detach())[:r].reshape(r, B, L * L)
+        res = ml_forward(c, yhat, L, noise_var, ctf=ctf, mean_const=False, precision=precision, pix_map=pix)
+        cost = res["cost"].mean()
+        if mean_const:
+            cost = cost + 0.5 * (yhat.abs() ** 2).sum(dim=(1, 2)).mean() / (L * L) / noise_var
+        ctx.plan = run_plan
+        ctx.pix = pix
         ctx.meta = (L, B, float(noise_var))
         ctx.save_for_backward(c, yhat, res["z"], res["minv"], ctf if ctf is not None else torch.empty(0))
         ctx.has_ctf = ctf is not None
-        return res["cost"].mean()
+        return cost
 
     @staticmethod
     def backward(ctx, grad_out):
         c, yhat, z, minv, ctf = ctx.saved_tensors
         L, B, noise_var = ctx.meta
-        g = ml_grad_images(
-            c, yhat, z, minv, L, noise_var, 1.0 / B, ctf=ctf if ctx.has_ctf else None, out=c
-        )
+        g = ml_grad_images(c, yhat, z, minv, L, noise_var, 1.0 / B, ctf=ctf if ctx.has_ctf else None, out=c,
+                           pix_map=ctx.pix)
         grad_v = ctx.plan.execute_adjoint(g.reshape(g.shape[0], -1), real_output=True)
         grad_v = grad_v[: g.shape[0]] * grad_out
-        return grad_v, None, None, None, None, None, None
+        return grad_v, None, None, None, None, None, None, None

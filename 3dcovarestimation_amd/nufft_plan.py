@@ -253,6 +253,39 @@ class NufftPlan(BaseNufftPlan):
                 )
             )
 
+    def set_points_flat(self, pts: torch.Tensor, n_images: int) -> None:
+        """Sets already-wrapped points (3, M) directly (M need not be n_images * L**2)."""
+        _native.require_device(pts, "points")
+        pts = pts.to(self.dtype).contiguous()
+        self.points = pts
+        self.points_N = n_images
+        self.points_L = 0
+        with torch.cuda.device(self.device):
+            _native.check(
+                self._lib.cov3d_nufft_setpts(
+                    self._handle, pts.shape[1], pts[0].data_ptr(), pts[1].data_ptr(), pts[2].data_ptr(),
+                    _native.stream_ptr(self.device),
+                )
+            )
+
+    def execute_forward_flat(self, signal: torch.Tensor) -> torch.Tensor:
+        """Type-2 NUFFT returning the raw (batch_size, M) values."""
+        if self.points is None:
+            raise RuntimeError("points must be set before execute_forward_flat")
+        _native.require_device(signal, "signal")
+        signal, n = self._pad_batch(signal)
+        real_in = not signal.is_complex()
+        sig = signal.to(self.dtype if real_in else self.complex_dtype).contiguous()
+        out = torch.zeros((self.batch_size, self.points.shape[1]), dtype=self.complex_dtype, device=self.device)
+        with torch.cuda.device(self.device):
+            _native.check(
+                self._lib.cov3d_nufft_execute_dir(
+                    self._handle, 2, out.data_ptr(), sig.data_ptr(), 1 if real_in else 0, n,
+                    _native.stream_ptr(self.device),
+                )
+            )
+        return out
+
     def _pad_batch(self, signal: torch.Tensor) -> Tuple[torch.Tensor, int]:
         n = signal.shape[0]
         if n < self.batch_size:

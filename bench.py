@@ -154,15 +154,18 @@ def main():
 
     # roofline of the dominant NUFFT kernel (algorithmic bytes per launch)
     nf = 2 * L
-    d = L * L
+    # points per image on the chosen path: one pixel of every +-k pair (Hermitian half plane)
+    ml_ops = importlib.import_module(pkg + ".ml_ops")
+    d = int(ml_ops.half_plane_pixels(L, "cpu").numel())
     dom = max(("interp", "spread"), key=lambda k: kern[k]["total_ms"] or 0)
     avg_ms = kern[dom]["avg_ms"]
+    # per point: 3 coordinates (12 B) + r complex values (8r B); plus the r fine grids once
     bytes_per_launch = B * (12 * d + 8 * r * d) + 8 * r * nf**3
     w = 7
     flops_per_launch = B * d * (w**3 * r * 4)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     roof = {
-        "kernel": f"k_{dom} (ES w=7, nf={nf}, r={r}, B={B})",
+        "kernel": f"k_{dom} (ES w=7, nf={nf}, r={r}, {B} images x {d} half-plane points)",
         "bound": "hbm",
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,

@@ -128,9 +128,15 @@ int cov3d_slice_adjoint(const void* sig, const void* grid, void* vol_out,
  * Outputs (any may be NULL): cost (B) f32, z (B, r) f32, m_out (B, r, r) f32,
  *   minv (B, r, r) f32, b_out (B, r) f32.
  * r must be in [1, 31].
+ * pix_map (npix ints, device) lists the grid pixels (row * L + col) that c
+ * holds per image, in order; NULL means all L*L pixels (npix 0 or L*L).  A
+ * Hermitian half plane (each +-k pair once, plus the centre pixel) gives the
+ * full-plane sums with weight 2 per pair, since the projections of real
+ * volumes satisfy X(-k) = conj(X(k)); c is then (r, B, npix).
  * ------------------------------------------------------------------------ */
 int cov3d_ml_forward(const void* c, const void* ctf, const void* yhat,
-                     const void* lam_inv, int64_t B, int64_t r, int64_t L,
+                     const void* lam_inv, const int* pix_map, int64_t npix,
+                     int64_t B, int64_t r, int64_t L,
                      double noise_var, int mean_const, int precision,
                      void* cost, void* z, void* m_out, void* minv,
                      void* b_out, void* stream);
@@ -140,11 +146,13 @@ int cov3d_ml_forward(const void* c, const void* ctf, const void* yhat,
  * with scale = grad_scale / (L*L * noise_var) (grad_scale = 1/B for the
  * batch-mean cost).  g (r, B, d) complex64 may alias c (in place).  Its
  * type-1 NUFFT (real part) is the eigenvolume gradient that autograd
- * through TorchNufftForward.backward (nufft_plan.py:441-465) produces. */
+ * through TorchNufftForward.backward (nufft_plan.py:441-465) produces.  With
+ * a half-plane pix_map the entries carry the pair weight 2 (centre 1), so
+ * the real part of the type-1 sum over the half plane is the full sum. */
 int cov3d_ml_grad_images(const void* c, const void* ctf, const void* yhat,
-                         const void* z, const void* minv, void* g,
-                         int64_t B, int64_t r, int64_t L, double noise_var,
-                         double grad_scale, void* stream);
+                         const void* z, const void* minv, const int* pix_map,
+                         int64_t npix, void* g, int64_t B, int64_t r, int64_t L,
+                         double noise_var, double grad_scale, void* stream);
 
 /* ------------------------------------------------------------------------
  * Helpers on the data feed and the mean-volume solver.
@@ -192,7 +200,7 @@ int cov3d_mlp_backward(const void* params, const void* x, const void* gy,
  * class, recorded on the stream the kernel runs on (no reference
  * counterpart; used by the benchmark's roofline figures).
  *   kind: 0 interpolation, 1 spreading, 2 deconvolve+pad, 3 deconvolve+crop,
- *         4 rocFFT, 5 bin sort.
+ *         4 rocFFT, 5 bin sort, 6 sorted-order permutes.
  * ------------------------------------------------------------------------ */
 int cov3d_timing_enable(int on);
 int cov3d_timing_read(int kind, double* host_total_ms, int64_t* host_count);

@@ -135,6 +135,31 @@ def test_ml_grad_images_vs_closed_form(dev):
     assert rel_l2(g.cpu().numpy(), ref) < 1e-4
 
 
+@pytest.mark.parametrize("L", [16, 15])
+def test_half_plane_equals_full_plane_for_hermitian_data(dev, L):
+    """With exactly Hermitian projections, the half-plane Gram/gradient equal the full-plane ones."""
+    ml = pkg("ml_ops")
+    pf = pkg("projection_funcs")
+    r, B, s2 = 4, 5, 0.7
+    g = torch.Generator().manual_seed(70)
+    # c = fft2c of real fields, so c(-k) = conj(c(k)) on the grid (odd L) / off the Nyquist lines (even L)
+    c = pf.centered_fft2(torch.randn((r, B, L, L), generator=g, dtype=torch.float64)).to(torch.complex64)
+    yhat = pf.centered_fft2(torch.randn((B, L, L), generator=g, dtype=torch.float64)).to(torch.complex64)
+    ctf = torch.rand((B, L, L), generator=g)
+    ctf = 0.5 * (ctf + torch.flip(ctf, dims=(1, 2)).roll(shifts=(1 - L % 2, 1 - L % 2), dims=(1, 2)))
+    c, yhat, ctf = c.to(dev), yhat.to(dev), ctf.to(dev)
+    full = ml.ml_forward(c.reshape(r, B, -1), yhat, L, s2, ctf=ctf, precision=0, want_m=True)
+    pix = ml.half_plane_pixels(L, dev)
+    ch = c.reshape(r, B, -1)[:, :, pix.long()].contiguous()
+    half = ml.ml_forward(ch, yhat, L, s2, ctf=ctf, precision=0, want_m=True, pix_map=pix)
+    assert rel_l2(half["m"].cpu().numpy(), full["m"].cpu().numpy()) < 1e-5
+    assert rel_l2(half["cost"].cpu().numpy(), full["cost"].cpu().numpy()) < 1e-5
+    gf = ml.ml_grad_images(c.reshape(r, B, -1), yhat, full["z"], full["minv"], L, s2, 1.0 / B, ctf=ctf)
+    gh = ml.ml_grad_images(ch, yhat, half["z"], half["minv"], L, s2, 1.0 / B, ctf=ctf, pix_map=pix)
+    mult = torch.where(pix == (L // 2) * L + L // 2, 1.0, 2.0).to(dev)
+    assert rel_l2(gh.cpu().numpy(), (gf[:, :, pix.long()] * mult).cpu().numpy()) < 1e-5
+
+
 # ---------------------------------------------------------------- feed, CG, MLP
 
 

@@ -32,21 +32,23 @@ def _oracle_cost_grad(plan, vols, images, ctf, noise_var):
     return cost, grad
 
 
-@pytest.mark.parametrize("form,precision,tol", [("fused", 0, 1e-4), ("fused", 1, 1e-3), ("autograd", 1, 1e-4)])
-def test_ml_cost_and_gradient_vs_fp64_oracle(dev, form, precision, tol):
+@pytest.mark.parametrize("form,precision,half,tol", [("fused", 0, True, 1e-4), ("fused", 0, False, 1e-4),
+                                                     ("fused", 1, True, 1e-3), ("autograd", 1, True, 1e-4)])
+@pytest.mark.parametrize("L", [12, 13])
+def test_ml_cost_and_gradient_vs_fp64_oracle(dev, form, precision, half, tol, L):
     npl = pkg("nufft_plan")
     sgd = pkg("covar_sgd")
-    L, r, B = 12, 3, 6
+    r, B = 3, 6
     pts, vols64, images64, ctf64 = _batch(dev, L, r, B)
     plan = npl.NufftPlan((L,) * 3, batch_size=r, device=dev)
     plan.setpts(pts)
     vols = torch.tensor(vols64, dtype=torch.float32, device=dev, requires_grad=True)
     cost = sgd.cost_maximum_liklihood(vols, torch.tensor(images64, dtype=torch.float32, device=dev), plan,
                                       torch.tensor(ctf64, dtype=torch.float32, device=dev), 0.5, form=form,
-                                      precision=precision)
+                                      precision=precision, half_plane=half)
     cost.backward()
     ref_cost, ref_grad = _oracle_cost_grad(plan, vols64, images64, ctf64, 0.5)
-    print(f"{form}/{precision}: cost {abs(cost.item() - ref_cost) / abs(ref_cost):.2e} "
+    print(f"{form}/{precision}/half={half}/L={L}: cost {abs(cost.item() - ref_cost) / abs(ref_cost):.2e} "
           f"grad {rel_l2(vols.grad.cpu().numpy(), ref_grad):.2e}")
     assert abs(cost.item() - ref_cost) / abs(ref_cost) < tol
     assert rel_l2(vols.grad.cpu().numpy(), ref_grad) < tol
