@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-images", type=int, default=256)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a plain 1 GiB device copy (calibrates HBM byte counters)")
     return ap.parse_args()
 
 
@@ -164,6 +166,15 @@ def main():
     w = 7
     flops_per_launch = B * d * (w**3 * r * 4)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    # measured HBM bytes per launch of the same kernel (rocprofv3 FETCH_SIZE / WRITE_SIZE,
+    # scaled by a 1 GiB copy) from the committed profile, when it covers this workload
+    traffic = None
+    tj = ROOT / "profiles" / f"hbm_traffic_sgd_L{L}_r{r}_B{B}.json"
+    if tj.exists():
+        try:
+            traffic = round(json.loads(tj.read_text())[f"k_{dom}"]["traffic_GB"] * 1e9)
+        except (KeyError, ValueError):
+            traffic = None
     roof = {
         "kernel": f"k_{dom} (ES w=7, nf={nf}, r={r}, {B} images x {d} half-plane points)",
         "bound": "hbm",
@@ -171,11 +182,19 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": traffic,
+        "algorithmic_bytes": bytes_per_launch,
         "valu_tflops": round(flops_per_launch / (avg_ms * 1e-3) / 1e12, 2),
         "valu_frac_fp32_peak": round(flops_per_launch / (avg_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
         "avg_launch_ms": round(avg_ms, 3),
     }
+    if args.copy_probe:
+        src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy probe: 3 x 1 GiB device-to-device copies")
     cpu = None
     if not args.no_cpu:
         log(f"CPU baseline on {args.cpu_images} images")
