@@ -224,6 +224,7 @@ class SyntheticConfig:
     voltage_kv: float = 300.0
     cs_mm: float = 2.7
     amplitude_contrast: float = 0.1
+    amp_jitter: float = 0.0  # per-image clean-signal amplitude 1 + amp_jitter * N(0, 1), unknown to the solver
 
 
 def config_by_name(name: str) -> SyntheticConfig:
@@ -235,7 +236,7 @@ def config_by_name(name: str) -> SyntheticConfig:
     if name == "L128":
         return SyntheticConfig(128, 100000, 20, list(np.geomspace(10, 0.1, 20)), 1.5, 0.02, (0.5, 3.0), 0.0, 1003)
     if name == "L128-mean":
-        return SyntheticConfig(128, 100000, 0, [], 1.5, 0.02, (0.5, 3.0), 0.0, 1004)
+        return SyntheticConfig(128, 100000, 0, [], 1.5, 0.02, (0.5, 3.0), 0.0, 1004, amp_jitter=0.05)
     raise KeyError(name)
 
 
@@ -280,6 +281,7 @@ def make_synthetic(cfg: SyntheticConfig, device, n_images: Optional[int] = None,
          torch.full((n,), cfg.cs_mm, dtype=torch.float64), torch.full((n,), cfg.amplitude_contrast, dtype=torch.float64),
          torch.full((n,), cfg.pixel_size, dtype=torch.float64)], dim=1).to(torch.float32).to(dev)
     latents = (torch.randn((n, r), generator=gen) * eigvals.sqrt()).to(dev) if r > 0 else None
+    amp = (1.0 + cfg.amp_jitter * torch.randn(n, generator=gen)).to(dev) if cfg.amp_jitter > 0 else None
     vols = torch.cat([mean.unsqueeze(0), eigvecs], dim=0).to(dev)
     plan = NufftPlan((L,) * 3, batch_size=r + 1, device=dev)
     images = torch.empty((n, L, L), dtype=torch.float32, device=dev)
@@ -294,6 +296,8 @@ def make_synthetic(cfg: SyntheticConfig, device, n_images: Optional[int] = None,
         img = proj[:, 1:].mul(latents[i:j, :, None, None]).sum(1) if r > 0 else torch.zeros_like(proj[:, 0])
         if not subtract_mean:
             img = img + proj[:, 0]
+        if amp is not None:
+            img = img * amp[i:j, None, None]
         images[i:j] = img
         if filters is not None:
             filters[i:j] = f
@@ -304,4 +308,5 @@ def make_synthetic(cfg: SyntheticConfig, device, n_images: Optional[int] = None,
         j = min(i + 8 * batch, n)
         images[i:j] += math.sqrt(noise_var) * torch.randn((j - i, L, L), device=dev, generator=g)
     ds = CovarDataset(images, rots, noise_var, filters=filters, ctf_params=ctf_params)
-    return SyntheticData(ds, mean.to(dev), eigvecs.to(dev), eigvals.to(dev), latents)
+    return SyntheticData(ds, mean.to(dev), eigvecs.to(dev), eigvals.to(dev), latents,
+                         extra={} if amp is None else {"amplitudes": amp})

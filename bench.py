@@ -11,7 +11,12 @@ adjoint); the plain FP32 autograd form on the same GPU is reported beside it,
 with per-kernel HIP-event times, the roofline of the dominant kernel and a
 CPU baseline (the CPU port under oracle/, on a bounded sample).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+The second half of the metric is the mean-volume CG at L=128 (100k images,
+r=0, 5% amplitude jitter): time and iterations to relative residual 1e-6 for
+plain CG and for PCG with the learned diagonal (and the probed exact
+diagonal as a reference point), reported under ``mean_cg``.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cg]
 """
 
 from __future__ import annotations
@@ -48,6 +53,9 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-images", type=int, default=256)
+    ap.add_argument("--no-cg", action="store_true", help="skip the mean-CG leg")
+    ap.add_argument("--cg-images", type=int, default=100000)
+    ap.add_argument("--cg-cap", type=int, default=20000, help="iteration cap of each CG solve")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 1 GiB device copy (calibrates HBM byte counters)")
     return ap.parse_args()
@@ -87,6 +95,44 @@ def sgd_run(pkg, data, L, r, B, steps, warmup, form, timing=False):
         native.timing_enable(False)
     del trainer
     return dt, kern
+
+
+def mean_cg_run(pkg, L, n_images, dev, cap=20000):
+    """Mean reconstruction system at L (100k images, r=0, 5% amplitude jitter): plain CG vs PCG to 1e-6."""
+    import torch
+
+    dsm = importlib.import_module(pkg + ".dataset")
+    mean = importlib.import_module(pkg + ".mean")
+    pc = importlib.import_module(pkg + ".precond")
+    cfg = dsm.config_by_name("L128-mean") if L == 128 else dsm.SyntheticConfig(L, n_images, 0, [], 1.5, 0.02,
+                                                                                (0.5, 3.0), amp_jitter=0.05)
+    t0 = time.perf_counter()
+    data = dsm.make_synthetic(cfg, dev, n_images=n_images, subtract_mean=False)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    lhs, rhs = mean.backproject_lhs_rhs(data.dataset, batch_size=1024)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    log(f"mean CG: {n_images} images generated in {t1 - t0:.1f} s, back-projected in {t2 - t1:.2f} s")
+    reg_lhs = mean.regularize_lhs(lhs)
+    try:
+        model = pc.load_default(dev)
+    except FileNotFoundError as e:
+        log(f"mean CG: {e}; learned leg skipped")
+        model = None
+    # one untimed pass creates the rocFFT plans of the Toeplitz operator and warms the MLP
+    pc.cg_comparison(reg_lhs, rhs, L, model, data.dataset.noise_var, maxiter=5, with_probe=False)
+    res = pc.cg_comparison(reg_lhs, rhs, L, model, data.dataset.noise_var, rtol=1e-6, maxiter=cap)
+    res["workload"] = (f"padded mean system (2L)^3 Toeplitz CG, L={L}, {n_images} images, r=0, 5% amplitude "
+                       f"jitter, nearest back-projection us=2, rtol 1e-6, x0 = direct estimate")
+    res["backprojection_s"] = round(t2 - t1, 3)
+    if model is not None:
+        res["time_speedup_learned_vs_plain"] = round(res["plain"]["seconds"] / res["learned"]["seconds"], 3)
+        res["iteration_ratio_learned_vs_plain"] = round(res["learned"]["iterations"] /
+                                                        max(res["plain"]["iterations"], 1), 3)
+    del data, lhs, rhs, reg_lhs
+    torch.cuda.empty_cache()
+    return res
 
 
 def cpu_baseline(L, r, B_cpu, noise_var=1.0):
@@ -195,6 +241,14 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         log("copy probe: 3 x 1 GiB device-to-device copies")
+    cg = None
+    if not args.no_cg:
+        del data
+        torch.cuda.empty_cache()
+        log("mean CG leg (plain CG vs learned-diagonal PCG)")
+        cg = mean_cg_run(pkg, L, args.cg_images, dev, args.cg_cap)
+        log(f"mean CG: plain {cg['plain']['iterations']} it / {cg['plain']['seconds']} s; learned "
+            f"{cg.get('learned', {}).get('iterations')} it / {cg.get('learned', {}).get('seconds')} s")
     cpu = None
     if not args.no_cpu:
         log(f"CPU baseline on {args.cpu_images} images")
@@ -221,6 +275,7 @@ def main():
         "speedup_vs_plain": round(ips / plain["value"], 3) if plain else None,
         "kernels_ms": {k: round(v["avg_ms"], 3) if v["avg_ms"] else None for k, v in kern.items()},
         "roofline": roof,
+        "mean_cg": cg,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -36,8 +36,8 @@ def test_error_path_returns_status_and_message():
     assert b"type" in lib.cov3d_last_error()
     with pytest.raises(RuntimeError, match="type must be"):
         native.check(rc)
-    assert lib.cov3d_ml_forward(None, None, None, None, 4, 40, 8, 1.0, 0, 1, None, None, None, None, None,
-                                None) == 1
+    assert lib.cov3d_ml_forward(None, None, None, None, None, 0, 4, 40, 8, 1.0, 0, 1, None, None, None, None,
+                                None, None) == 1
     assert b"rank" in lib.cov3d_last_error()
     assert lib.cov3d_nufft_setpts(None, 10, None, None, None, None) == 1
 

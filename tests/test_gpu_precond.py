@@ -1,0 +1,79 @@
+"""Learned diagonal preconditioner on the GPU: training by the native kernels,
+their gradients against the FP64 oracle, and the gain of the trained network
+over plain CG on acquisitions it was not trained on."""
+
+import numpy as np
+import pytest
+import torch
+
+from tests.conftest import oracle, pkg
+
+pytestmark = pytest.mark.gpu
+
+
+def test_mlp_gradient_matches_oracle_finite_differences(dev):
+    """Native backward vs central differences of the FP64 oracle forward (random directions)."""
+    pc = pkg("precond")
+    o = oracle()
+    n, n_in = 300, pc.N_FEATURES
+    mlp = pc.DiagonalMLP(n_in, device=dev, seed=4)
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal((n, n_in))
+    gy = rng.standard_normal(n)
+    grad = mlp.backward(torch.tensor(x, dtype=torch.float32, device=dev),
+                        torch.tensor(gy, dtype=torch.float32, device=dev)).cpu().numpy().astype(np.float64)
+    p = mlp.params.cpu().numpy().astype(np.float64)
+    f = lambda q: float(gy @ o.mlp_forward(q, x))
+    for k in range(4):
+        d = rng.standard_normal(p.size)
+        d /= np.linalg.norm(d)
+        h = 1e-4
+        fd = (f(p + h * d) - f(p - h * d)) / (2 * h)
+        assert abs(grad @ d - fd) < 1e-4 * np.abs(grad).sum() / np.sqrt(p.size) + 1e-4 * abs(fd), (k, grad @ d, fd)
+
+
+def test_training_on_gpu_fits_probed_diagonals(dev):
+    pc = pkg("precond")
+    X, Y = pc.training_set(4, dev, L_choices=(16, 24), rows_per_dataset=2048, seed0=777)
+    assert X.shape == (4 * 2048, pc.N_FEATURES) and bool(torch.isfinite(X).all()) and bool(torch.isfinite(Y).all())
+    model = pc.DiagonalMLP(pc.N_FEATURES, device=dev, seed=1)
+    log = []
+    model.fit(X, Y, steps=400, batch=2048, log=log)
+    print("normalised MSE", log[0], log[-1])
+    assert log[-1] < 0.2 * log[0]
+    rms = float((model.predict_log_diag(X) - Y).pow(2).mean().sqrt())
+    assert rms < 0.25 * float(Y.std())
+
+
+def _held_out_system(dev, L, n_images, seed):
+    """Padded mean system of synthetic images whose poses/CTFs are outside the training seeds."""
+    ds = pkg("dataset")
+    mean = pkg("mean")
+    cfg = ds.SyntheticConfig(L, n_images, 0, [], 1.5, 0.05, (0.5, 3.0), seed=seed, amp_jitter=0.05)
+    data = ds.make_synthetic(cfg, dev, subtract_mean=False)
+    lhs, rhs = mean.backproject_lhs_rhs(data.dataset, batch_size=1024)
+    return mean.regularize_lhs(lhs), rhs, data.dataset.noise_var
+
+
+@pytest.mark.parametrize("L,n_images", [(32, 8000), (64, 30000)])
+def test_trained_network_beats_plain_cg(dev, L, n_images):
+    """The committed network vs plain CG on acquisitions outside its training seeds.
+
+    The padded mean system converges slowly in its weakly sampled cube corners,
+    so the comparison is made at rtol 1e-4 with a 6000-iteration cap.
+    """
+    pc = pkg("precond")
+    model = pc.load_default(dev)
+    seed = 910_000 + L  # training acquisitions use seeds 5001 + 7k
+    lhs, rhs, noise_var = _held_out_system(dev, L, n_images, seed)
+    cap = 6000
+    res = pc.cg_comparison(lhs, rhs, L, model, noise_var=noise_var, rtol=1e-4, maxiter=cap)
+    print({k: v for k, v in res.items()})
+    plain, learned, probe = res["plain"], res["learned"], res["probe"]
+    assert learned["preconditioned"] and learned["fallback"] is None
+    assert learned["converged"] and learned["iterations"] < cap
+    # plain CG judges the final residual of the preconditioned solve
+    assert learned["final_residual"] < 3e-4
+    # the gain: at most half of plain CG's iterations (capped), within 2x of the exact Jacobi diagonal
+    assert learned["iterations"] <= 0.5 * plain["iterations"]
+    assert learned["iterations"] <= 2.0 * probe["iterations"] + 10
