@@ -377,15 +377,17 @@ __global__ void k_deconv_crop(const Cplx<T>* __restrict__ fine, T* __restrict__ 
 // order) and the sorted layouts the bin kernels use.  Both walk j in input
 // order so that the c[t][j] side is coalesced; the sorted side is a
 // contiguous run per point.
-//   to_sorted:   cs[spos[j]][t]        = c[t][j]      (spreading input)
+//   to_sorted:   cs[spos[j]][t]        = c[t][j]      (spreading input, row stride ldc >= ntrans,
+//                                                      zero past ntrans)
 //   from_sorted: c[t][j] = cs[t / TC][spos[j]][t % TC] (interpolation output)
 template <typename T>
 __global__ void k_to_sorted(const Cplx<T>* __restrict__ c, Cplx<T>* __restrict__ cs,
-                            const int* __restrict__ spos, int64_t M, int ntrans) {
+                            const int* __restrict__ spos, int64_t M, int ntrans, int ldc) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= M) return;
-  Cplx<T>* dst = cs + (int64_t)spos[j] * ntrans;
+  Cplx<T>* dst = cs + (int64_t)spos[j] * ldc;
   for (int t = 0; t < ntrans; ++t) dst[t] = c[(int64_t)t * M + j];
+  for (int t = ntrans; t < ldc; ++t) dst[t] = Cplx<T>{(T)0, (T)0};
 }
 
 template <typename T>
@@ -785,6 +787,112 @@ __global__ void __launch_bounds__(kGatherBlock)
   }
 }
 
+// ES kernel in FP32 on the native v_sqrt_f32 / v_exp_f32 (about 1 ulp each):
+// exp(beta (sqrt(s) - 1)) = 2^(beta log2(e) (sqrt(s) - 1)), beta_l2 = beta log2(e).
+__device__ __forceinline__ float es_eval_native(float t, float c, float beta_l2) {
+  const float s = fmaxf(1.f - c * t * t, 0.f);
+  return __builtin_amdgcn_exp2f(beta_l2 * (__builtin_amdgcn_sqrtf(s) - 1.f));
+}
+
+__device__ __forceinline__ float lane_bcast(float v, int src) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+
+// Type-1 spreading as a gather without LDS staging (FP32).  Ownership is as
+// in k_spread_gather: one 64-lane workgroup per 4^3 bin, one cell per lane,
+// TC transforms in registers, each cell written once (deterministic, no
+// atomics).  The points of each neighbouring bin are taken 64 at a time:
+// every lane tests its own point against the block and a ballot lists the
+// hits.  For each hit, the point's coordinates are broadcast with readlane;
+// lanes 0..11 evaluate the 12 distinct kernel weights of the block (4 cell
+// offsets x 3 axes) and every lane picks its three with bpermute; the
+// point's TC values are read at a wave-uniform address (scalar loads) and
+// enter the FMAs as scalar operands.  The sorted values have row stride ldc =
+// gridDim.y * TC with zeros past ntrans, so every load is unconditional.  Nothing occupies LDS, so occupancy is
+// bounded by registers only.
+template <int TC>
+__global__ void __launch_bounds__(kGatherBlock)
+    k_spread_sc(const Cplx<float>* __restrict__ cs, Cplx<float>* __restrict__ fine, BinGeom g,
+                const int* __restrict__ bin_start, const int* __restrict__ bin_count,
+                const float* __restrict__ sv0, const float* __restrict__ sv1,
+                const float* __restrict__ sv2, int ntrans, int ldc, int W, float kc, float beta_l2) {
+  const int bin = xcd_remap(blockIdx.x, gridDim.x);
+  const int t0 = blockIdx.y * TC;
+  const int lane = threadIdx.x;
+  int b0, b1, b2;
+  bin_coords(g, bin, b0, b1, b2);
+  const int c0 = lane >> 4, c1 = (lane >> 2) & 3, c2 = lane & 3;
+  const float halfw = 0.5f * (float)W;
+  const int R = (W + 2 * kBinEdge - 1) / (2 * kBinEdge);
+  // weight lanes: axis wa = lane / 4 (lanes >= 12 duplicate axis 2), cell offset wc = lane % 4
+  const int wa = min(lane >> 2, 2), wc = lane & 3;
+  float ax[TC], ay[TC];
+#pragma unroll
+  for (int tt = 0; tt < TC; ++tt) ax[tt] = ay[tt] = 0.f;
+
+  for (int d0 = -R; d0 <= R; ++d0)
+    for (int d1 = -R; d1 <= R; ++d1)
+      for (int d2 = -R; d2 <= R; ++d2) {
+        int n0 = b0 + d0, n1 = b1 + d1, n2 = b2 + d2, sh0 = 0, sh1 = 0, sh2 = 0;
+        if (n0 < 0) { n0 += g.nb[0]; sh0 = -g.nf[0]; }
+        if (n0 >= g.nb[0]) { n0 -= g.nb[0]; sh0 = g.nf[0]; }
+        if (n1 < 0) { n1 += g.nb[1]; sh1 = -g.nf[1]; }
+        if (n1 >= g.nb[1]) { n1 -= g.nb[1]; sh1 = g.nf[1]; }
+        if (n2 < 0) { n2 += g.nb[2]; sh2 = -g.nf[2]; }
+        if (n2 >= g.nb[2]) { n2 -= g.nb[2]; sh2 = g.nf[2]; }
+        const int nbin = bin_index(g, n0, n1, n2);
+        const int start = bin_start[nbin];
+        const int end = start + bin_count[nbin];
+        // first cell of this block in the neighbour's super-bin frame (unwrapped)
+        const int q0 = kBinEdge * b0 - (n0 >> 1) * kSuperEdge - sh0;
+        const int q1 = kBinEdge * b1 - (n1 >> 1) * kSuperEdge - sh1;
+        const int q2 = kBinEdge * b2 - (n2 >> 1) * kSuperEdge - sh2;
+        const int qa = wa == 0 ? q0 : (wa == 1 ? q1 : q2);
+        for (int base = start; base < end; base += kGatherBlock) {
+          float v0 = 0.f, v1 = 0.f, v2 = 0.f;
+          bool hit = false;
+          if (base + lane < end) {
+            v0 = sv0[base + lane];
+            v1 = sv1[base + lane];
+            v2 = sv2[base + lane];
+            const int j0 = (int)ceilf(v0 - halfw), j1 = (int)ceilf(v1 - halfw), j2 = (int)ceilf(v2 - halfw);
+            hit = !(q0 > j0 + W - 1 || q0 + 3 < j0 || q1 > j1 + W - 1 || q1 + 3 < j1 || q2 > j2 + W - 1 ||
+                    q2 + 3 < j2);
+          }
+          unsigned long long mask = __ballot(hit);
+          while (mask) {
+            const int q = __builtin_ctzll(mask);
+            mask &= mask - 1;
+            // all TC values at once (rows are zero-padded to ldc): wide scalar loads issued
+            // before the weights are formed, one wait
+            const Cplx<float>* src = cs + (int64_t)(base + q) * ldc + t0;
+            Cplx<float> v[TC];
+#pragma unroll
+            for (int tt = 0; tt < TC; ++tt) v[tt] = src[tt];
+            const float p0 = lane_bcast(v0, q), p1 = lane_bcast(v1, q), p2 = lane_bcast(v2, q);
+            const float pa = wa == 0 ? p0 : (wa == 1 ? p1 : p2);
+            const int j0a = (int)ceilf(pa - halfw);
+            const int xa = qa + wc - j0a;
+            const float wl = (xa >= 0 && xa < W) ? es_eval_native((float)(j0a + xa) - pa, kc, beta_l2) : 0.f;
+            const float w = __shfl(wl, c0) * __shfl(wl, 4 + c1) * __shfl(wl, 8 + c2);
+#pragma unroll
+            for (int tt = 0; tt < TC; ++tt) {
+              ax[tt] += w * v[tt].x;
+              ay[tt] += w * v[tt].y;
+            }
+          }
+        }
+      }
+  const int g0 = kBinEdge * b0 + c0, g1 = kBinEdge * b1 + c1, g2 = kBinEdge * b2 + c2;
+  const int64_t nfine = (int64_t)g.nf[0] * g.nf[1] * g.nf[2];
+  const int64_t off = ((int64_t)g0 * g.nf[1] + g1) * g.nf[2] + g2;
+#pragma unroll
+  for (int tt = 0; tt < TC; ++tt) {
+    const int t = t0 + tt;
+    if (t < ntrans) fine[t * nfine + off] = Cplx<float>{ax[tt], ay[tt]};
+  }
+}
+
 // Type-1 spreading, point-driven ("tap-parallel"): one workgroup per
 // (super-bin subproblem, transform chunk) accumulates into an LDS tile
 // [TC][re|im][cell] of the 8^3 super-bin plus halo.  Each wavefront takes one
@@ -905,7 +1013,7 @@ struct Plan {
   int64_t ntrans = 1;
   int w = 7;
   int tc = 2;  // transforms per interpolation workgroup
-  int spread_mode = 0;  // 0 = output-driven gather, 1 = point-driven LDS tiles
+  int spread_mode = 0;  // 0 = LDS-staged gather, 1 = point-driven LDS tiles, 2 = LDS-free gather (FP32)
   int interp_mode = 0;  // 0 = thread per point, 1 = wavefront per point (tap-parallel)
   int max_tc_gather = 20;  // transforms per gather workgroup
   int tc_spread = 2;    // transforms per spreading workgroup (tile mode)
@@ -927,7 +1035,8 @@ struct Plan {
   int* pt_pos = nullptr;
   int* sidx = nullptr;
   int* spos = nullptr;
-  void* csort = nullptr;  // sorted-order staging of point values (M * ntrans complex)
+  void* csort = nullptr;  // sorted-order staging of point values (M * ldc_cap complex)
+  int ldc_cap = 0;        // row capacity of csort
   void* sv[3] = {nullptr, nullptr, nullptr};
   rocfft_plan fft_fwd = nullptr, fft_bwd = nullptr;
   rocfft_execution_info info = nullptr;
@@ -1025,15 +1134,28 @@ static int run_interp(Plan* p, void* c_out, int ntrans_run, hipStream_t st) {
 
 template <typename T, int TC>
 static int launch_spread(Plan* p, const void* c_in, int ntrans_run, hipStream_t st) {
+  const bool sc = sizeof(T) == 4 && p->spread_mode == 2;
+  const int ldc = sc ? (int)ceil_div(ntrans_run, TC) * TC : ntrans_run;
+  COV3D_CHECK_ARG(ldc <= p->ldc_cap, "sorted row stride %d exceeds %d", ldc, p->ldc_cap);
   {
     ScopedTiming tm(kTPermute, st);
     hipLaunchKernelGGL((k_to_sorted<T>), dim3((unsigned)ceil_div(p->M, kBlock)), dim3(kBlock), 0, st,
-                       (const Cplx<T>*)c_in, (Cplx<T>*)p->csort, p->spos, p->M, ntrans_run);
+                       (const Cplx<T>*)c_in, (Cplx<T>*)p->csort, p->spos, p->M, ntrans_run, ldc);
     COV3D_LAUNCH_CHECK();
   }
   dim3 grid((unsigned)p->nbins, (unsigned)ceil_div(ntrans_run, TC));
   const T kc = (T)(4.0 / ((double)p->w * p->w));
   ScopedTiming tm(kTSpread, st);
+  if constexpr (sizeof(T) == 4) {
+    if (p->spread_mode == 2) {
+      hipLaunchKernelGGL((k_spread_sc<TC>), grid, dim3(kGatherBlock), 0, st, (const Cplx<float>*)p->csort,
+                         (Cplx<float>*)p->fine, p->geom, p->bin_start, p->bin_count, (const float*)p->sv[0],
+                         (const float*)p->sv[1], (const float*)p->sv[2], ntrans_run, ldc, p->w, (float)kc,
+                         (float)(p->beta * 1.4426950408889634));
+      COV3D_LAUNCH_CHECK();
+      return 0;
+    }
+  }
   hipLaunchKernelGGL((k_spread_gather<T, TC>), grid, dim3(kGatherBlock), 0, st, (const Cplx<T>*)p->csort,
                      (Cplx<T>*)p->fine, p->geom, p->bin_start, p->bin_count, (const T*)p->sv[0],
                      (const T*)p->sv[1], (const T*)p->sv[2], p->M, ntrans_run, p->w, kc,
@@ -1082,15 +1204,29 @@ static int dispatch_spread_tiles_w(Plan* p, int ntrans_run, hipStream_t st) {
   }
 }
 
-// Transforms per gather workgroup: at most p->max_tc_gather (<= 20) in
-// registers, split evenly.
+// Transforms per gather workgroup: at most max_tc (<= 20) in registers,
+// split evenly over ceil(n / max_tc) chunks, rounded up to an instantiated TC.
+static int gather_tc(int n, int max_tc) {
+  const int chunks = (int)ceil_div(n, max_tc);
+  const int need = (int)ceil_div(n, chunks);
+  for (int tc : {1, 2, 4, 8, 10, 16})
+    if (need <= tc) return tc;
+  return 20;
+}
+
+// Row stride of the sorted spreading input for n transforms: chunks * TC.
+static int gather_ldc(int n, int max_tc) {
+  const int tc = gather_tc(n, max_tc);
+  return (int)ceil_div(n, tc) * tc;
+}
+
 template <typename T>
 static int run_spread(Plan* p, const void* c_in, int ntrans_run, hipStream_t st) {
   if (p->spread_mode == 1) {
     {
       ScopedTiming tm(kTPermute, st);
       hipLaunchKernelGGL((k_to_sorted<T>), dim3((unsigned)ceil_div(p->M, kBlock)), dim3(kBlock), 0, st,
-                         (const Cplx<T>*)c_in, (Cplx<T>*)p->csort, p->spos, p->M, ntrans_run);
+                         (const Cplx<T>*)c_in, (Cplx<T>*)p->csort, p->spos, p->M, ntrans_run, ntrans_run);
       COV3D_LAUNCH_CHECK();
     }
     if (p->dtype == 1 || p->tc_spread == 1) return dispatch_spread_tiles_w<T, 1>(p, ntrans_run, st);
@@ -1100,15 +1236,15 @@ static int run_spread(Plan* p, const void* c_in, int ntrans_run, hipStream_t st)
     }
     return dispatch_spread_tiles_w<T, 1>(p, ntrans_run, st);
   }
-  const int chunks = (int)ceil_div(ntrans_run, p->max_tc_gather);
-  const int need = (int)ceil_div(ntrans_run, chunks);
-  if (need <= 1) return launch_spread<T, 1>(p, c_in, ntrans_run, st);
-  if (need <= 2) return launch_spread<T, 2>(p, c_in, ntrans_run, st);
-  if (need <= 4) return launch_spread<T, 4>(p, c_in, ntrans_run, st);
-  if (need <= 8) return launch_spread<T, 8>(p, c_in, ntrans_run, st);
-  if (need <= 10) return launch_spread<T, 10>(p, c_in, ntrans_run, st);
-  if (need <= 16) return launch_spread<T, 16>(p, c_in, ntrans_run, st);
-  return launch_spread<T, 20>(p, c_in, ntrans_run, st);
+  switch (gather_tc(ntrans_run, p->max_tc_gather)) {
+    case 1: return launch_spread<T, 1>(p, c_in, ntrans_run, st);
+    case 2: return launch_spread<T, 2>(p, c_in, ntrans_run, st);
+    case 4: return launch_spread<T, 4>(p, c_in, ntrans_run, st);
+    case 8: return launch_spread<T, 8>(p, c_in, ntrans_run, st);
+    case 10: return launch_spread<T, 10>(p, c_in, ntrans_run, st);
+    case 16: return launch_spread<T, 16>(p, c_in, ntrans_run, st);
+    default: return launch_spread<T, 20>(p, c_in, ntrans_run, st);
+  }
 }
 
 static int env_int(const char* name, int dflt) {
@@ -1192,8 +1328,9 @@ static int setpts_impl(Plan* p, int64_t M, const T* x, const T* y, const T* z, h
     COV3D_HIP(hipMalloc(&p->pt_pos, Ma * sizeof(int)));
     COV3D_HIP(hipMalloc(&p->sidx, Ma * sizeof(int)));
     COV3D_HIP(hipMalloc(&p->spos, Ma * sizeof(int)));
-    // interpolation writes whole chunks of up to 4 transforms: round ntrans up to 4
-    COV3D_HIP(hipMalloc(&p->csort, Ma * ceil_div(p->ntrans, 4) * 4 * (p->dtype == 1 ? 16 : 8)));
+    // interpolation writes whole chunks of up to 4 transforms (ntrans rounded up to 4); the
+    // spreading input uses row stride gather_ldc(n) for any n <= ntrans
+    COV3D_HIP(hipMalloc(&p->csort, Ma * p->ldc_cap * (p->dtype == 1 ? 16 : 8)));
     for (int k = 0; k < 3; ++k) COV3D_HIP(hipMalloc(&p->sv[k], Ma * sizeof(T)));
     const int64_t ca = cap > 8 ? cap : 8;
     COV3D_HIP(hipMalloc(&p->sub_sb, ca * sizeof(int)));
@@ -1269,9 +1406,16 @@ static int make_plan(int type, const int64_t* n, int iflag, int64_t ntrans, doub
   p->beta = es_beta(p->w);
   const int tc_env = env_int("COV3D_NUFFT_TC", 2);
   p->tc = dtype == 1 ? 1 : (tc_env == 1 || tc_env == 4 ? tc_env : 2);
-  p->spread_mode = env_int("COV3D_SPREAD_MODE", 0) == 1 ? 1 : 0;
+  {
+    // 2 = LDS-free gather (FP32 default), 0 = LDS-staged gather (FP64 default), 1 = LDS atomic tiles
+    const int sm = env_int("COV3D_SPREAD_MODE", p->dtype == 0 ? 2 : 0);
+    p->spread_mode = (sm == 1 || (sm == 2 && p->dtype == 0)) ? sm : 0;
+  }
   p->interp_mode = env_int("COV3D_INTERP_MODE", 0) == 1 ? 1 : 0;
   p->max_tc_gather = std::min(std::max(env_int("COV3D_GATHER_TC", 20), 1), 20);
+  p->ldc_cap = (int)ceil_div(ntrans, 4) * 4;
+  for (int n_run = 1; n_run <= ntrans; ++n_run)
+    p->ldc_cap = std::max(p->ldc_cap, gather_ldc(n_run, p->max_tc_gather));
   const int tcs_env = env_int("COV3D_SPREAD_TC", 2);
   p->tc_spread = dtype == 1 ? 1 : (tcs_env == 1 || tcs_env == 4 ? tcs_env : 2);
   const size_t csize = dtype == 1 ? 16 : 8;
