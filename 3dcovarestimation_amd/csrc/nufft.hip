@@ -155,6 +155,13 @@ __device__ __forceinline__ T es_eval(T t, T c, T beta) {
   return exp(beta * (sqrt(s) - (T)1));
 }
 
+// ES kernel in FP32 on the native v_sqrt_f32 / v_exp_f32 (about 1 ulp each):
+// exp(beta (sqrt(s) - 1)) = 2^(beta log2(e) (sqrt(s) - 1)), beta_l2 = beta log2(e).
+__device__ __forceinline__ float es_eval_native(float t, float c, float beta_l2) {
+  const float s = fmaxf(1.f - c * t * t, 0.f);
+  return __builtin_amdgcn_exp2f(beta_l2 * (__builtin_amdgcn_sqrtf(s) - 1.f));
+}
+
 __global__ void k_fill_int(int* p, int64_t n, int v) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -417,13 +424,67 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblk) {
   return (bid & 7) * (nblk >> 3) + (bid >> 3);
 }
 
+// W kernel weights of a point at coordinate u; FP32 uses the native exp/sqrt.
 template <typename T, int W>
 __device__ __forceinline__ void stencil_weights(T u, T c, T beta, int& l0, T* w) {
   const T halfw = (T)(0.5 * W);
   int j0 = (int)ceil(u - halfw);
+  if constexpr (sizeof(T) == 4) {
+    const float beta_l2 = beta * 1.4426950408889634f;
 #pragma unroll
-  for (int a = 0; a < W; ++a) w[a] = es_eval<T>((T)(j0 + a) - u, c, beta);
+    for (int a = 0; a < W; ++a) w[a] = es_eval_native((float)(j0 + a) - u, c, beta_l2);
+  } else {
+#pragma unroll
+    for (int a = 0; a < W; ++a) w[a] = es_eval<T>((T)(j0 + a) - u, c, beta);
+  }
   l0 = j0 + W / 2;
+}
+
+// Fine-grid index of an unwrapped coordinate within one period of [0, n).
+__device__ __forceinline__ int wrap1(int m, int n) {
+  m += m < 0 ? n : 0;
+  return m >= n ? m - n : m;
+}
+
+// Stages the (8 + W)^3 halo tile of one super-bin, TC transforms from t0, into
+// LDS as [cell][TC].  Loads are issued kStageGroup iterations at a time into
+// registers before any is stored, so a thread has up to kStageGroup * TC
+// global loads in flight instead of one iteration's.
+constexpr int kStageGroup = 7;
+
+template <typename T, int W, int TC>
+__device__ __forceinline__ void stage_tile(const Cplx<T>* __restrict__ fine, Cplx<T>* tile, const BinGeom& g,
+                                           int o0, int o1, int o2, int t0, int ntrans) {
+  constexpr int E = kSuperEdge + W;
+  constexpr int ncell = E * E * E;
+  constexpr int NIT = (ncell + kBlock - 1) / kBlock;
+  const int64_t nfine = (int64_t)g.nf[0] * g.nf[1] * g.nf[2];
+#pragma unroll
+  for (int it0 = 0; it0 < NIT; it0 += kStageGroup) {
+    Cplx<T> v[kStageGroup][TC];
+#pragma unroll
+    for (int k = 0; k < kStageGroup; ++k) {
+      const int cidx = threadIdx.x + (it0 + k) * kBlock;
+      if (it0 + k < NIT && cidx < ncell) {
+        const int e2 = cidx % E;
+        const int e1 = (cidx / E) % E;
+        const int e0 = cidx / (E * E);
+        const int64_t off = ((int64_t)wrap1(o0 + e0, g.nf[0]) * g.nf[1] + wrap1(o1 + e1, g.nf[1])) * g.nf[2] +
+                            wrap1(o2 + e2, g.nf[2]);
+#pragma unroll
+        for (int tt = 0; tt < TC; ++tt)
+          v[k][tt] = t0 + tt < ntrans ? fine[(t0 + tt) * nfine + off] : Cplx<T>{(T)0, (T)0};
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kStageGroup; ++k) {
+      const int cidx = threadIdx.x + (it0 + k) * kBlock;
+      if (it0 + k < NIT && cidx < ncell) {
+#pragma unroll
+        for (int tt = 0; tt < TC; ++tt) tile[cidx * TC + tt] = v[k][tt];
+      }
+    }
+  }
 }
 
 // Type-2 interpolation: c[t, j] = sum_taps phi * fine[t, tap].
@@ -448,23 +509,8 @@ __global__ void __launch_bounds__(kBlock)
   constexpr int E = kSuperEdge + W;
   constexpr int ncell = E * E * E;
   const int o0 = s0 * kSuperEdge - W / 2, o1 = s1 * kSuperEdge - W / 2, o2 = s2 * kSuperEdge - W / 2;
-  const int64_t nfine = (int64_t)g.nf[0] * g.nf[1] * g.nf[2];
-  for (int cidx = threadIdx.x; cidx < ncell; cidx += blockDim.x) {
-    int e2 = cidx % E;
-    int e1 = (cidx / E) % E;
-    int e0 = cidx / (E * E);
-    int m0 = ((o0 + e0) % g.nf[0] + g.nf[0]) % g.nf[0];
-    int m1 = ((o1 + e1) % g.nf[1] + g.nf[1]) % g.nf[1];
-    int m2 = ((o2 + e2) % g.nf[2] + g.nf[2]) % g.nf[2];
-    int64_t off = ((int64_t)m0 * g.nf[1] + m1) * g.nf[2] + m2;
-#pragma unroll
-    for (int tt = 0; tt < TC; ++tt) {
-      int t = t0 + tt;
-      Cplx<T> v = {(T)0, (T)0};
-      if (t < ntrans) v = fine[t * nfine + off];
-      tile[cidx * TC + tt] = v;
-    }
-  }
+  // halo cells lie within one period of the grid (nf >= 2W), so one conditional wrap suffices
+  stage_tile<T, W, TC>(fine, tile, g, o0, o1, o2, t0, ntrans);
   __syncthreads();
   const int p0 = sd.sub_pt[sub];
   const int np = sd.sub_n[sub];
@@ -480,30 +526,65 @@ __global__ void __launch_bounds__(kBlock)
     T ax[TC], ay[TC];
 #pragma unroll
     for (int tt = 0; tt < TC; ++tt) ax[tt] = ay[tt] = (T)0;
-    for (int a0 = 0; a0 < W; ++a0) {
-#pragma unroll
-      for (int a1 = 0; a1 < W; ++a1) {
-        const T w01 = w0[a0] * w1[a1];
+    if constexpr (TC % 2 == 0 && sizeof(T) == 4 && W <= 8) {
+      // Stencil rows (a0, a1) in order, double-buffered: the W cells of the next row are
+      // read from LDS while the FMAs of the current row run (occupancy is bounded by the
+      // tile, so registers are spent on reads in flight).
+      constexpr int H = TC / 2;
+      const float4* t4 = reinterpret_cast<const float4*>(tile);
+      float4 buf[2][W * H];
+      auto load_row = [&](int a0, int a1, float4* b) {
         const int row = ((l0 + a0) * E + (l1 + a1)) * E + l2;
 #pragma unroll
+        for (int a2 = 0; a2 < W; ++a2)
+#pragma unroll
+          for (int h = 0; h < H; ++h) b[a2 * H + h] = t4[(row + a2) * H + h];
+      };
+      load_row(0, 0, buf[0]);
+#pragma unroll
+      for (int k = 0; k < W * W; ++k) {
+        const int a0 = k / W, a1 = k % W;
+        if (k + 1 < W * W) load_row((k + 1) / W, (k + 1) % W, buf[(k + 1) & 1]);
+        const float w01 = w0[a0] * w1[a1];
+#pragma unroll
         for (int a2 = 0; a2 < W; ++a2) {
-          const T ww = w01 * w2[a2];
-          if constexpr (TC % 2 == 0 && sizeof(T) == 4) {
-            const float4* cell = reinterpret_cast<const float4*>(tile + (row + a2) * TC);
+          const float ww = w01 * w2[a2];
 #pragma unroll
-            for (int h = 0; h < TC / 2; ++h) {
-              const float4 v = cell[h];
-              ax[2 * h] += ww * v.x;
-              ay[2 * h] += ww * v.y;
-              ax[2 * h + 1] += ww * v.z;
-              ay[2 * h + 1] += ww * v.w;
-            }
-          } else {
+          for (int h = 0; h < H; ++h) {
+            const float4 v = buf[k & 1][a2 * H + h];
+            ax[2 * h] += ww * v.x;
+            ay[2 * h] += ww * v.y;
+            ax[2 * h + 1] += ww * v.z;
+            ay[2 * h + 1] += ww * v.w;
+          }
+        }
+      }
+    } else {
+      for (int a0 = 0; a0 < W; ++a0) {
 #pragma unroll
-            for (int tt = 0; tt < TC; ++tt) {
-              const Cplx<T> v = tile[(row + a2) * TC + tt];
-              ax[tt] += ww * v.x;
-              ay[tt] += ww * v.y;
+        for (int a1 = 0; a1 < W; ++a1) {
+          const T w01 = w0[a0] * w1[a1];
+          const int row = ((l0 + a0) * E + (l1 + a1)) * E + l2;
+#pragma unroll
+          for (int a2 = 0; a2 < W; ++a2) {
+            const T ww = w01 * w2[a2];
+            if constexpr (TC % 2 == 0 && sizeof(T) == 4) {
+              const float4* cell = reinterpret_cast<const float4*>(tile + (row + a2) * TC);
+#pragma unroll
+              for (int h = 0; h < TC / 2; ++h) {
+                const float4 v = cell[h];
+                ax[2 * h] += ww * v.x;
+                ay[2 * h] += ww * v.y;
+                ax[2 * h + 1] += ww * v.z;
+                ay[2 * h + 1] += ww * v.w;
+              }
+            } else {
+#pragma unroll
+              for (int tt = 0; tt < TC; ++tt) {
+                const Cplx<T> v = tile[(row + a2) * TC + tt];
+                ax[tt] += ww * v.x;
+                ay[tt] += ww * v.y;
+              }
             }
           }
         }
@@ -513,6 +594,109 @@ __global__ void __launch_bounds__(kBlock)
     Cplx<T>* dst = c + ((int64_t)blockIdx.y * M + p) * TC;
 #pragma unroll
     for (int tt = 0; tt < TC; ++tt) dst[tt] = Cplx<T>{ax[tt], ay[tt]};
+  }
+}
+
+// Type-2 interpolation with eight lanes per point (FP32, W <= 8, TC even).
+// Same subproblem and LDS tile as k_interp.  Lane a0 = lane % 8 of a point's
+// group sums the W^2 taps of stencil plane a0; the group then adds its eight
+// partial sums with xor shuffles.  Consecutive a0 planes are E^2 cells apart,
+// which is 4 (mod 32) LDS banks for E = 15 and 16-byte cells, so the eight
+// 16-byte reads of a group fall on disjoint bank quads instead of the random
+// addresses of one-thread-per-point.  Each lane evaluates the three axis
+// weights of index a0; the group exchanges them with shuffles.
+template <int W, int TC>
+__global__ void __launch_bounds__(kBlock)
+    k_interp_rows(const Cplx<float>* __restrict__ fine, Cplx<float>* __restrict__ c, SubDesc sd,
+                  BinGeom g, const float* __restrict__ sv0, const float* __restrict__ sv1,
+                  const float* __restrict__ sv2, int64_t M, int ntrans, float kc, float beta) {
+  static_assert(W <= 8 && TC % 2 == 0, "eight lanes per point hold at most 8 stencil planes");
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  Cplx<float>* tile = reinterpret_cast<Cplx<float>*>(smem_raw);
+  const int sub = xcd_remap(blockIdx.x, gridDim.x);
+  if (sub >= *sd.nsub) return;
+  const int t0 = blockIdx.y * TC;
+  const int sb = sd.sub_sb[sub];
+  const int s2 = sb % g.nsb[2];
+  const int s1 = (sb / g.nsb[2]) % g.nsb[1];
+  const int s0 = sb / (g.nsb[2] * g.nsb[1]);
+  constexpr int E = kSuperEdge + W;
+  constexpr int ncell = E * E * E;
+  const int o0 = s0 * kSuperEdge - W / 2, o1 = s1 * kSuperEdge - W / 2, o2 = s2 * kSuperEdge - W / 2;
+  stage_tile<float, W, TC>(fine, tile, g, o0, o1, o2, t0, ntrans);
+  __syncthreads();
+  const int p0 = sd.sub_pt[sub];
+  const int pend = p0 + sd.sub_n[sub];
+  const int lane = threadIdx.x & 63;
+  const int gbase = lane & ~7;
+  const int a0 = lane & 7;
+  const float halfw = 0.5f * (float)W;
+  const float beta_l2 = beta * 1.4426950408889634f;
+  const float4* tile4 = reinterpret_cast<const float4*>(tile);
+  for (int pb = p0; pb < pend; pb += kBlock / 8) {
+    const int p = pb + (threadIdx.x >> 3);
+    const bool valid = p < pend;
+    float u0 = 0.f, u1 = 0.f, u2 = 0.f;
+    if (valid) {
+      u0 = sv0[p];
+      u1 = sv1[p];
+      u2 = sv2[p];
+    }
+    const int j00 = (int)ceilf(u0 - halfw), j01 = (int)ceilf(u1 - halfw), j02 = (int)ceilf(u2 - halfw);
+    const bool tap = valid && a0 < W;
+    const float wo0 = tap ? es_eval_native((float)(j00 + a0) - u0, kc, beta_l2) : 0.f;
+    const float wo1 = tap ? es_eval_native((float)(j01 + a0) - u1, kc, beta_l2) : 0.f;
+    const float wo2 = tap ? es_eval_native((float)(j02 + a0) - u2, kc, beta_l2) : 0.f;
+    float w1[W], w2[W];
+#pragma unroll
+    for (int a = 0; a < W; ++a) {
+      w1[a] = __shfl(wo1, gbase + a);
+      w2[a] = __shfl(wo2, gbase + a);
+    }
+    const int l0 = min(max(j00 + W / 2, 0), kSuperEdge);
+    const int l1 = min(max(j01 + W / 2, 0), kSuperEdge);
+    const int l2 = min(max(j02 + W / 2, 0), kSuperEdge);
+    float ax[TC], ay[TC];
+#pragma unroll
+    for (int tt = 0; tt < TC; ++tt) ax[tt] = ay[tt] = 0.f;
+    if (tap) {
+#pragma unroll
+      for (int a1 = 0; a1 < W; ++a1) {
+        const float w01 = wo0 * w1[a1];
+        const int row = ((l0 + a0) * E + (l1 + a1)) * E + l2;
+#pragma unroll
+        for (int a2 = 0; a2 < W; ++a2) {
+          const float ww = w01 * w2[a2];
+#pragma unroll
+          for (int h = 0; h < TC / 2; ++h) {
+            const float4 v = tile4[(row + a2) * (TC / 2) + h];
+            ax[2 * h] += ww * v.x;
+            ay[2 * h] += ww * v.y;
+            ax[2 * h + 1] += ww * v.z;
+            ay[2 * h + 1] += ww * v.w;
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int tt = 0; tt < TC; ++tt) {
+#pragma unroll
+      for (int m = 1; m < 8; m <<= 1) {
+        ax[tt] += __shfl_xor(ax[tt], m);
+        ay[tt] += __shfl_xor(ay[tt], m);
+      }
+    }
+    if (valid && a0 < TC) {
+      // sorted, chunk-major output: cs[chunk][p][TC]; lane a0 writes transform a0 of its group
+      float rx = ax[0], ry = ay[0];
+#pragma unroll
+      for (int tt = 1; tt < TC; ++tt)
+        if (a0 == tt) {
+          rx = ax[tt];
+          ry = ay[tt];
+        }
+      c[((int64_t)blockIdx.y * M + p) * TC + a0] = Cplx<float>{rx, ry};
+    }
   }
 }
 
@@ -787,13 +971,6 @@ __global__ void __launch_bounds__(kGatherBlock)
   }
 }
 
-// ES kernel in FP32 on the native v_sqrt_f32 / v_exp_f32 (about 1 ulp each):
-// exp(beta (sqrt(s) - 1)) = 2^(beta log2(e) (sqrt(s) - 1)), beta_l2 = beta log2(e).
-__device__ __forceinline__ float es_eval_native(float t, float c, float beta_l2) {
-  const float s = fmaxf(1.f - c * t * t, 0.f);
-  return __builtin_amdgcn_exp2f(beta_l2 * (__builtin_amdgcn_sqrtf(s) - 1.f));
-}
-
 __device__ __forceinline__ float lane_bcast(float v, int src) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
 }
@@ -1014,7 +1191,7 @@ struct Plan {
   int w = 7;
   int tc = 2;  // transforms per interpolation workgroup
   int spread_mode = 0;  // 0 = LDS-staged gather, 1 = point-driven LDS tiles, 2 = LDS-free gather (FP32)
-  int interp_mode = 0;  // 0 = thread per point, 1 = wavefront per point (tap-parallel)
+  int interp_mode = 0;  // 0 = thread per point, 1 = wavefront per point, 2 = eight lanes per point
   int max_tc_gather = 20;  // transforms per gather workgroup
   int tc_spread = 2;    // transforms per spreading workgroup (tile mode)
   double beta = 0, eps = 1e-6;
@@ -1085,6 +1262,9 @@ static int launch_interp(Plan* p, void* c_out, int ntrans_run, hipStream_t st) {
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     COV3D_HIP(hipFuncSetAttribute((const void*)k_interp_tp<T, W, TC>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    if constexpr (sizeof(T) == 4 && W <= 8 && TC % 2 == 0)
+      COV3D_HIP(hipFuncSetAttribute((const void*)k_interp_rows<W, TC>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     attr_done = true;
   }
   SubDesc sd{p->sub_sb, p->sub_pt, p->sub_n, p->nsub};
@@ -1092,7 +1272,17 @@ static int launch_interp(Plan* p, void* c_out, int ntrans_run, hipStream_t st) {
   const T kc = (T)(4.0 / ((double)W * W));
   {
     ScopedTiming tm(kTInterp, st);
-    if (p->interp_mode == 1)
+    bool done = false;
+    if constexpr (sizeof(T) == 4 && W <= 8 && TC % 2 == 0) {
+      if (p->interp_mode == 2) {
+        hipLaunchKernelGGL((k_interp_rows<W, TC>), grid, dim3(kBlock), smem, st, (const Cplx<float>*)p->fine,
+                           (Cplx<float>*)p->csort, sd, p->geom, (const float*)p->sv[0], (const float*)p->sv[1],
+                           (const float*)p->sv[2], p->M, ntrans_run, (float)kc, (float)p->beta);
+        done = true;
+      }
+    }
+    if (done) {
+    } else if (p->interp_mode == 1)
       hipLaunchKernelGGL((k_interp_tp<T, W, TC>), grid, dim3(kBlock), smem, st, (const Cplx<T>*)p->fine,
                          (Cplx<T>*)p->csort, sd, p->geom, (const T*)p->sv[0], (const T*)p->sv[1],
                          (const T*)p->sv[2], p->M, ntrans_run, kc, (T)p->beta);
@@ -1411,7 +1601,11 @@ static int make_plan(int type, const int64_t* n, int iflag, int64_t ntrans, doub
     const int sm = env_int("COV3D_SPREAD_MODE", p->dtype == 0 ? 2 : 0);
     p->spread_mode = (sm == 1 || (sm == 2 && p->dtype == 0)) ? sm : 0;
   }
-  p->interp_mode = env_int("COV3D_INTERP_MODE", 0) == 1 ? 1 : 0;
+  {
+    // 0 = thread per point (default), 1 = wavefront per point, 2 = eight lanes per point (FP32, W <= 8)
+    const int im = env_int("COV3D_INTERP_MODE", 0);
+    p->interp_mode = (im == 1 || im == 2) ? im : 0;
+  }
   p->max_tc_gather = std::min(std::max(env_int("COV3D_GATHER_TC", 20), 1), 20);
   p->ldc_cap = (int)ceil_div(ntrans, 4) * 4;
   for (int n_run = 1; n_run <= ntrans; ++n_run)
