@@ -425,6 +425,30 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblk) {
 }
 
 // W kernel weights of a point at coordinate u; FP32 uses the native exp/sqrt.
+// Bin of gather workgroup bid for L2 reuse: workgroups go to the 8 XCDs
+// round-robin (bid % 8), so XCD x runs bids x, x + 8, ... in order.  Each XCD
+// takes a contiguous share of 8^3-bin tiles and walks each tile b0-slowest;
+// its few hundred resident workgroups then cover a compact slab whose
+// 27-neighbourhood (the points a workgroup gathers) fits in that XCD's L2.
+// Grids whose bin counts are not multiples of 8 (or tile counts of 8) keep
+// the plain order.
+__device__ __forceinline__ void gather_bin(int bid, int nblk, const BinGeom& g, int& b0, int& b1, int& b2) {
+  constexpr int TB = 8;
+  const int T0 = g.nb[0] / TB, T1 = g.nb[1] / TB, T2 = g.nb[2] / TB;
+  const int ntile = T0 * T1 * T2;
+  if (g.nb[0] % TB || g.nb[1] % TB || g.nb[2] % TB || ntile % 8 || nblk != ntile * TB * TB * TB) {
+    bin_coords(g, bid, b0, b1, b2);
+    return;
+  }
+  const int x = bid & 7, k = bid >> 3;
+  const int tile = x * (ntile >> 3) + k / (TB * TB * TB);
+  const int i = k % (TB * TB * TB);
+  const int t2 = tile % T2, t1 = (tile / T2) % T1, t0 = tile / (T2 * T1);
+  b0 = t0 * TB + i / (TB * TB);
+  b1 = t1 * TB + (i / TB) % TB;
+  b2 = t2 * TB + i % TB;
+}
+
 template <typename T, int W>
 __device__ __forceinline__ void stencil_weights(T u, T c, T beta, int& l0, T* w) {
   const T halfw = (T)(0.5 * W);
@@ -846,11 +870,10 @@ __global__ void __launch_bounds__(kGatherBlock)
   __shared__ T s_w[kGatherBatch][3][16];
   __shared__ int s_j0[kGatherBatch][3];
   __shared__ __align__(16) Cplx<T> s_c[kGatherBatch][TC];
-  const int bin = xcd_remap(blockIdx.x, gridDim.x);
   const int t0 = blockIdx.y * TC;
   const int lane = threadIdx.x;
   int b0, b1, b2;
-  bin_coords(g, bin, b0, b1, b2);
+  gather_bin(blockIdx.x, gridDim.x, g, b0, b1, b2);
   const int c0 = lane >> 4, c1 = (lane >> 2) & 3, c2 = lane & 3;
   const T halfw = (T)(0.5 * W);
   // neighbour bins within reach of a W-wide support: ceil((W/2) / 4)
@@ -993,11 +1016,10 @@ __global__ void __launch_bounds__(kGatherBlock)
                 const int* __restrict__ bin_start, const int* __restrict__ bin_count,
                 const float* __restrict__ sv0, const float* __restrict__ sv1,
                 const float* __restrict__ sv2, int ntrans, int ldc, int W, float kc, float beta_l2) {
-  const int bin = xcd_remap(blockIdx.x, gridDim.x);
   const int t0 = blockIdx.y * TC;
   const int lane = threadIdx.x;
   int b0, b1, b2;
-  bin_coords(g, bin, b0, b1, b2);
+  gather_bin(blockIdx.x, gridDim.x, g, b0, b1, b2);
   const int c0 = lane >> 4, c1 = (lane >> 2) & 3, c2 = lane & 3;
   const float halfw = 0.5f * (float)W;
   const int R = (W + 2 * kBinEdge - 1) / (2 * kBinEdge);

@@ -54,6 +54,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-images", type=int, default=256)
     ap.add_argument("--no-cg", action="store_true", help="skip the mean-CG leg")
+    ap.add_argument("--subspace-batches", type=int, default=2,
+                    help="SGD steps of the FP64-trajectory subspace comparison (0 skips it)")
     ap.add_argument("--cg-images", type=int, default=100000)
     ap.add_argument("--cg-cap", type=int, default=20000, help="iteration cap of each CG solve")
     ap.add_argument("--copy-probe", action="store_true",
@@ -95,6 +97,30 @@ def sgd_run(pkg, data, L, r, B, steps, warmup, form, timing=False):
         native.timing_enable(False)
     del trainer
     return dt, kern
+
+
+def subspace_run(pkg, data, L, r, B, n_batches):
+    """Subspace of the chosen and plain forms vs an FP64 trajectory (same start, same batches)."""
+    import torch
+
+    val = importlib.import_module(pkg + ".validation")
+    ds = data.dataset
+    signal_var = max(float(ds.images[: n_batches * B].var()) - ds.noise_var, 1e-6)
+    v0 = torch.randn((r,) + (L,) * 3, generator=torch.Generator().manual_seed(0)) * (signal_var / L) ** 0.5
+    bases, secs = val.sgd_subspaces(ds, r, v0, batch_size=B, epochs=1, n_images=n_batches * B,
+                                    forms=("fp64", "chosen", "plain_fp32"))
+    err = val.subspace_errors(bases)
+    q0, _ = torch.linalg.qr(v0.reshape(r, -1).double().T.to(bases["fp64"].device))
+    return {
+        "chosen_vs_fp64": err["chosen"],
+        "plain_fp32_vs_fp64": err["plain_fp32"],
+        "fp64_moved_from_start": val.subspace_distance(q0, bases["fp64"]),
+        "steps": n_batches,
+        "seconds": {k: round(v, 2) for k, v in secs.items()},
+        "definition": (f"RMS sine of principal angles between r={r} eigen-subspaces after {n_batches} Adam steps "
+                       f"of {B} images from the same start; reference trajectory in FP64 (parameters, images, "
+                       f"NUFFT eps 1e-12, plain autograd objective)"),
+    }
 
 
 def mean_cg_run(pkg, L, n_images, dev, cap=20000):
@@ -241,6 +267,11 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         log("copy probe: 3 x 1 GiB device-to-device copies")
+    sub = None
+    if args.subspace_batches > 0:
+        log(f"subspace agreement vs FP64 over {args.subspace_batches} steps")
+        sub = subspace_run(pkg, data, L, r, B, args.subspace_batches)
+        log(f"subspace: chosen {sub['chosen_vs_fp64']:.2e}, plain {sub['plain_fp32_vs_fp64']:.2e} ({sub['seconds']})")
     cg = None
     if not args.no_cg:
         del data
@@ -275,6 +306,7 @@ def main():
         "speedup_vs_plain": round(ips / plain["value"], 3) if plain else None,
         "kernels_ms": {k: round(v["avg_ms"], 3) if v["avg_ms"] else None for k, v in kern.items()},
         "roofline": roof,
+        "subspace_error": sub,
         "mean_cg": cg,
         "cpu_baseline": cpu,
     }
