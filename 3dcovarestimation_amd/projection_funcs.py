@@ -54,9 +54,38 @@ def crop_image(images: torch.Tensor, L_crop: int) -> torch.Tensor:
     return images[..., idx, :][..., idx].reshape(*images.shape[:-2], L_crop, L_crop)
 
 
+_SIGNS = {}
+
+
+def _checkerboard(shape, dims, device, dtype) -> Tuple[torch.Tensor, torch.Tensor]:
+    """(-1)^(sum of indices over dims), broadcastable over the other axes, and that times (-1)^(sum N/2)."""
+    key = (tuple(shape), tuple(dims), str(device), dtype)
+    if key not in _SIGNS:
+        bshape = [1] * len(shape)
+        s = torch.ones(bshape, dtype=dtype, device=device)
+        half = 0
+        for d in dims:
+            n = shape[d]
+            v = torch.ones(n, dtype=dtype, device=device)
+            v[1::2] = -1
+            vs = [1] * len(shape)
+            vs[d] = n
+            s = s * v.reshape(vs)
+            half += n // 2
+        _SIGNS[key] = (s, s * (-1.0 if half % 2 else 1.0))
+    return _SIGNS[key]
+
+
 def _centered(fn, tensor: torch.Tensor, dim: List[int], size: Optional[List[int]] = None) -> torch.Tensor:
     if size is not None:
         tensor = pad_tensor(tensor, list(size), dim)
+    dims = [d % tensor.ndim for d in dim]
+    if all(tensor.shape[d] % 2 == 0 for d in dims):
+        # even sizes: fftshift(F(ifftshift(x)))[k] = (-1)^(k + N/2) F((-1)^m x)[k] per axis, for the forward
+        # and the inverse transform alike, so the two shifts become sign flips
+        real = tensor.real.dtype if tensor.is_complex() else tensor.dtype
+        s_in, s_out = _checkerboard(tensor.shape, dims, tensor.device, real)
+        return fn(tensor * s_in, dim=dim) * s_out
     return torch.fft.fftshift(fn(torch.fft.ifftshift(tensor, dim=dim), dim=dim), dim=dim)
 
 

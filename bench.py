@@ -54,7 +54,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-images", type=int, default=256)
     ap.add_argument("--no-cg", action="store_true", help="skip the mean-CG leg")
-    ap.add_argument("--subspace-batches", type=int, default=2,
+    ap.add_argument("--subspace-batches", type=int, default=4,
                     help="SGD steps of the FP64-trajectory subspace comparison (0 skips it)")
     ap.add_argument("--cg-images", type=int, default=100000)
     ap.add_argument("--cg-cap", type=int, default=20000, help="iteration cap of each CG solve")
@@ -107,7 +107,8 @@ def subspace_run(pkg, data, L, r, B, n_batches):
     ds = data.dataset
     signal_var = max(float(ds.images[: n_batches * B].var()) - ds.noise_var, 1e-6)
     v0 = torch.randn((r,) + (L,) * 3, generator=torch.Generator().manual_seed(0)) * (signal_var / L) ** 0.5
-    bases, secs = val.sgd_subspaces(ds, r, v0, batch_size=B, epochs=1, n_images=n_batches * B,
+    # lr 1e-6 per image: the reference workflow's default for the eigenvector fit (workflow.py:377)
+    bases, secs = val.sgd_subspaces(ds, r, v0, batch_size=B, epochs=1, n_images=n_batches * B, lr=1e-6,
                                     forms=("fp64", "chosen", "plain_fp32"))
     err = val.subspace_errors(bases)
     q0, _ = torch.linalg.qr(v0.reshape(r, -1).double().T.to(bases["fp64"].device))
@@ -118,7 +119,7 @@ def subspace_run(pkg, data, L, r, B, n_batches):
         "steps": n_batches,
         "seconds": {k: round(v, 2) for k, v in secs.items()},
         "definition": (f"RMS sine of principal angles between r={r} eigen-subspaces after {n_batches} Adam steps "
-                       f"of {B} images from the same start; reference trajectory in FP64 (parameters, images, "
+                       f"of {B} images (lr 1e-6 per image) from the same start; reference trajectory in FP64 (parameters, images, "
                        f"NUFFT eps 1e-12, plain autograd objective)"),
     }
 
