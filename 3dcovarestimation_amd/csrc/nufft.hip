@@ -533,12 +533,58 @@ __global__ void __launch_bounds__(kBlock)
   constexpr int E = kSuperEdge + W;
   constexpr int ncell = E * E * E;
   const int o0 = s0 * kSuperEdge - W / 2, o1 = s1 * kSuperEdge - W / 2, o2 = s2 * kSuperEdge - W / 2;
-  // halo cells lie within one period of the grid (nf >= 2W), so one conditional wrap suffices
-  stage_tile<T, W, TC>(fine, tile, g, o0, o1, o2, t0, ntrans);
-  __syncthreads();
   const int p0 = sd.sub_pt[sub];
   const int np = sd.sub_n[sub];
-  for (int p = p0 + threadIdx.x; p < p0 + np; p += blockDim.x) {
+  // Bank-aware point order (FP32, two transforms per 16-byte cell).  All lanes read the same tap
+  // offset together, and ds_read_b128 serves 16 lanes per clock from 64 banks, so a read is
+  // conflict-free when the 16 lanes' base cells differ mod 16.  Points are bucketed by base
+  // cell mod 16 and dealt round-robin over the buckets: bucket b, rank r goes to slot
+  // sum_b' min(c_b', r) + #{b' < b : c_b' > r}.
+  constexpr bool kBankOrder = TC == 2 && sizeof(T) == 4 && W <= 8;
+  constexpr int kPer = kMaxSub / kBlock;
+  __shared__ int s_order[kBankOrder ? kMaxSub : 1];
+  __shared__ int s_cnt[16];
+  if constexpr (kBankOrder) {
+    if (threadIdx.x < 16) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+  }
+  // halo cells lie within one period of the grid (nf >= 2W), so one conditional wrap suffices
+  stage_tile<T, W, TC>(fine, tile, g, o0, o1, o2, t0, ntrans);
+  int rb[kPer], rr[kPer];
+  if constexpr (kBankOrder) {
+    const T halfw = (T)(0.5 * W);
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int slot = threadIdx.x + k * kBlock;
+      if (slot < np) {
+        const int p = p0 + slot;
+        const int m0 = min(max((int)ceil(sv0[p] - halfw) + W / 2, 0), kSuperEdge);
+        const int m1 = min(max((int)ceil(sv1[p] - halfw) + W / 2, 0), kSuperEdge);
+        const int m2 = min(max((int)ceil(sv2[p] - halfw) + W / 2, 0), kSuperEdge);
+        rb[k] = ((m0 * E + m1) * E + m2) & 15;
+        rr[k] = atomicAdd(&s_cnt[rb[k]], 1);
+      }
+    }
+  }
+  __syncthreads();
+  if constexpr (kBankOrder) {
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int slot = threadIdx.x + k * kBlock;
+      if (slot < np) {
+        int pos = 0;
+#pragma unroll
+        for (int b = 0; b < 16; ++b) {
+          const int cb = s_cnt[b];
+          pos += min(cb, rr[k]) + ((b < rb[k] && cb > rr[k]) ? 1 : 0);
+        }
+        s_order[pos] = slot;
+      }
+    }
+    __syncthreads();
+  }
+  for (int slot = threadIdx.x; slot < np; slot += blockDim.x) {
+    const int p = p0 + (kBankOrder ? s_order[slot] : slot);
     T w0[W], w1[W], w2[W];
     int l0, l1, l2;
     stencil_weights<T, W>(sv0[p], kc, beta, l0, w0);
