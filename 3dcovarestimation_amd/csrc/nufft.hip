@@ -1054,9 +1054,12 @@ __device__ __forceinline__ float lane_bcast(float v, int src) {
 // offsets x 3 axes) and every lane picks its three with bpermute; the
 // point's TC values are read at a wave-uniform address (scalar loads) and
 // enter the FMAs as scalar operands.  The sorted values have row stride ldc =
-// gridDim.y * TC with zeros past ntrans, so every load is unconditional.  Nothing occupies LDS, so occupancy is
-// bounded by registers only.
-template <int TC>
+// gridDim.y * TC with zeros past ntrans, so every load is unconditional.
+// With TOUCH, each hit lane first reads its own value row with vector loads,
+// which leaves the rows in L2 for the scalar loads (measured: 28.4 -> 24.8 ms
+// at L=128, r=20).  Nothing occupies LDS, so occupancy is bounded by
+// registers only.
+template <int TC, bool TOUCH>
 __global__ void __launch_bounds__(kGatherBlock)
     k_spread_sc(const Cplx<float>* __restrict__ cs, Cplx<float>* __restrict__ fine, BinGeom g,
                 const int* __restrict__ bin_start, const int* __restrict__ bin_count,
@@ -1075,23 +1078,39 @@ __global__ void __launch_bounds__(kGatherBlock)
 #pragma unroll
   for (int tt = 0; tt < TC; ++tt) ax[tt] = ay[tt] = 0.f;
 
-  for (int d0 = -R; d0 <= R; ++d0)
-    for (int d1 = -R; d1 <= R; ++d1)
-      for (int d2 = -R; d2 <= R; ++d2) {
-        int n0 = b0 + d0, n1 = b1 + d1, n2 = b2 + d2, sh0 = 0, sh1 = 0, sh2 = 0;
-        if (n0 < 0) { n0 += g.nb[0]; sh0 = -g.nf[0]; }
-        if (n0 >= g.nb[0]) { n0 -= g.nb[0]; sh0 = g.nf[0]; }
-        if (n1 < 0) { n1 += g.nb[1]; sh1 = -g.nf[1]; }
-        if (n1 >= g.nb[1]) { n1 -= g.nb[1]; sh1 = g.nf[1]; }
-        if (n2 < 0) { n2 += g.nb[2]; sh2 = -g.nf[2]; }
-        if (n2 >= g.nb[2]) { n2 -= g.nb[2]; sh2 = g.nf[2]; }
-        const int nbin = bin_index(g, n0, n1, n2);
+  const int nside = 2 * R + 1;
+  const int nnb = nside * nside * nside;
+  // neighbour bin nbi of this block: its bin index and the block's first cell in the neighbour's
+  // super-bin frame (unwrapped across the periodic boundary)
+  auto neighbour = [&](int nbi, int& nbin, int& q0, int& q1, int& q2) {
+    const int d0 = nbi / (nside * nside) - R, d1 = (nbi / nside) % nside - R, d2 = nbi % nside - R;
+    int n0 = b0 + d0, n1 = b1 + d1, n2 = b2 + d2, sh0 = 0, sh1 = 0, sh2 = 0;
+    if (n0 < 0) { n0 += g.nb[0]; sh0 = -g.nf[0]; }
+    if (n0 >= g.nb[0]) { n0 -= g.nb[0]; sh0 = g.nf[0]; }
+    if (n1 < 0) { n1 += g.nb[1]; sh1 = -g.nf[1]; }
+    if (n1 >= g.nb[1]) { n1 -= g.nb[1]; sh1 = g.nf[1]; }
+    if (n2 < 0) { n2 += g.nb[2]; sh2 = -g.nf[2]; }
+    if (n2 >= g.nb[2]) { n2 -= g.nb[2]; sh2 = g.nf[2]; }
+    nbin = bin_index(g, n0, n1, n2);
+    q0 = kBinEdge * b0 - (n0 >> 1) * kSuperEdge - sh0;
+    q1 = kBinEdge * b1 - (n1 >> 1) * kSuperEdge - sh1;
+    q2 = kBinEdge * b2 - (n2 >> 1) * kSuperEdge - sh2;
+  };
+  // TOUCH: before the hits of a batch read their value rows with scalar loads, each hit lane
+  // reads its own row once with vector loads (one per 64-byte line), which leaves the rows in
+  // L2 for the scalar loads (these alone are not served from L2 across workgroups as well)
+  auto touch_row = [&](int pt) {
+    const float* row = reinterpret_cast<const float*>(cs + (int64_t)pt * ldc + t0);
+    float t = 0.f;
+#pragma unroll
+    for (int o = 0; o < 2 * TC; o += 16) t += row[o];
+    return t;
+  };
+  for (int nbi = 0; nbi < nnb; ++nbi) {
+        int nbin, q0, q1, q2;
+        neighbour(nbi, nbin, q0, q1, q2);
         const int start = bin_start[nbin];
         const int end = start + bin_count[nbin];
-        // first cell of this block in the neighbour's super-bin frame (unwrapped)
-        const int q0 = kBinEdge * b0 - (n0 >> 1) * kSuperEdge - sh0;
-        const int q1 = kBinEdge * b1 - (n1 >> 1) * kSuperEdge - sh1;
-        const int q2 = kBinEdge * b2 - (n2 >> 1) * kSuperEdge - sh2;
         const int qa = wa == 0 ? q0 : (wa == 1 ? q1 : q2);
         for (int base = start; base < end; base += kGatherBlock) {
           float v0 = 0.f, v1 = 0.f, v2 = 0.f;
@@ -1105,6 +1124,12 @@ __global__ void __launch_bounds__(kGatherBlock)
                     q2 + 3 < j2);
           }
           unsigned long long mask = __ballot(hit);
+          if constexpr (TOUCH) {
+            if (hit) {
+              const float t = touch_row(base + lane);
+              asm volatile("" ::"v"(t));
+            }
+          }
           while (mask) {
             const int q = __builtin_ctzll(mask);
             mask &= mask - 1;
@@ -1258,7 +1283,7 @@ struct Plan {
   int64_t ntrans = 1;
   int w = 7;
   int tc = 2;  // transforms per interpolation workgroup
-  int spread_mode = 0;  // 0 = LDS-staged gather, 1 = point-driven LDS tiles, 2 = LDS-free gather (FP32)
+  int spread_mode = 0;  // 0 = LDS-staged gather, 1 = LDS atomic tiles, 2/3 = LDS-free gather (FP32; 3 with L2 touch)
   int interp_mode = 0;  // 0 = thread per point, 1 = wavefront per point, 2 = eight lanes per point
   int max_tc_gather = 20;  // transforms per gather workgroup
   int tc_spread = 2;    // transforms per spreading workgroup (tile mode)
@@ -1392,7 +1417,7 @@ static int run_interp(Plan* p, void* c_out, int ntrans_run, hipStream_t st) {
 
 template <typename T, int TC>
 static int launch_spread(Plan* p, const void* c_in, int ntrans_run, hipStream_t st) {
-  const bool sc = sizeof(T) == 4 && p->spread_mode == 2;
+  const bool sc = sizeof(T) == 4 && (p->spread_mode == 2 || p->spread_mode == 3);
   const int ldc = sc ? (int)ceil_div(ntrans_run, TC) * TC : ntrans_run;
   COV3D_CHECK_ARG(ldc <= p->ldc_cap, "sorted row stride %d exceeds %d", ldc, p->ldc_cap);
   {
@@ -1405,8 +1430,16 @@ static int launch_spread(Plan* p, const void* c_in, int ntrans_run, hipStream_t 
   const T kc = (T)(4.0 / ((double)p->w * p->w));
   ScopedTiming tm(kTSpread, st);
   if constexpr (sizeof(T) == 4) {
+    if (p->spread_mode == 3) {
+      hipLaunchKernelGGL((k_spread_sc<TC, true>), grid, dim3(kGatherBlock), 0, st, (const Cplx<float>*)p->csort,
+                         (Cplx<float>*)p->fine, p->geom, p->bin_start, p->bin_count, (const float*)p->sv[0],
+                         (const float*)p->sv[1], (const float*)p->sv[2], ntrans_run, ldc, p->w, (float)kc,
+                         (float)(p->beta * 1.4426950408889634));
+      COV3D_LAUNCH_CHECK();
+      return 0;
+    }
     if (p->spread_mode == 2) {
-      hipLaunchKernelGGL((k_spread_sc<TC>), grid, dim3(kGatherBlock), 0, st, (const Cplx<float>*)p->csort,
+      hipLaunchKernelGGL((k_spread_sc<TC, false>), grid, dim3(kGatherBlock), 0, st, (const Cplx<float>*)p->csort,
                          (Cplx<float>*)p->fine, p->geom, p->bin_start, p->bin_count, (const float*)p->sv[0],
                          (const float*)p->sv[1], (const float*)p->sv[2], ntrans_run, ldc, p->w, (float)kc,
                          (float)(p->beta * 1.4426950408889634));
@@ -1665,9 +1698,10 @@ static int make_plan(int type, const int64_t* n, int iflag, int64_t ntrans, doub
   const int tc_env = env_int("COV3D_NUFFT_TC", 2);
   p->tc = dtype == 1 ? 1 : (tc_env == 1 || tc_env == 4 ? tc_env : 2);
   {
-    // 2 = LDS-free gather (FP32 default), 0 = LDS-staged gather (FP64 default), 1 = LDS atomic tiles
-    const int sm = env_int("COV3D_SPREAD_MODE", p->dtype == 0 ? 2 : 0);
-    p->spread_mode = (sm == 1 || (sm == 2 && p->dtype == 0)) ? sm : 0;
+    // 3 = LDS-free gather with L2 touch (FP32 default), 2 = the same without the touch,
+    // 0 = LDS-staged gather (FP64 default), 1 = LDS atomic tiles
+    const int sm = env_int("COV3D_SPREAD_MODE", p->dtype == 0 ? 3 : 0);
+    p->spread_mode = (sm == 1 || ((sm == 2 || sm == 3) && p->dtype == 0)) ? sm : 0;
   }
   {
     // 0 = thread per point (default), 1 = wavefront per point, 2 = eight lanes per point (FP32, W <= 8)

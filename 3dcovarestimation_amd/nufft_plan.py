@@ -216,6 +216,7 @@ class NufftPlan(BaseNufftPlan):
                 )
             )
         self.points = None
+        self._pts_synced = False
         self.points_N = 0
         self.points_L = 0
 
@@ -237,46 +238,57 @@ class NufftPlan(BaseNufftPlan):
         return {"nf": tuple(nf), "w": w.value, "beta": beta.value}
 
     def setpts(self, points: torch.Tensor) -> None:
-        """points: (n, 3, Lp**2) radians; wrapped to [-pi, pi) (nufft_plan.py:318-336)."""
+        """points: (n, 3, Lp**2) radians; wrapped to [-pi, pi) (nufft_plan.py:318-336).
+
+        The wrapped points are kept in ``self.points``; the device-side bin sort runs on the
+        first execute after this call (a plan whose points only feed a companion plan, as in
+        the half-plane ML path, never sorts them).
+        """
         _native.require_device(points, "points")
         self.points_N = points.shape[0]
         self.points_L = int(round(points.shape[-1] ** 0.5))
         pts = points.transpose(0, 1).reshape(3, -1)
         pts = (torch.remainder(pts + torch.pi, 2 * torch.pi) - torch.pi).to(self.dtype).contiguous()
         self.points = pts
-        M = pts.shape[1]
-        with torch.cuda.device(self.device):
-            _native.check(
-                self._lib.cov3d_nufft_setpts(
-                    self._handle, M, pts[0].data_ptr(), pts[1].data_ptr(), pts[2].data_ptr(),
-                    _native.stream_ptr(self.device),
-                )
-            )
+        self._pts_synced = False
 
     def set_points_flat(self, pts: torch.Tensor, n_images: int) -> None:
         """Sets already-wrapped points (3, M) directly (M need not be n_images * L**2)."""
         _native.require_device(pts, "points")
-        pts = pts.to(self.dtype).contiguous()
-        self.points = pts
+        self.points = pts.to(self.dtype).contiguous()
         self.points_N = n_images
         self.points_L = 0
-        with torch.cuda.device(self.device):
-            _native.check(
-                self._lib.cov3d_nufft_setpts(
-                    self._handle, pts.shape[1], pts[0].data_ptr(), pts[1].data_ptr(), pts[2].data_ptr(),
-                    _native.stream_ptr(self.device),
+        self._pts_synced = False
+
+    def _sync_points(self) -> None:
+        if self.points is None:
+            raise RuntimeError("setpts must be called before executing the plan")
+        if not getattr(self, "_pts_synced", False):
+            pts = self.points
+            with torch.cuda.device(self.device):
+                _native.check(
+                    self._lib.cov3d_nufft_setpts(
+                        self._handle, pts.shape[1], pts[0].data_ptr(), pts[1].data_ptr(), pts[2].data_ptr(),
+                        _native.stream_ptr(self.device),
+                    )
                 )
-            )
+            self._pts_synced = True
+
+    def _output(self, shape, dtype, n: int) -> torch.Tensor:
+        """Output buffer: the kernels write rows [0, n); rows of a zero-padded short batch are cleared."""
+        out = torch.empty(shape, dtype=dtype, device=self.device)
+        if n < shape[0]:
+            out[n:].zero_()
+        return out
 
     def execute_forward_flat(self, signal: torch.Tensor) -> torch.Tensor:
         """Type-2 NUFFT returning the raw (batch_size, M) values."""
-        if self.points is None:
-            raise RuntimeError("points must be set before execute_forward_flat")
+        self._sync_points()
         _native.require_device(signal, "signal")
         signal, n = self._pad_batch(signal)
         real_in = not signal.is_complex()
         sig = signal.to(self.dtype if real_in else self.complex_dtype).contiguous()
-        out = torch.zeros((self.batch_size, self.points.shape[1]), dtype=self.complex_dtype, device=self.device)
+        out = self._output((self.batch_size, self.points.shape[1]), self.complex_dtype, n)
         with torch.cuda.device(self.device):
             _native.check(
                 self._lib.cov3d_nufft_execute_dir(
@@ -296,14 +308,13 @@ class NufftPlan(BaseNufftPlan):
 
     def execute_forward(self, signal: torch.Tensor) -> torch.Tensor:
         """signal (n<=batch_size, *sz) real/complex -> (batch_size, N, Lp, Lp) complex (nufft_plan.py:338-364)."""
-        if self.points is None:
-            raise RuntimeError("setpts must be called before execute_forward")
+        self._sync_points()
         _native.require_device(signal, "signal")
         signal, n = self._pad_batch(signal)
         real_in = not signal.is_complex()
         sig = signal.to(self.dtype if real_in else self.complex_dtype).contiguous()
         M = self.points.shape[1]
-        out = torch.zeros((self.batch_size, M), dtype=self.complex_dtype, device=self.device)
+        out = self._output((self.batch_size, M), self.complex_dtype, n)
         with torch.cuda.device(self.device):
             _native.check(
                 self._lib.cov3d_nufft_execute_dir(
@@ -319,8 +330,7 @@ class NufftPlan(BaseNufftPlan):
         ``real_output=True`` returns only the real part (what every real-volume
         caller takes) without materialising the imaginary half.
         """
-        if self.points is None:
-            raise RuntimeError("setpts must be called before execute_adjoint")
+        self._sync_points()
         _native.require_device(signal, "signal")
         signal, n = self._pad_batch(signal)
         M = self.points.shape[1]
@@ -328,7 +338,7 @@ class NufftPlan(BaseNufftPlan):
         if sig.shape[1] != M:
             raise ValueError(f"signal has {sig.shape[1]} samples per transform, plan has {M} points")
         out_dtype = self.dtype if real_output else self.complex_dtype
-        out = torch.zeros((self.batch_size, *self.sz), dtype=out_dtype, device=self.device)
+        out = self._output((self.batch_size, *self.sz), out_dtype, n)
         with torch.cuda.device(self.device):
             _native.check(
                 self._lib.cov3d_nufft_execute_dir(
