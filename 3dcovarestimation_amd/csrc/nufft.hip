@@ -393,6 +393,18 @@ __global__ void k_to_sorted(const Cplx<T>* __restrict__ c, Cplx<T>* __restrict__
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= M) return;
   Cplx<T>* dst = cs + (int64_t)spos[j] * ldc;
+  if constexpr (sizeof(T) == 4) {
+    if ((ldc & 1) == 0) {
+      // FP32 rows of even length: transform pairs as 16-byte stores (rows are 16-byte aligned)
+      float4* d4 = reinterpret_cast<float4*>(dst);
+      for (int t = 0; t < ldc; t += 2) {
+        const Cplx<T> a = t < ntrans ? c[(int64_t)t * M + j] : Cplx<T>{0.f, 0.f};
+        const Cplx<T> b = t + 1 < ntrans ? c[(int64_t)(t + 1) * M + j] : Cplx<T>{0.f, 0.f};
+        d4[t >> 1] = float4{a.x, a.y, b.x, b.y};
+      }
+      return;
+    }
+  }
   for (int t = 0; t < ntrans; ++t) dst[t] = c[(int64_t)t * M + j];
   for (int t = ntrans; t < ldc; ++t) dst[t] = Cplx<T>{(T)0, (T)0};
 }
@@ -403,6 +415,18 @@ __global__ void k_from_sorted(const Cplx<T>* __restrict__ cs, Cplx<T>* __restric
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= M) return;
   const int64_t sp = spos[j];
+  if constexpr (sizeof(T) == 4) {
+    if (tc == 2) {
+      // chunk h holds transforms 2h, 2h + 1 of a point in one 16-byte word
+      const float4* s4 = reinterpret_cast<const float4*>(cs);
+      for (int h = 0; 2 * h < ntrans; ++h) {
+        const float4 v = s4[(int64_t)h * M + sp];
+        c[(int64_t)(2 * h) * M + j] = Cplx<T>{v.x, v.y};
+        if (2 * h + 1 < ntrans) c[(int64_t)(2 * h + 1) * M + j] = Cplx<T>{v.z, v.w};
+      }
+      return;
+    }
+  }
   for (int t = 0; t < ntrans; ++t) {
     const int h = t / tc, tt = t - h * tc;
     c[(int64_t)t * M + j] = cs[((int64_t)h * M + sp) * tc + tt];
