@@ -387,6 +387,96 @@ __global__ void k_deconv_crop(const Cplx<T>* __restrict__ fine, T* __restrict__ 
 //   to_sorted:   cs[spos[j]][t]        = c[t][j]      (spreading input, row stride ldc >= ntrans,
 //                                                      zero past ntrans)
 //   from_sorted: c[t][j] = cs[t / TC][spos[j]][t % TC] (interpolation output)
+// Real volumes (type 2): modes -> zero-padded, deconvolved REAL fine grid, the input of a
+// real-to-complex FFT that writes only the Hermitian half.  One wavefront per fine-grid row
+// (t, m0, m1) writes the nf2 values of the row with 16-byte stores (nf2 is a multiple of 8);
+// rows outside the modes of axes 0/1 are zero.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    k_deconv_pad_r(const T* __restrict__ f, T* __restrict__ fine_r, int64_t ntrans, int64_t nrun, int N0, int N1,
+                   int N2, int nf0, int nf1, int nf2, const T* __restrict__ ip0, const T* __restrict__ ip1,
+                   const T* __restrict__ ip2) {
+  const int64_t row = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int64_t nrows = ntrans * nf0 * nf1;
+  if (row >= nrows) return;
+  const int m1 = (int)(row % nf1);
+  const int64_t r1 = row / nf1;
+  const int m0 = (int)(r1 % nf0);
+  const int64_t t = r1 / nf0;
+  auto mode_index = [](int m, int N, int nf) -> int {
+    int hp = (N + 1) / 2, hn = N / 2;
+    if (m < hp) return m + hn;
+    if (m >= nf - hn) return m - nf + hn;
+    return -1;
+  };
+  const int i0 = mode_index(m0, N0, nf0), i1 = mode_index(m1, N1, nf1);
+  const bool live = t < nrun && i0 >= 0 && i1 >= 0;
+  const T s01 = live ? ip0[i0] * ip1[i1] : (T)0;
+  const T* src = f + ((t * N0 + (live ? i0 : 0)) * N1 + (live ? i1 : 0)) * N2;
+  T* dst = fine_r + row * nf2;
+  for (int m = 4 * lane; m < nf2; m += 4 * 64) {
+    T v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int i2 = mode_index(m + k, N2, nf2);
+      v[k] = (live && i2 >= 0) ? src[i2] * (s01 * ip2[i2]) : (T)0;
+    }
+    if constexpr (sizeof(T) == 4) {
+      *reinterpret_cast<float4*>(dst + m) = float4{v[0], v[1], v[2], v[3]};
+    } else {
+      reinterpret_cast<double2*>(dst + m)[0] = double2{v[0], v[1]};
+      reinterpret_cast<double2*>(dst + m)[1] = double2{v[2], v[3]};
+    }
+  }
+}
+
+// Real outputs (type 1): Re(F(G)) is the complex-to-real transform of the Hermitian part of the
+// spread grid, H[m] = (G[m] + conj(G[-m])) / 2, of which only m2 <= nf2/2 is stored
+// (conjugated when the transform sign is negative; one thread per half-grid cell).
+template <typename T>
+__global__ void k_fold_half(const Cplx<T>* __restrict__ G, Cplx<T>* __restrict__ H, int64_t ntrans, int nf0,
+                            int nf1, int nf2, int conj_out) {
+  const int h2 = nf2 / 2 + 1;
+  const int64_t nh = (int64_t)nf0 * nf1 * h2;
+  const int64_t nfine = (int64_t)nf0 * nf1 * nf2;
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nh * ntrans) return;
+  const int64_t t = idx / nh;
+  const int64_t r = idx - t * nh;
+  const int m2 = (int)(r % h2);
+  const int64_t r2 = r / h2;
+  const int m1 = (int)(r2 % nf1);
+  const int m0 = (int)(r2 / nf1);
+  const int n0 = m0 ? nf0 - m0 : 0, n1 = m1 ? nf1 - m1 : 0, n2 = m2 ? nf2 - m2 : 0;
+  const Cplx<T> a = G[t * nfine + ((int64_t)m0 * nf1 + m1) * nf2 + m2];
+  const Cplx<T> b = G[t * nfine + ((int64_t)n0 * nf1 + n1) * nf2 + n2];
+  const T im = (T)0.5 * (a.y - b.y);
+  H[idx] = Cplx<T>{(T)0.5 * (a.x + b.x), conj_out ? -im : im};
+}
+
+// Real fine grid (output of the complex-to-real transform) -> deconvolved real modes.
+template <typename T>
+__global__ void k_deconv_crop_r(const T* __restrict__ fine_r, T* __restrict__ f, int64_t ntrans, int N0, int N1,
+                                int N2, int nf0, int nf1, int nf2, const T* __restrict__ ip0,
+                                const T* __restrict__ ip1, const T* __restrict__ ip2) {
+  const int64_t nm = (int64_t)N0 * N1 * N2;
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nm * ntrans) return;
+  const int64_t t = idx / nm;
+  const int64_t r = idx - t * nm;
+  const int i2 = (int)(r % N2);
+  const int64_t r2 = r / N2;
+  const int i1 = (int)(r2 % N1);
+  const int i0 = (int)(r2 / N1);
+  auto fine_index = [](int i, int N, int nf) -> int {
+    int n = i - N / 2;
+    return n >= 0 ? n : n + nf;
+  };
+  const int m0 = fine_index(i0, N0, nf0), m1 = fine_index(i1, N1, nf1), m2 = fine_index(i2, N2, nf2);
+  f[idx] = fine_r[((t * nf0 + m0) * (int64_t)nf1 + m1) * nf2 + m2] * (ip0[i0] * ip1[i1] * ip2[i2]);
+}
+
 template <typename T>
 __global__ void k_to_sorted(const Cplx<T>* __restrict__ c, Cplx<T>* __restrict__ cs,
                             const int* __restrict__ spos, int64_t M, int ntrans, int ldc) {
@@ -500,13 +590,17 @@ __device__ __forceinline__ int wrap1(int m, int n) {
 // global loads in flight instead of one iteration's.
 constexpr int kStageGroup = 7;
 
+// half_h2 > 0: the fine grid holds only the Hermitian half m2 < half_h2 (= nf2/2 + 1) of the
+// transform of a real volume; a cell m with m2 >= half_h2 is conj(F[-m]).  conj_all conjugates
+// every value (the half grid was computed with the opposite sign).
 template <typename T, int W, int TC>
 __device__ __forceinline__ void stage_tile(const Cplx<T>* __restrict__ fine, Cplx<T>* tile, const BinGeom& g,
-                                           int o0, int o1, int o2, int t0, int ntrans) {
+                                           int o0, int o1, int o2, int t0, int ntrans, int half_h2 = 0,
+                                           int conj_all = 0) {
   constexpr int E = kSuperEdge + W;
   constexpr int ncell = E * E * E;
   constexpr int NIT = (ncell + kBlock - 1) / kBlock;
-  const int64_t nfine = (int64_t)g.nf[0] * g.nf[1] * g.nf[2];
+  const int64_t nfine = half_h2 ? (int64_t)g.nf[0] * g.nf[1] * half_h2 : (int64_t)g.nf[0] * g.nf[1] * g.nf[2];
 #pragma unroll
   for (int it0 = 0; it0 < NIT; it0 += kStageGroup) {
     Cplx<T> v[kStageGroup][TC];
@@ -517,11 +611,27 @@ __device__ __forceinline__ void stage_tile(const Cplx<T>* __restrict__ fine, Cpl
         const int e2 = cidx % E;
         const int e1 = (cidx / E) % E;
         const int e0 = cidx / (E * E);
-        const int64_t off = ((int64_t)wrap1(o0 + e0, g.nf[0]) * g.nf[1] + wrap1(o1 + e1, g.nf[1])) * g.nf[2] +
-                            wrap1(o2 + e2, g.nf[2]);
+        int m0 = wrap1(o0 + e0, g.nf[0]), m1 = wrap1(o1 + e1, g.nf[1]), m2 = wrap1(o2 + e2, g.nf[2]);
+        int64_t off;
+        bool neg = false;
+        if (half_h2) {
+          if (m2 >= half_h2) {
+            m0 = m0 ? g.nf[0] - m0 : 0;
+            m1 = m1 ? g.nf[1] - m1 : 0;
+            m2 = g.nf[2] - m2;
+            neg = true;
+          }
+          neg = neg != (conj_all != 0);
+          off = ((int64_t)m0 * g.nf[1] + m1) * half_h2 + m2;
+        } else {
+          off = ((int64_t)m0 * g.nf[1] + m1) * g.nf[2] + m2;
+        }
 #pragma unroll
-        for (int tt = 0; tt < TC; ++tt)
-          v[k][tt] = t0 + tt < ntrans ? fine[(t0 + tt) * nfine + off] : Cplx<T>{(T)0, (T)0};
+        for (int tt = 0; tt < TC; ++tt) {
+          Cplx<T> x = t0 + tt < ntrans ? fine[(t0 + tt) * nfine + off] : Cplx<T>{(T)0, (T)0};
+          if (neg) x.y = -x.y;
+          v[k][tt] = x;
+        }
       }
     }
 #pragma unroll
@@ -544,7 +654,7 @@ template <typename T, int W, int TC>
 __global__ void __launch_bounds__(kBlock)
     k_interp(const Cplx<T>* __restrict__ fine, Cplx<T>* __restrict__ c, SubDesc sd, BinGeom g,
              const T* __restrict__ sv0, const T* __restrict__ sv1, const T* __restrict__ sv2,
-             const int* __restrict__ sidx, int64_t M, int ntrans, T kc, T beta) {
+             const int* __restrict__ sidx, int64_t M, int ntrans, T kc, T beta, int half_h2, int conj_all) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Cplx<T>* tile = reinterpret_cast<Cplx<T>*>(smem_raw);
   const int sub = xcd_remap(blockIdx.x, gridDim.x);
@@ -573,7 +683,7 @@ __global__ void __launch_bounds__(kBlock)
     __syncthreads();
   }
   // halo cells lie within one period of the grid (nf >= 2W), so one conditional wrap suffices
-  stage_tile<T, W, TC>(fine, tile, g, o0, o1, o2, t0, ntrans);
+  stage_tile<T, W, TC>(fine, tile, g, o0, o1, o2, t0, ntrans, half_h2, conj_all);
   int rb[kPer], rr[kPer];
   if constexpr (kBankOrder) {
     const T halfw = (T)(0.5 * W);
@@ -703,7 +813,8 @@ template <int W, int TC>
 __global__ void __launch_bounds__(kBlock)
     k_interp_rows(const Cplx<float>* __restrict__ fine, Cplx<float>* __restrict__ c, SubDesc sd,
                   BinGeom g, const float* __restrict__ sv0, const float* __restrict__ sv1,
-                  const float* __restrict__ sv2, int64_t M, int ntrans, float kc, float beta) {
+                  const float* __restrict__ sv2, int64_t M, int ntrans, float kc, float beta, int half_h2,
+                  int conj_all) {
   static_assert(W <= 8 && TC % 2 == 0, "eight lanes per point hold at most 8 stencil planes");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Cplx<float>* tile = reinterpret_cast<Cplx<float>*>(smem_raw);
@@ -717,7 +828,7 @@ __global__ void __launch_bounds__(kBlock)
   constexpr int E = kSuperEdge + W;
   constexpr int ncell = E * E * E;
   const int o0 = s0 * kSuperEdge - W / 2, o1 = s1 * kSuperEdge - W / 2, o2 = s2 * kSuperEdge - W / 2;
-  stage_tile<float, W, TC>(fine, tile, g, o0, o1, o2, t0, ntrans);
+  stage_tile<float, W, TC>(fine, tile, g, o0, o1, o2, t0, ntrans, half_h2, conj_all);
   __syncthreads();
   const int p0 = sd.sub_pt[sub];
   const int pend = p0 + sd.sub_n[sub];
@@ -1333,6 +1444,13 @@ struct Plan {
   int ldc_cap = 0;        // row capacity of csort
   void* sv[3] = {nullptr, nullptr, nullptr};
   rocfft_plan fft_fwd = nullptr, fft_bwd = nullptr;
+  // real volumes: R2C for type 2, C2R of the folded Hermitian half for type 1 (COV3D_REAL_FFT=0 disables)
+  rocfft_plan fft_r2c = nullptr, fft_c2r = nullptr;
+  int real_fft = 1;
+  void* rbuf = nullptr;       // real grid (type 2) or Hermitian half grid (type 1), allocated on first use
+  size_t rbuf_bytes = 0;
+  int stage_half = 0;         // interpolation reads a Hermitian half grid with this m2 extent (0: full grid)
+  int stage_conj = 0;         // ... conjugated
   rocfft_execution_info info = nullptr;
   void* fft_work = nullptr;
   size_t work_size = 0;
@@ -1394,7 +1512,8 @@ static int launch_interp(Plan* p, void* c_out, int ntrans_run, hipStream_t st) {
       if (p->interp_mode == 2) {
         hipLaunchKernelGGL((k_interp_rows<W, TC>), grid, dim3(kBlock), smem, st, (const Cplx<float>*)p->fine,
                            (Cplx<float>*)p->csort, sd, p->geom, (const float*)p->sv[0], (const float*)p->sv[1],
-                           (const float*)p->sv[2], p->M, ntrans_run, (float)kc, (float)p->beta);
+                           (const float*)p->sv[2], p->M, ntrans_run, (float)kc, (float)p->beta, p->stage_half,
+                           p->stage_conj);
         done = true;
       }
     }
@@ -1406,7 +1525,8 @@ static int launch_interp(Plan* p, void* c_out, int ntrans_run, hipStream_t st) {
     else
       hipLaunchKernelGGL((k_interp<T, W, TC>), grid, dim3(kBlock), smem, st, (const Cplx<T>*)p->fine,
                          (Cplx<T>*)p->csort, sd, p->geom, (const T*)p->sv[0], (const T*)p->sv[1],
-                         (const T*)p->sv[2], p->sidx, p->M, ntrans_run, kc, (T)p->beta);
+                         (const T*)p->sv[2], p->sidx, p->M, ntrans_run, kc, (T)p->beta, p->stage_half,
+                         p->stage_conj);
   }
   COV3D_LAUNCH_CHECK();
   ScopedTiming tm2(kTPermute, st);
@@ -1576,11 +1696,84 @@ static int exec_fft(Plan* p, bool forward_sign, hipStream_t st) {
   return 0;
 }
 
+static int exec_fft_real(Plan* p, rocfft_plan plan, void* in, void* out, hipStream_t st) {
+  COV3D_FFT(rocfft_execution_info_set_stream(p->info, st));
+  ScopedTiming tm(kTFft, st);
+  void* ib[1] = {in};
+  void* ob[1] = {out};
+  COV3D_FFT(rocfft_execute(plan, ib, ob, p->info));
+  return 0;
+}
+
+// Real-transform resources, created on the first real execute: the R2C / C2R plans (rocFFT
+// compiles kernels per plan, so plans that never see a real volume do not pay for them), the
+// staging buffer, and a larger shared work buffer if these plans need one.
+static int ensure_real(Plan* p) {
+  if (!p->fft_r2c) {
+    size_t lengths[3] = {(size_t)p->nf[2], (size_t)p->nf[1], (size_t)p->nf[0]};
+    rocfft_precision prec = p->dtype == 1 ? rocfft_precision_double : rocfft_precision_single;
+    if (rocfft_plan_create(&p->fft_r2c, rocfft_placement_notinplace, rocfft_transform_type_real_forward, prec, 3,
+                           lengths, (size_t)p->ntrans, nullptr) != rocfft_status_success ||
+        rocfft_plan_create(&p->fft_c2r, rocfft_placement_notinplace, rocfft_transform_type_real_inverse, prec, 3,
+                           lengths, (size_t)p->ntrans, nullptr) != rocfft_status_success) {
+      if (p->fft_r2c) rocfft_plan_destroy(p->fft_r2c);
+      p->fft_r2c = nullptr;
+      set_error("rocfft real-transform plan creation failed");
+      return COV3D_ERR_FFT;
+    }
+    size_t w3 = 0, w4 = 0;
+    rocfft_plan_get_work_buffer_size(p->fft_r2c, &w3);
+    rocfft_plan_get_work_buffer_size(p->fft_c2r, &w4);
+    const size_t need = std::max(w3, w4);
+    if (need > p->work_size) {
+      // the old work buffer may still be in use by queued transforms of this plan
+      COV3D_HIP(hipDeviceSynchronize());
+      hipFree(p->fft_work);
+      p->fft_work = nullptr;
+      p->work_size = 0;
+      if (hipMalloc(&p->fft_work, need) != hipSuccess) {
+        set_error("hipMalloc of %zu-byte FFT work buffer failed", need);
+        return COV3D_ERR_HIP;
+      }
+      p->work_size = need;
+      COV3D_FFT(rocfft_execution_info_set_work_buffer(p->info, p->fft_work, p->work_size));
+    }
+  }
+  if (!p->rbuf && hipMalloc(&p->rbuf, p->rbuf_bytes) != hipSuccess) {
+    p->rbuf = nullptr;
+    set_error("hipMalloc of %zu-byte real-transform buffer failed", p->rbuf_bytes);
+    return COV3D_ERR_HIP;
+  }
+  return 0;
+}
+
 template <typename T>
 static int exec_type2(Plan* p, const void* f, bool f_real, void* c, int64_t nrun, hipStream_t st) {
   const int64_t nfine = p->nf[0] * p->nf[1] * p->nf[2];
   const int64_t total = nfine * p->ntrans;
   dim3 g1((unsigned)ceil_div(total, kBlock));
+  if (f_real && p->real_fft && p->interp_mode != 1) {
+    // real volume: real fine grid -> R2C (Hermitian half) -> interpolation mirrors m2 > nf2/2
+    int rc = ensure_real(p);
+    if (rc) return rc;
+    {
+      ScopedTiming tm(kTDeconvPad, st);
+      const int64_t nrows = p->ntrans * p->nf[0] * p->nf[1];
+      hipLaunchKernelGGL((k_deconv_pad_r<T>), dim3((unsigned)ceil_div(nrows, kBlock / 64)), dim3(kBlock), 0, st,
+                         (const T*)f, (T*)p->rbuf, p->ntrans, nrun,
+                         (int)p->N[0], (int)p->N[1], (int)p->N[2], (int)p->nf[0], (int)p->nf[1], (int)p->nf[2],
+                         (const T*)p->invp[0], (const T*)p->invp[1], (const T*)p->invp[2]);
+      COV3D_LAUNCH_CHECK();
+    }
+    rc = exec_fft_real(p, p->fft_r2c, p->rbuf, p->fine, st);
+    if (rc) return rc;
+    p->stage_half = (int)(p->nf[2] / 2 + 1);
+    p->stage_conj = p->iflag2 > 0 ? 1 : 0;  // R2C has the negative sign
+    rc = run_interp(p, c, (int)nrun, st);
+    p->stage_half = 0;
+    p->stage_conj = 0;
+    return rc;
+  }
   {
     ScopedTiming tm(kTDeconvPad, st);
     if (f_real)
@@ -1612,11 +1805,33 @@ static int exec_type1(Plan* p, const void* c, void* f, bool f_real, int64_t nrun
     int rc = run_spread<T>(p, c, (int)nrun, st);
     if (rc) return rc;
   }
+  const int64_t nm = p->N[0] * p->N[1] * p->N[2];
+  dim3 g1((unsigned)ceil_div(nm * nrun, kBlock));
+  if (f_real && p->real_fft) {
+    // real output: Hermitian half of the spread grid -> C2R (positive sign; the half is conjugated
+    // when type 1 takes the negative sign) -> real fine grid -> deconvolved real modes
+    int rc = ensure_real(p);
+    if (rc) return rc;
+    const int64_t nh = p->nf[0] * p->nf[1] * (p->nf[2] / 2 + 1) * p->ntrans;
+    {
+      ScopedTiming tm(kTDeconvCrop, st);
+      hipLaunchKernelGGL((k_fold_half<T>), dim3((unsigned)ceil_div(nh, kBlock)), dim3(kBlock), 0, st,
+                         (const Cplx<T>*)p->fine, (Cplx<T>*)p->rbuf, p->ntrans, (int)p->nf[0], (int)p->nf[1],
+                         (int)p->nf[2], p->iflag2 > 0 ? 1 : 0);
+      COV3D_LAUNCH_CHECK();
+    }
+    rc = exec_fft_real(p, p->fft_c2r, p->rbuf, p->fine, st);
+    if (rc) return rc;
+    ScopedTiming tm(kTDeconvCrop, st);
+    hipLaunchKernelGGL((k_deconv_crop_r<T>), g1, dim3(kBlock), 0, st, (const T*)p->fine, (T*)f, nrun,
+                       (int)p->N[0], (int)p->N[1], (int)p->N[2], (int)p->nf[0], (int)p->nf[1], (int)p->nf[2],
+                       (const T*)p->invp[0], (const T*)p->invp[1], (const T*)p->invp[2]);
+    COV3D_LAUNCH_CHECK();
+    return 0;
+  }
   // type 1 uses the opposite sign of type 2
   int rc = exec_fft(p, !(p->iflag2 < 0), st);
   if (rc) return rc;
-  const int64_t nm = p->N[0] * p->N[1] * p->N[2];
-  dim3 g1((unsigned)ceil_div(nm * nrun, kBlock));
   ScopedTiming tm(kTDeconvCrop, st);
   if (f_real)
     hipLaunchKernelGGL((k_deconv_crop<T, true>), g1, dim3(kBlock), 0, st, (const Cplx<T>*)p->fine,
@@ -1694,8 +1909,11 @@ static int destroy_plan(Plan* p) {
   hipFree(p->sub_start);
   hipFree(p->nsub);
   hipFree(p->fft_work);
+  hipFree(p->rbuf);
   if (p->fft_fwd) rocfft_plan_destroy(p->fft_fwd);
   if (p->fft_bwd) rocfft_plan_destroy(p->fft_bwd);
+  if (p->fft_r2c) rocfft_plan_destroy(p->fft_r2c);
+  if (p->fft_c2r) rocfft_plan_destroy(p->fft_c2r);
   if (p->info) rocfft_execution_info_destroy(p->info);
   delete p;
   return 0;
@@ -1792,6 +2010,12 @@ static int make_plan(int type, const int64_t* n, int iflag, int64_t ntrans, doub
   size_t w1 = 0, w2 = 0;
   rocfft_plan_get_work_buffer_size(p->fft_fwd, &w1);
   rocfft_plan_get_work_buffer_size(p->fft_bwd, &w2);
+  p->real_fft = env_int("COV3D_REAL_FFT", 1) != 0 ? 1 : 0;
+  {
+    const size_t tsz = dtype == 1 ? 8 : 4;
+    p->rbuf_bytes = (size_t)ntrans * std::max<size_t>((size_t)nfine * tsz,
+                                                        (size_t)p->nf[0] * p->nf[1] * (p->nf[2] / 2 + 1) * 2 * tsz);
+  }
   p->work_size = std::max(w1, w2);
   if (p->work_size) {
     if (hipMalloc(&p->fft_work, p->work_size) != hipSuccess) return fail(COV3D_ERR_HIP);
