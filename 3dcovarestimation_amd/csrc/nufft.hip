@@ -645,6 +645,56 @@ __device__ __forceinline__ void stage_tile(const Cplx<T>* __restrict__ fine, Cpl
   }
 }
 
+// Bank-aware visiting order of the points of each interpolation subproblem (FP32 tiles of two
+// transforms per 16-byte cell, W <= 8).  All lanes of k_interp read the same tap offset together,
+// and ds_read_b128 serves 16 lanes per clock from 64 banks (measured), so a read is conflict-free
+// when the 16 lanes' base cells differ mod 16.  Points are bucketed by base cell mod 16 and dealt
+// round-robin over the buckets: bucket b, rank r goes to slot sum_b' min(c_b', r) +
+// #{b' < b : c_b' > r}.  iord[p0 + slot] is the subproblem-local index of the point at that slot.
+// The order depends on the points only, so it is computed once per setpts.
+template <int W>
+__global__ void __launch_bounds__(kBlock)
+    k_bank_order(SubDesc sd, const float* __restrict__ sv0, const float* __restrict__ sv1,
+                 const float* __restrict__ sv2, int* __restrict__ iord) {
+  const int sub = blockIdx.x;
+  if (sub >= *sd.nsub) return;
+  constexpr int E = kSuperEdge + W;
+  constexpr int kPer = kMaxSub / kBlock;
+  __shared__ int s_cnt[16];
+  const int p0 = sd.sub_pt[sub];
+  const int np = sd.sub_n[sub];
+  if (threadIdx.x < 16) s_cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const float halfw = 0.5f * (float)W;
+  int rb[kPer], rr[kPer];
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int slot = threadIdx.x + k * kBlock;
+    if (slot < np) {
+      const int p = p0 + slot;
+      const int m0 = min(max((int)ceilf(sv0[p] - halfw) + W / 2, 0), kSuperEdge);
+      const int m1 = min(max((int)ceilf(sv1[p] - halfw) + W / 2, 0), kSuperEdge);
+      const int m2 = min(max((int)ceilf(sv2[p] - halfw) + W / 2, 0), kSuperEdge);
+      rb[k] = ((m0 * E + m1) * E + m2) & 15;
+      rr[k] = atomicAdd(&s_cnt[rb[k]], 1);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int slot = threadIdx.x + k * kBlock;
+    if (slot < np) {
+      int pos = 0;
+#pragma unroll
+      for (int b = 0; b < 16; ++b) {
+        const int cb = s_cnt[b];
+        pos += min(cb, rr[k]) + ((b < rb[k] && cb > rr[k]) ? 1 : 0);
+      }
+      iord[p0 + pos] = slot;
+    }
+  }
+}
+
 // Type-2 interpolation: c[t, j] = sum_taps phi * fine[t, tap].
 // One workgroup per (super-bin subproblem, transform chunk).  The 8^3 tile
 // plus a W-cell halo is staged in LDS as [cell][TC/2] 16-byte words holding
@@ -654,7 +704,8 @@ template <typename T, int W, int TC>
 __global__ void __launch_bounds__(kBlock)
     k_interp(const Cplx<T>* __restrict__ fine, Cplx<T>* __restrict__ c, SubDesc sd, BinGeom g,
              const T* __restrict__ sv0, const T* __restrict__ sv1, const T* __restrict__ sv2,
-             const int* __restrict__ sidx, int64_t M, int ntrans, T kc, T beta, int half_h2, int conj_all) {
+             const int* __restrict__ sidx, int64_t M, int ntrans, T kc, T beta, int half_h2, int conj_all,
+             const int* __restrict__ iord) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Cplx<T>* tile = reinterpret_cast<Cplx<T>*>(smem_raw);
   const int sub = xcd_remap(blockIdx.x, gridDim.x);
@@ -669,56 +720,12 @@ __global__ void __launch_bounds__(kBlock)
   const int o0 = s0 * kSuperEdge - W / 2, o1 = s1 * kSuperEdge - W / 2, o2 = s2 * kSuperEdge - W / 2;
   const int p0 = sd.sub_pt[sub];
   const int np = sd.sub_n[sub];
-  // Bank-aware point order (FP32, two transforms per 16-byte cell).  All lanes read the same tap
-  // offset together, and ds_read_b128 serves 16 lanes per clock from 64 banks, so a read is
-  // conflict-free when the 16 lanes' base cells differ mod 16.  Points are bucketed by base
-  // cell mod 16 and dealt round-robin over the buckets: bucket b, rank r goes to slot
-  // sum_b' min(c_b', r) + #{b' < b : c_b' > r}.
-  constexpr bool kBankOrder = TC == 2 && sizeof(T) == 4 && W <= 8;
-  constexpr int kPer = kMaxSub / kBlock;
-  __shared__ int s_order[kBankOrder ? kMaxSub : 1];
-  __shared__ int s_cnt[16];
-  if constexpr (kBankOrder) {
-    if (threadIdx.x < 16) s_cnt[threadIdx.x] = 0;
-    __syncthreads();
-  }
+  // Points are visited in the bank-aware order computed at setpts (k_bank_order), when present.
   // halo cells lie within one period of the grid (nf >= 2W), so one conditional wrap suffices
   stage_tile<T, W, TC>(fine, tile, g, o0, o1, o2, t0, ntrans, half_h2, conj_all);
-  int rb[kPer], rr[kPer];
-  if constexpr (kBankOrder) {
-    const T halfw = (T)(0.5 * W);
-#pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-      const int slot = threadIdx.x + k * kBlock;
-      if (slot < np) {
-        const int p = p0 + slot;
-        const int m0 = min(max((int)ceil(sv0[p] - halfw) + W / 2, 0), kSuperEdge);
-        const int m1 = min(max((int)ceil(sv1[p] - halfw) + W / 2, 0), kSuperEdge);
-        const int m2 = min(max((int)ceil(sv2[p] - halfw) + W / 2, 0), kSuperEdge);
-        rb[k] = ((m0 * E + m1) * E + m2) & 15;
-        rr[k] = atomicAdd(&s_cnt[rb[k]], 1);
-      }
-    }
-  }
   __syncthreads();
-  if constexpr (kBankOrder) {
-#pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-      const int slot = threadIdx.x + k * kBlock;
-      if (slot < np) {
-        int pos = 0;
-#pragma unroll
-        for (int b = 0; b < 16; ++b) {
-          const int cb = s_cnt[b];
-          pos += min(cb, rr[k]) + ((b < rb[k] && cb > rr[k]) ? 1 : 0);
-        }
-        s_order[pos] = slot;
-      }
-    }
-    __syncthreads();
-  }
   for (int slot = threadIdx.x; slot < np; slot += blockDim.x) {
-    const int p = p0 + (kBankOrder ? s_order[slot] : slot);
+    const int p = p0 + (iord ? iord[p0 + slot] : slot);
     T w0[W], w1[W], w2[W];
     int l0, l1, l2;
     stencil_weights<T, W>(sv0[p], kc, beta, l0, w0);
@@ -1440,6 +1447,8 @@ struct Plan {
   int* pt_pos = nullptr;
   int* sidx = nullptr;
   int* spos = nullptr;
+  int* iord = nullptr;        // bank-aware interpolation order (FP32, two transforms per cell, W <= 8)
+  bool iord_valid = false;
   void* csort = nullptr;  // sorted-order staging of point values (M * ldc_cap complex)
   int ldc_cap = 0;        // row capacity of csort
   void* sv[3] = {nullptr, nullptr, nullptr};
@@ -1464,6 +1473,9 @@ static int free_points(Plan* p) {
   hipFree(p->pt_pos);
   hipFree(p->sidx);
   hipFree(p->spos);
+  hipFree(p->iord);
+  p->iord = nullptr;
+  p->iord_valid = false;
   hipFree(p->csort);
   for (int k = 0; k < 3; ++k) hipFree(p->sv[k]);
   hipFree(p->sub_sb);
@@ -1526,7 +1538,7 @@ static int launch_interp(Plan* p, void* c_out, int ntrans_run, hipStream_t st) {
       hipLaunchKernelGGL((k_interp<T, W, TC>), grid, dim3(kBlock), smem, st, (const Cplx<T>*)p->fine,
                          (Cplx<T>*)p->csort, sd, p->geom, (const T*)p->sv[0], (const T*)p->sv[1],
                          (const T*)p->sv[2], p->sidx, p->M, ntrans_run, kc, (T)p->beta, p->stage_half,
-                         p->stage_conj);
+                         p->stage_conj, (TC == 2 && p->iord_valid) ? (const int*)p->iord : nullptr);
   }
   COV3D_LAUNCH_CHECK();
   ScopedTiming tm2(kTPermute, st);
@@ -1858,6 +1870,7 @@ static int setpts_impl(Plan* p, int64_t M, const T* x, const T* y, const T* z, h
     COV3D_HIP(hipMalloc(&p->pt_pos, Ma * sizeof(int)));
     COV3D_HIP(hipMalloc(&p->sidx, Ma * sizeof(int)));
     COV3D_HIP(hipMalloc(&p->spos, Ma * sizeof(int)));
+    COV3D_HIP(hipMalloc(&p->iord, Ma * sizeof(int)));
     // interpolation writes whole chunks of up to 4 transforms (ntrans rounded up to 4); the
     // spreading input uses row stride gather_ldc(n) for any n <= ntrans
     COV3D_HIP(hipMalloc(&p->csort, Ma * p->ldc_cap * (p->dtype == 1 ? 16 : 8)));
@@ -1894,6 +1907,27 @@ static int setpts_impl(Plan* p, int64_t M, const T* x, const T* y, const T* z, h
                        p->pt_pos, p->bin_start, p->sidx, p->spos, (T*)p->sv[0], (T*)p->sv[1],
                        (T*)p->sv[2]);
     COV3D_LAUNCH_CHECK();
+  }
+  p->iord_valid = false;
+  if constexpr (sizeof(T) == 4) {
+    if (M > 0 && p->tc == 2 && p->w <= 8) {
+      SubDesc sd{p->sub_sb, p->sub_pt, p->sub_n, p->nsub};
+      const float* s0 = (const float*)p->sv[0];
+      const float* s1 = (const float*)p->sv[1];
+      const float* s2 = (const float*)p->sv[2];
+      const dim3 gsub((unsigned)p->sub_cap);
+      switch (p->w) {
+#define COV3D_W(WW)                                                                                      \
+  case WW:                                                                                               \
+    hipLaunchKernelGGL((k_bank_order<WW>), gsub, dim3(kBlock), 0, st, sd, s0, s1, s2, p->iord); \
+    break;
+        COV3D_W(2) COV3D_W(3) COV3D_W(4) COV3D_W(5) COV3D_W(6) COV3D_W(7) COV3D_W(8)
+#undef COV3D_W
+        default: break;
+      }
+      COV3D_LAUNCH_CHECK();
+      p->iord_valid = true;
+    }
   }
   p->pts_set = true;
   return 0;
