@@ -179,3 +179,49 @@ def test_kernel_variants_match_nudft(dev, monkeypatch, spread_mode, interp_mode,
     assert rel_l2(out1, ref1) < TOL_F32
     # deterministic: the gather kernels write each cell once
     assert np.array_equal(out1, plan.execute_adjoint(c).cpu().numpy())
+
+
+@pytest.mark.parametrize("dtype_code,tol", [(0, TOL_F32), (1, 1e-9)])
+@pytest.mark.parametrize("iflag", [-1, 1])
+def test_c_abi_both_signs_real_and_complex(dev, dtype_code, tol, iflag):
+    """Type 2 / type 1 through the C ABI for both transform signs, with real volumes (the R2C /
+    C2R half-grid path, whose conjugations depend on the sign) and complex volumes (C2C)."""
+    import ctypes
+
+    o = oracle()
+    native = pkg("_native")
+    lib = native.load()
+    L, r = 10, 3
+    tdt = torch.float32 if dtype_code == 0 else torch.float64
+    cdt = torch.complex64 if dtype_code == 0 else torch.complex128
+    pts = _points(L, 2, 1, 400 + iflag, dev, dtype=tdt, spread=1.2)
+    P = pts.transpose(0, 1).reshape(3, -1)
+    P = (torch.remainder(P + torch.pi, 2 * torch.pi) - torch.pi).contiguous()
+    M = P.shape[1]
+    h = ctypes.c_void_p()
+    eps = 1e-6 if dtype_code == 0 else 1e-12
+    native.check(lib.cov3d_nufft_makeplan(0, L, L, L, iflag, r, eps, dtype_code, ctypes.byref(h)))
+    st = native.stream_ptr(dev)
+    try:
+        native.check(lib.cov3d_nufft_setpts(h, M, P[0].data_ptr(), P[1].data_ptr(), P[2].data_ptr(), st))
+        P64 = P.cpu().numpy().astype(np.float64)
+        g = torch.Generator().manual_seed(3 + iflag)
+        vol = torch.randn((r, L, L, L), generator=g, dtype=torch.float64).to(dev).to(tdt).contiguous()
+        volc = torch.complex(vol, torch.randn((r, L, L, L), generator=g, dtype=torch.float64).to(dev).to(tdt))
+        for v, is_real in ((vol, 1), (volc.contiguous(), 0)):
+            c = torch.empty((r, M), dtype=cdt, device=dev)
+            native.check(lib.cov3d_nufft_execute_dir(h, 2, c.data_ptr(), v.data_ptr(), is_real, r, st))
+            ref = o.nudft_type2(v.cpu().numpy().astype(np.complex128), P64, iflag=iflag)
+            assert rel_l2(c.cpu().numpy(), ref) < tol, ("type2", is_real)
+        cin = torch.complex(torch.randn((r, M), generator=g, dtype=torch.float64),
+                            torch.randn((r, M), generator=g, dtype=torch.float64)).to(dev).to(cdt).contiguous()
+        ref1 = o.nudft_type1(cin.cpu().numpy().astype(np.complex128), P64, (L, L, L), iflag=-iflag)
+        fr = torch.empty((r, L, L, L), dtype=tdt, device=dev)
+        native.check(lib.cov3d_nufft_execute_dir(h, 1, cin.data_ptr(), fr.data_ptr(), 1, r, st))
+        assert rel_l2(fr.cpu().numpy(), ref1.real) < tol, "type1 real"
+        fc = torch.empty((r, L, L, L), dtype=cdt, device=dev)
+        native.check(lib.cov3d_nufft_execute_dir(h, 1, cin.data_ptr(), fc.data_ptr(), 0, r, st))
+        assert rel_l2(fc.cpu().numpy(), ref1) < tol, "type1 complex"
+    finally:
+        torch.cuda.synchronize(dev)
+        lib.cov3d_nufft_destroy(h)
