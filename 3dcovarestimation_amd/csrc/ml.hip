@@ -17,6 +17,9 @@
 namespace cov3d {
 
 constexpr int kMlBlock = 256;
+// The split-BF16 forward runs 16 waves per image: each Gram iteration waits on a dependent load
+// chain (pixel map -> CTF, values), so more waves per workgroup hide it.
+constexpr int kMlBlockMfma = 1024;
 constexpr int kRmax = 32;  // rows of the Gram tile (r + 1 <= 32)
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -114,7 +117,7 @@ __device__ void chol_solve(const double* A, double* x, int r) {
 }
 
 template <int PREC>
-__global__ void __launch_bounds__(kMlBlock)
+__global__ void __launch_bounds__(PREC == 1 ? kMlBlockMfma : kMlBlock)
     k_ml_forward(const float2* __restrict__ c, const float* __restrict__ ctf,
                  const float2* __restrict__ yhat, const float* __restrict__ lam_inv,
                  const int* __restrict__ pix_map, int64_t np, int64_t B,
@@ -134,15 +137,16 @@ __global__ void __launch_bounds__(kMlBlock)
   const int wave = tid >> 6;
   const int lane = tid & 63;
 
-  if (PREC == 1) {
-    // ---- split-BF16 MFMA Gram: wave w covers 32-pixel chunks w, w+4, ...
+  if constexpr (PREC == 1) {
+    // ---- split-BF16 MFMA Gram: wave w of nw covers 32-pixel chunks w, w + nw, ...
+    const int nw = (int)(blockDim.x >> 6);
     const int m = lane & 31;
     const int h = lane >> 5;
     f32x16 acc;
 #pragma unroll
     for (int v = 0; v < 16; ++v) acc[v] = 0.f;
     const int64_t nchunk = (np + 31) / 32;
-    for (int64_t ch = wave; ch < nchunk; ch += 4) {
+    for (int64_t ch = wave; ch < nchunk; ch += nw) {
       const int64_t k0 = ch * 32 + 16 * h;
       float2 vals[16];
 #pragma unroll
@@ -168,14 +172,20 @@ __global__ void __launch_bounds__(kMlBlock)
       }
     }
     // D layout of the 32x32 tile: lane l, register v -> row 8*(v/4) + 4*(l/32) + v%4, col l%32.
+    // Waves are folded into the 4 partial tiles in rounds: round j adds waves 4j .. 4j + 3.
+    for (int j = 0; 4 * j < nw; ++j) {
+      if ((wave >> 2) == j) {
 #pragma unroll
-    for (int v = 0; v < 16; ++v) {
-      int row = 8 * (v >> 2) + 4 * h + (v & 3);
-      s_part[wave][row * kRmax + m] = acc[v];
+        for (int v = 0; v < 16; ++v) {
+          int row = 8 * (v >> 2) + 4 * h + (v & 3);
+          float* dst = &s_part[wave & 3][row * kRmax + m];
+          *dst = (j == 0 ? 0.f : *dst) + acc[v];
+        }
+      }
+      __syncthreads();
     }
-    __syncthreads();
   } else {
-    // ---- FP32 Gram: stage 64 pixels of all rows in LDS; thread k owns the
+    // ---- FP32 Gram (256 threads): stage 64 pixels of all rows in LDS; thread k owns the
     // upper-triangle pairs k, k + 256, k + 512 ((r+1)(r+2)/2 <= 528).
     __shared__ float2 s_rows[kRmax][64];
     const int npair = (r + 1) * (r + 2) / 2;
@@ -335,7 +345,7 @@ extern "C" int cov3d_ml_forward(const void* c, const void* ctf, const void* yhat
   const int64_t np = pix_map ? npix : L * L;
   hipStream_t st = cov3d::as_stream(stream);
   if (precision == 1)
-    hipLaunchKernelGGL((cov3d::k_ml_forward<1>), dim3((unsigned)B), dim3(cov3d::kMlBlock), 0, st,
+    hipLaunchKernelGGL((cov3d::k_ml_forward<1>), dim3((unsigned)B), dim3(cov3d::kMlBlockMfma), 0, st,
                        (const float2*)c, (const float*)ctf, (const float2*)yhat,
                        (const float*)lam_inv, pix_map, np, B, (int)r, (int)L, (float)noise_var, mean_const,
                        (float*)cost, (float*)z, (float*)m_out, (float*)minv, (float*)b_out);

@@ -99,6 +99,42 @@ def sgd_run(pkg, data, L, r, B, steps, warmup, form, timing=False):
     return dt, kern
 
 
+def contraction_run(pkg, data, L, r, B, reps=20):
+    """ml_forward (split-BF16 MFMA Gram + FP64 Cholesky) on one batch's half-plane type-2 values."""
+    import torch
+
+    npl = importlib.import_module(pkg + ".nufft_plan")
+    ml = importlib.import_module(pkg + ".ml_ops")
+    pf = importlib.import_module(pkg + ".projection_funcs")
+    dev = data.dataset.images.device
+    images, pts, filt, _ = data.dataset[0:B]
+    pix = ml.half_plane_pixels(L, dev)
+    plan = npl.NufftPlan((L,) * 3, batch_size=r, device=dev)
+    plan.set_points_flat(pts.transpose(0, 1).reshape(3, B, L * L)[:, :, pix.long()].reshape(3, -1), B)
+    vols = torch.randn((r, L, L, L), device=dev) / L
+    c = plan.execute_forward_flat(vols)[:r].reshape(r, B, pix.numel())
+    yhat = pf.centered_fft2(images.float()) / L
+    out = {}
+    for name, prec in (("bf16x3", ml.PRECISION_BF16X3), ("fp32", ml.PRECISION_FP32)):
+        ml.ml_forward(c, yhat, L, 1.0, ctf=filt, precision=prec, pix_map=pix)
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record()
+        for _ in range(reps):
+            ml.ml_forward(c, yhat, L, 1.0, ctf=filt, precision=prec, pix_map=pix)
+        ev1.record()
+        torch.cuda.synchronize()
+        ms = ev0.elapsed_time(ev1) / reps
+        npx = pix.numel()
+        # useful Gram flops: (r+1)^2 products over 2*npix reals, x3 for the split-BF16 products
+        useful = B * (r + 1) ** 2 * 2 * (2 * npx) * (3 if prec == ml.PRECISION_BF16X3 else 1)
+        read = B * (r * npx * 8 + npx * 8 + npx * 4)
+        out[name] = {"ms": round(ms, 3), "tflops": round(useful / (ms * 1e-3) / 1e12, 2),
+                     "mfma_frac_bf16_peak": round(useful / (ms * 1e-3) / 1e12 / 2500.0, 4) if prec else None,
+                     "GBs": round(read / (ms * 1e-3) / 1e9, 1)}
+    del plan
+    return out
+
+
 def subspace_run(pkg, data, L, r, B, n_batches):
     """Subspace of the chosen and plain forms vs an FP64 trajectory (same start, same batches)."""
     import torch
@@ -268,6 +304,16 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         log("copy probe: 3 x 1 GiB device-to-device copies")
+    contr = None
+    try:
+        contr = contraction_run(pkg, data, L, r, B)
+        log(f"contractions: {contr}")
+    except Exception as e:  # reported, never required
+        contr = {"error": repr(e)}
+    # achieved algorithmic GB/s of both NUFFT kernels
+    per_kernel = {k: {"avg_ms": round(kern[k]["avg_ms"], 3),
+                      "achieved_GBs": round(bytes_per_launch / (kern[k]["avg_ms"] * 1e-3) / 1e9, 1)}
+                  for k in ("interp", "spread") if kern[k]["avg_ms"]}
     sub = None
     if args.subspace_batches > 0:
         log(f"subspace agreement vs FP64 over {args.subspace_batches} steps")
@@ -307,6 +353,8 @@ def main():
         "speedup_vs_plain": round(ips / plain["value"], 3) if plain else None,
         "kernels_ms": {k: round(v["avg_ms"], 3) if v["avg_ms"] else None for k, v in kern.items()},
         "roofline": roof,
+        "nufft_kernels": per_kernel,
+        "contractions": contr,
         "subspace_error": sub,
         "mean_cg": cg,
         "cpu_baseline": cpu,
