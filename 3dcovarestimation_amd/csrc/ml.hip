@@ -251,26 +251,25 @@ __global__ void __launch_bounds__(PREC == 1 ? kMlBlockMfma : kMlBlock)
   }
   __syncthreads();
   block_cholesky(s_A, r);
+  // The triangular solves keep their vectors in LDS (s_z, and column tid of s_minv): private
+  // arrays indexed by the runtime rank would live in scratch memory.
   if (tid == 0) {
-    double bvec[kRmax];
-    for (int a = 0; a < r; ++a) bvec[a] = s_G[a * (kRmax + 1) + r] * inv_nv;
-    double x[kRmax];
-    for (int a = 0; a < r; ++a) x[a] = bvec[a];
-    chol_solve(s_A, x, r);
+    for (int a = 0; a < r; ++a) s_z[a] = s_G[a * (kRmax + 1) + r] * inv_nv;
+    chol_solve(s_A, s_z, r);
     double bz = 0, logdet = 0;
     for (int a = 0; a < r; ++a) {
-      s_z[a] = x[a];
-      bz += bvec[a] * x[a];
+      const double ba = s_G[a * (kRmax + 1) + r] * inv_nv;
+      bz += ba * s_z[a];
       logdet += 2.0 * log(s_A[a * (kRmax + 1) + a]);
-      if (b_out) b_out[i * r + a] = (float)bvec[a];
-      if (z_out) z_out[i * r + a] = (float)x[a];
+      if (b_out) b_out[i * r + a] = (float)ba;
+      if (z_out) z_out[i * r + a] = (float)s_z[a];
     }
     double cst = 0.5 * (-bz + logdet);
     if (mean_const) cst += 0.5 * s_G[r * (kRmax + 1) + r] * inv_nv;
     if (cost) cost[i] = (float)cst;
   }
   if (minv_out && tid < r) {
-    double x[kRmax];
+    double* x = s_minv + tid * (kRmax + 1);
     for (int a = 0; a < r; ++a) x[a] = (a == tid) ? 1.0 : 0.0;
     chol_solve(s_A, x, r);
     for (int a = 0; a < r; ++a) minv_out[(i * r + a) * r + tid] = (float)x[a];

@@ -1305,6 +1305,240 @@ __global__ void __launch_bounds__(kGatherBlock)
   }
 }
 
+// Type-1 spreading of one 4^3 bin as an FP32 matrix product on the matrix
+// cores (v_mfma_f32_16x16x4_f32, full FP32 products and sums):
+//   F[cell][col] = sum_p W[cell][p] V[p][col],  W[cell][p] = wx_p[c0] wy_p[c1] wz_p[c2],
+// with the bin's 64 cells as 4 row blocks of 16 (c0 = block, (c1, c2) = row / 4, row % 4),
+// the 2 TC real columns of the point values as NCB column blocks of 16, and the hits
+// 4 per step.  One 64-lane workgroup per bin (kernel width W <= 8, so the 27
+// neighbour bins hold every point that reaches it).  Lane n < 27 loads neighbour
+// n's point range; the ranges are walked as one flat index space, 64 points per
+// chunk, the next chunk's coordinates loaded while the current one is processed.
+// Each chunk's hits are appended to an LDS list (coordinates, the neighbour's
+// block offset, sorted index) and lane f of the new entries evaluates weight
+// f % 12 of hit f / 12 (4 cells on each axis).  Whenever 64 hits are pending
+// they are consumed 4 per step: each lane forms its A entries from 3 LDS reads and
+// loads its B entry from the point's sorted value row, two steps ahead.  Columns
+// past 2 TC read whatever follows the row (the buffer carries a tail of slack) and
+// only feed output columns that are never stored.  Each cell
+// is written once (deterministic, no atomics); a lane pair exchanges halves so
+// that every store is 16 bytes.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+constexpr int kHitCap = 128;
+constexpr int kCsortSlack = 64;  // complex values of slack after the sorted value rows
+
+template <int TC>
+__global__ void __launch_bounds__(kGatherBlock)
+    k_spread_mfma(const Cplx<float>* __restrict__ cs, Cplx<float>* __restrict__ fine, BinGeom g,
+                  const int* __restrict__ bin_start, const int* __restrict__ bin_count,
+                  const float* __restrict__ sv0, const float* __restrict__ sv1,
+                  const float* __restrict__ sv2, int ntrans, int ldc, int W, float kc, float beta_l2) {
+  constexpr int NC = 2 * TC;
+  constexpr int NCB = (NC + 15) / 16;
+  constexpr int kNb = 27;
+  __shared__ float4 s_rec[kHitCap];  // p0, p1, p2, packed block offsets q0 | q1 << 10 | q2 << 20 (+512 each)
+  __shared__ int s_idx[kHitCap];
+  __shared__ float s_w[kHitCap * 12];  // per hit: wx[0..3], wy[0..3], wz[0..3]
+  const int t0 = blockIdx.y * TC;
+  const int lane = threadIdx.x;
+  int b0, b1, b2;
+  gather_bin(blockIdx.x, gridDim.x, g, b0, b1, b2);
+  const float halfw = 0.5f * (float)W;
+  const int grp = lane >> 4, sub = lane & 15;
+  f32x4 acc[4][NCB];
+#pragma unroll
+  for (int c0 = 0; c0 < 4; ++c0)
+#pragma unroll
+    for (int cb = 0; cb < NCB; ++cb) acc[c0][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  // ---- neighbour table: lane n < 27 holds neighbour n's first sorted point, the exclusive
+  // prefix of the point counts and the packed offset of this block in the neighbour's frame
+  int nb_start = 0, nb_cnt = 0, nb_pack = 0;
+  if (lane < kNb) {
+    int n0 = b0 + lane / 9 - 1, n1 = b1 + (lane / 3) % 3 - 1, n2 = b2 + lane % 3 - 1;
+    int sh0 = 0, sh1 = 0, sh2 = 0;
+    if (n0 < 0) { n0 += g.nb[0]; sh0 = -g.nf[0]; }
+    if (n0 >= g.nb[0]) { n0 -= g.nb[0]; sh0 = g.nf[0]; }
+    if (n1 < 0) { n1 += g.nb[1]; sh1 = -g.nf[1]; }
+    if (n1 >= g.nb[1]) { n1 -= g.nb[1]; sh1 = g.nf[1]; }
+    if (n2 < 0) { n2 += g.nb[2]; sh2 = -g.nf[2]; }
+    if (n2 >= g.nb[2]) { n2 -= g.nb[2]; sh2 = g.nf[2]; }
+    const int nbin = bin_index(g, n0, n1, n2);
+    nb_start = bin_start[nbin];
+    nb_cnt = bin_count[nbin];
+    // the block's first cell in the neighbour's super-bin frame (unwrapped across the boundary)
+    const int q0 = kBinEdge * b0 - (n0 >> 1) * kSuperEdge - sh0;
+    const int q1 = kBinEdge * b1 - (n1 >> 1) * kSuperEdge - sh1;
+    const int q2 = kBinEdge * b2 - (n2 >> 1) * kSuperEdge - sh2;
+    nb_pack = (q0 + 512) | ((q1 + 512) << 10) | ((q2 + 512) << 20);
+  }
+  int incl = nb_cnt;
+#pragma unroll
+  for (int off = 1; off < 32; off <<= 1) {
+    const int t = __shfl_up(incl, off);
+    if (lane >= off) incl += t;
+  }
+  const int nb_pre = incl - nb_cnt;
+  const int total = __builtin_amdgcn_readlane(incl, kNb - 1);
+
+  auto load_b = [&](int slot, float* bv) {
+    const float* row = reinterpret_cast<const float*>(cs + (int64_t)s_idx[slot] * ldc + t0);
+#pragma unroll
+    for (int cb = 0; cb < NCB; ++cb) bv[cb] = row[sub + 16 * cb];
+  };
+  auto mma = [&](int slot, const float* bv) {
+    const float4 wx = *reinterpret_cast<const float4*>(&s_w[slot * 12]);
+    const float wyz = s_w[slot * 12 + 4 + (sub >> 2)] * s_w[slot * 12 + 8 + (sub & 3)];
+    const float a[4] = {wx.x * wyz, wx.y * wyz, wx.z * wyz, wx.w * wyz};
+#pragma unroll
+    for (int c0 = 0; c0 < 4; ++c0)
+#pragma unroll
+      for (int cb = 0; cb < NCB; ++cb)
+        acc[c0][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[c0], bv[cb], acc[c0][cb], 0, 0, 0);
+  };
+  // steps over hit slots [0, nfull), nfull a multiple of 8; the B entries of step s + 2 are
+  // loaded during step s (two register sets, no copies between them).  The loads are issued
+  // unconditionally (the last ones repeat valid slots) so that each wait covers exactly the
+  // set it needs.
+  auto steps = [&](int nfull) {
+    float ba[NCB], bb[NCB];
+    load_b(grp, ba);
+    load_b(4 + grp, bb);
+    for (int s = 0; s < nfull; s += 8) {
+      const int sn = s + 8 < nfull ? s + 8 : s;
+      mma(s + grp, ba);
+      load_b(sn + grp, ba);
+      mma(s + 4 + grp, bb);
+      load_b(sn + 4 + grp, bb);
+    }
+  };
+  // flat index f of the neighbourhood -> sorted point index and packed offset; ncur is the
+  // neighbour of the chunk's first index (wave-uniform)
+  auto locate = [&](int base, int ncur, int& pi, int& pk, int& nlast) {
+    const int f = base + lane;
+    int n = ncur;
+    for (int m = ncur + 1; m < kNb; ++m) {
+      const int pm = __builtin_amdgcn_readlane(nb_pre, m);
+      if (pm > base + kGatherBlock - 1) break;
+      if (f >= pm) n = m;
+    }
+    pi = __shfl(nb_start, n) + f - __shfl(nb_pre, n);
+    pk = __shfl(nb_pack, n);
+    nlast = __builtin_amdgcn_readlane(n, kGatherBlock - 1);
+  };
+
+  int nh = 0;  // pending hits in the list (wave-uniform)
+  int ncur = 0;
+  int pi_n = 0, pk_n = 0;
+  float v0n = 0.f, v1n = 0.f, v2n = 0.f;
+  if (total > 0) {
+    locate(0, 0, pi_n, pk_n, ncur);
+    if (lane < total) {
+      v0n = sv0[pi_n];
+      v1n = sv1[pi_n];
+      v2n = sv2[pi_n];
+    }
+  }
+  for (int base = 0; base < total; base += kGatherBlock) {
+    const int pi = pi_n, pk = pk_n;
+    const float v0 = v0n, v1 = v1n, v2 = v2n;
+    const bool valid = base + lane < total;
+    if (base + kGatherBlock < total) {
+      locate(base + kGatherBlock, ncur, pi_n, pk_n, ncur);
+      if (base + kGatherBlock + lane < total) {
+        v0n = sv0[pi_n];
+        v1n = sv1[pi_n];
+        v2n = sv2[pi_n];
+      }
+    }
+    bool hit = false;
+    if (valid) {
+      const int q0 = (pk & 1023) - 512, q1 = ((pk >> 10) & 1023) - 512, q2 = ((pk >> 20) & 1023) - 512;
+      const int j0 = (int)ceilf(v0 - halfw), j1 = (int)ceilf(v1 - halfw), j2 = (int)ceilf(v2 - halfw);
+      hit = !(q0 > j0 + W - 1 || q0 + 3 < j0 || q1 > j1 + W - 1 || q1 + 3 < j1 || q2 > j2 + W - 1 ||
+              q2 + 3 < j2);
+    }
+    const unsigned long long mask = __ballot(hit);
+    const int nnew = __popcll(mask);
+    if (nnew == 0) continue;
+    if (hit) {
+      const int r = nh + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+      s_rec[r] = make_float4(v0, v1, v2, __int_as_float(pk));
+      s_idx[r] = pi;
+    }
+    __syncthreads();
+    for (int f = lane; f < 12 * nnew; f += kGatherBlock) {
+      const int slot = nh + f / 12, e = f % 12, ax = e >> 2, c = e & 3;
+      const float4 rec = s_rec[slot];
+      const float p = ax == 0 ? rec.x : (ax == 1 ? rec.y : rec.z);
+      const int qa = ((__float_as_int(rec.w) >> (10 * ax)) & 1023) - 512;
+      const int x = qa + c - (int)ceilf(p - halfw);
+      s_w[slot * 12 + e] = (x >= 0 && x < W) ? es_eval_native((float)(qa + c) - p, kc, beta_l2) : 0.f;
+    }
+    __syncthreads();
+    nh += nnew;
+    if (nh >= kGatherBlock) {
+      const int nfull = nh & ~7;
+      steps(nfull);
+      const int rem = nh - nfull;  // < 8: moved to the front of the list
+      float4 rec;
+      int idx = 0;
+      float w[12];
+      if (lane < rem) {
+        rec = s_rec[nfull + lane];
+        idx = s_idx[nfull + lane];
+#pragma unroll
+        for (int e = 0; e < 12; ++e) w[e] = s_w[(nfull + lane) * 12 + e];
+      }
+      __syncthreads();
+      if (lane < rem) {
+        s_rec[lane] = rec;
+        s_idx[lane] = idx;
+#pragma unroll
+        for (int e = 0; e < 12; ++e) s_w[lane * 12 + e] = w[e];
+      }
+      __syncthreads();
+      nh = rem;
+    }
+  }
+  if (nh > 0) {
+    // pad the last steps with zero-weight entries that read a valid row
+    const int nfull = (nh + 7) & ~7;
+    if (lane >= nh && lane < nfull) {
+      s_idx[lane] = s_idx[0];
+#pragma unroll
+      for (int e = 0; e < 12; ++e) s_w[lane * 12 + e] = 0.f;
+    }
+    __syncthreads();
+    steps(nfull);
+  }
+
+  // D layout: lane l, register v -> cell (c0, c1 = l / 16, c2 = v), column l % 16 (+ 16 cb).
+  // Lanes 2m (real parts) and 2m + 1 (imaginary parts) of one transform swap halves, then the
+  // even lane stores cells c2 = 0, 1 and the odd lane cells 2, 3 as complex pairs.
+  const int64_t nfine = (int64_t)g.nf[0] * g.nf[1] * g.nf[2];
+  const bool odd = sub & 1;
+  const int g1 = kBinEdge * b1 + grp;
+  const int g2 = kBinEdge * b2 + (odd ? 2 : 0);
+#pragma unroll
+  for (int cb = 0; cb < NCB; ++cb) {
+    const int tt = (sub + 16 * cb) >> 1;
+    const int t = t0 + tt;
+#pragma unroll
+    for (int c0 = 0; c0 < 4; ++c0) {
+      const f32x4 d = acc[c0][cb];
+      const float r0 = __shfl_xor(odd ? d[0] : d[2], 1);
+      const float r1 = __shfl_xor(odd ? d[1] : d[3], 1);
+      const float4 out = odd ? make_float4(r0, d[2], r1, d[3]) : make_float4(d[0], r0, d[1], r1);
+      if (tt < TC && t < ntrans) {
+        const int64_t off = ((int64_t)(kBinEdge * b0 + c0) * g.nf[1] + g1) * g.nf[2] + g2;
+        *reinterpret_cast<float4*>(fine + t * nfine + off) = out;
+      }
+    }
+  }
+}
+
 // Type-1 spreading, point-driven ("tap-parallel"): one workgroup per
 // (super-bin subproblem, transform chunk) accumulates into an LDS tile
 // [TC][re|im][cell] of the 8^3 super-bin plus halo.  Each wavefront takes one
@@ -1425,7 +1659,7 @@ struct Plan {
   int64_t ntrans = 1;
   int w = 7;
   int tc = 2;  // transforms per interpolation workgroup
-  int spread_mode = 0;  // 0 = LDS-staged gather, 1 = LDS atomic tiles, 2/3 = LDS-free gather (FP32; 3 with L2 touch)
+  int spread_mode = 0;  // 0 = LDS-staged gather, 1 = LDS atomic tiles, 2/3 = LDS-free gather (FP32; 3 with L2 touch), 4 = FP32 MFMA gather
   int interp_mode = 0;  // 0 = thread per point, 1 = wavefront per point, 2 = eight lanes per point
   int max_tc_gather = 20;  // transforms per gather workgroup
   int tc_spread = 2;    // transforms per spreading workgroup (tile mode)
@@ -1573,7 +1807,7 @@ static int run_interp(Plan* p, void* c_out, int ntrans_run, hipStream_t st) {
 
 template <typename T, int TC>
 static int launch_spread(Plan* p, const void* c_in, int ntrans_run, hipStream_t st) {
-  const bool sc = sizeof(T) == 4 && (p->spread_mode == 2 || p->spread_mode == 3);
+  const bool sc = sizeof(T) == 4 && (p->spread_mode >= 2 && p->spread_mode <= 4);
   const int ldc = sc ? (int)ceil_div(ntrans_run, TC) * TC : ntrans_run;
   COV3D_CHECK_ARG(ldc <= p->ldc_cap, "sorted row stride %d exceeds %d", ldc, p->ldc_cap);
   {
@@ -1586,7 +1820,15 @@ static int launch_spread(Plan* p, const void* c_in, int ntrans_run, hipStream_t 
   const T kc = (T)(4.0 / ((double)p->w * p->w));
   ScopedTiming tm(kTSpread, st);
   if constexpr (sizeof(T) == 4) {
-    if (p->spread_mode == 3) {
+    if (p->spread_mode == 4 && p->w <= 8) {
+      hipLaunchKernelGGL((k_spread_mfma<TC>), grid, dim3(kGatherBlock), 0, st, (const Cplx<float>*)p->csort,
+                         (Cplx<float>*)p->fine, p->geom, p->bin_start, p->bin_count, (const float*)p->sv[0],
+                         (const float*)p->sv[1], (const float*)p->sv[2], ntrans_run, ldc, p->w, (float)kc,
+                         (float)(p->beta * 1.4426950408889634));
+      COV3D_LAUNCH_CHECK();
+      return 0;
+    }
+    if (p->spread_mode == 3 || p->spread_mode == 4) {
       hipLaunchKernelGGL((k_spread_sc<TC, true>), grid, dim3(kGatherBlock), 0, st, (const Cplx<float>*)p->csort,
                          (Cplx<float>*)p->fine, p->geom, p->bin_start, p->bin_count, (const float*)p->sv[0],
                          (const float*)p->sv[1], (const float*)p->sv[2], ntrans_run, ldc, p->w, (float)kc,
@@ -1872,8 +2114,15 @@ static int setpts_impl(Plan* p, int64_t M, const T* x, const T* y, const T* z, h
     COV3D_HIP(hipMalloc(&p->spos, Ma * sizeof(int)));
     COV3D_HIP(hipMalloc(&p->iord, Ma * sizeof(int)));
     // interpolation writes whole chunks of up to 4 transforms (ntrans rounded up to 4); the
-    // spreading input uses row stride gather_ldc(n) for any n <= ntrans
-    COV3D_HIP(hipMalloc(&p->csort, Ma * p->ldc_cap * (p->dtype == 1 ? 16 : 8)));
+    // spreading input uses row stride gather_ldc(n) for any n <= ntrans.  The matrix-core
+    // spreading reads up to 7 values past a row's chunk: a zeroed tail keeps the last row's
+    // reads inside the buffer.
+    {
+      const size_t csz = p->dtype == 1 ? 16 : 8;
+      const size_t body = (size_t)Ma * p->ldc_cap * csz;
+      COV3D_HIP(hipMalloc(&p->csort, body + kCsortSlack * csz));
+      COV3D_HIP(hipMemset((char*)p->csort + body, 0, kCsortSlack * csz));
+    }
     for (int k = 0; k < 3; ++k) COV3D_HIP(hipMalloc(&p->sv[k], Ma * sizeof(T)));
     const int64_t ca = cap > 8 ? cap : 8;
     COV3D_HIP(hipMalloc(&p->sub_sb, ca * sizeof(int)));
@@ -1974,10 +2223,10 @@ static int make_plan(int type, const int64_t* n, int iflag, int64_t ntrans, doub
   const int tc_env = env_int("COV3D_NUFFT_TC", 2);
   p->tc = dtype == 1 ? 1 : (tc_env == 1 || tc_env == 4 ? tc_env : 2);
   {
-    // 3 = LDS-free gather with L2 touch (FP32 default), 2 = the same without the touch,
-    // 0 = LDS-staged gather (FP64 default), 1 = LDS atomic tiles
-    const int sm = env_int("COV3D_SPREAD_MODE", p->dtype == 0 ? 3 : 0);
-    p->spread_mode = (sm == 1 || ((sm == 2 || sm == 3) && p->dtype == 0)) ? sm : 0;
+    // 4 = FP32 matrix-core gather (FP32 default; W > 8 uses 3), 3 = LDS-free gather with L2 touch,
+    // 2 = the same without the touch, 0 = LDS-staged gather (FP64 default), 1 = LDS atomic tiles
+    const int sm = env_int("COV3D_SPREAD_MODE", p->dtype == 0 ? 4 : 0);
+    p->spread_mode = (sm == 1 || ((sm >= 2 && sm <= 4) && p->dtype == 0)) ? sm : 0;
   }
   {
     // 0 = thread per point (default), 1 = wavefront per point, 2 = eight lanes per point (FP32, W <= 8)

@@ -153,7 +153,8 @@ def test_vol_forward_fourier_slice_closed_form(dev):
     assert rel_l2(lhs.cpu().numpy(), rhs.cpu().numpy()) < 1e-5
 
 
-@pytest.mark.parametrize("spread_mode,interp_mode,ntrans", [("0", "2", 3), ("2", "1", 5), ("0", "0", 20), ("2", "2", 21), ("3", "0", 7)])
+@pytest.mark.parametrize("spread_mode,interp_mode,ntrans", [("0", "2", 3), ("2", "1", 5), ("0", "0", 20), ("2", "2", 21), ("3", "0", 7),
+                                                           ("4", "0", 20), ("4", "0", 3), ("4", "2", 21)])
 def test_kernel_variants_match_nudft(dev, monkeypatch, spread_mode, interp_mode, ntrans):
     """Every selectable spreading / interpolation kernel (read at plan creation from
     COV3D_SPREAD_MODE / COV3D_INTERP_MODE) against the NUDFT, including transform
