@@ -17,10 +17,20 @@
 namespace cov3d {
 
 constexpr int kMlBlock = 256;
-// The split-BF16 forward runs 16 waves per image: each Gram iteration waits on a dependent load
-// chain (pixel map -> CTF, values), so more waves per workgroup hide it.
-constexpr int kMlBlockMfma = 1024;
 constexpr int kRmax = 32;  // rows of the Gram tile (r + 1 <= 32)
+// Split-BF16 forward: 8 waves per image.  Each wave takes 32-pixel chunks: its lanes load the
+// chunk's rows coalesced (lane = pixel, two rows per instruction), scale and split them, and
+// store the BF16 pairs in a per-wave LDS tile [row][pixel], from which the MFMA operands are
+// read as 16-byte rows.
+constexpr int kMlBlockMfma = 512;
+constexpr int kMlChunk = 32;
+constexpr int kMlTileLd = kMlChunk + 4;  // u32 per tile row (the pad spreads rows over banks)
+// LDS of the reduction / solve phase: s_part[4][32 x 32] f32, then s_G, s_A, s_minv ((32 x 33) f64
+// each) and s_z (32 f64)
+constexpr int kMlPartBytes = 4 * kRmax * kRmax * 4;
+constexpr int kMlPhase2Bytes = kMlPartBytes + 3 * kRmax * (kRmax + 1) * 8 + kRmax * 8;
+constexpr int kMlTileBytes = (kMlBlockMfma / 64) * 2 * kRmax * kMlTileLd * 4;
+constexpr int kMlRowsBytes = kRmax * 64 * 8;  // FP32 path: 64 pixels of every row
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
@@ -124,11 +134,15 @@ __global__ void __launch_bounds__(PREC == 1 ? kMlBlockMfma : kMlBlock)
                  int r, int L, float noise_var, int mean_const, float* __restrict__ cost,
                  float* __restrict__ z_out, float* __restrict__ m_out,
                  float* __restrict__ minv_out, float* __restrict__ b_out) {
-  __shared__ float s_part[4][kRmax * kRmax];
-  __shared__ double s_G[kRmax * (kRmax + 1)];
-  __shared__ double s_A[kRmax * (kRmax + 1)];
-  __shared__ double s_z[kRmax];
-  __shared__ double s_minv[kRmax * (kRmax + 1)];
+  // The split-BF16 tiles and the reduction arrays are never live together: one buffer holds both.
+  constexpr int kSmem = PREC == 1 ? (kMlTileBytes > kMlPhase2Bytes ? kMlTileBytes : kMlPhase2Bytes)
+                                  : kMlPhase2Bytes + kMlRowsBytes;
+  __shared__ __align__(16) unsigned char s_raw[kSmem];
+  float(*s_part)[kRmax * kRmax] = reinterpret_cast<float(*)[kRmax * kRmax]>(s_raw);
+  double* s_G = reinterpret_cast<double*>(s_raw + kMlPartBytes);
+  double* s_A = s_G + kRmax * (kRmax + 1);
+  double* s_minv = s_A + kRmax * (kRmax + 1);
+  double* s_z = s_minv + kRmax * (kRmax + 1);
   const int64_t i = blockIdx.x;
   const int64_t d = (int64_t)L * L;
   const float inv_l = 1.f / (float)L;
@@ -140,37 +154,72 @@ __global__ void __launch_bounds__(PREC == 1 ? kMlBlockMfma : kMlBlock)
   if constexpr (PREC == 1) {
     // ---- split-BF16 MFMA Gram: wave w of nw covers 32-pixel chunks w, w + nw, ...
     const int nw = (int)(blockDim.x >> 6);
-    const int m = lane & 31;
-    const int h = lane >> 5;
+    unsigned* t_hi = reinterpret_cast<unsigned*>(s_raw) + wave * 2 * kRmax * kMlTileLd;
+    unsigned* t_lo = t_hi + kRmax * kMlTileLd;
+    // rows above r stay zero
+    for (int e = (r + 1) * kMlTileLd + lane; e < kRmax * kMlTileLd; e += 64) t_hi[e] = t_lo[e] = 0u;
+    const int p = lane & 31, rp = lane >> 5;  // load role: pixel p, rows rp, rp + 2, ...
+    const int m = lane & 31, h = lane >> 5;   // MFMA role: row m, pixel quarter h
+    const int nrow2 = (r + 2) >> 1;           // row pairs covering rows 0..r
     f32x16 acc;
 #pragma unroll
     for (int v = 0; v < 16; ++v) acc[v] = 0.f;
-    const int64_t nchunk = (np + 31) / 32;
+    const int64_t nchunk = (np + kMlChunk - 1) / kMlChunk;
     for (int64_t ch = wave; ch < nchunk; ch += nw) {
-      const int64_t k0 = ch * 32 + 16 * h;
-      float2 vals[16];
+      const int64_t k = ch * kMlChunk + p;
+      float sc = 0.f, sy = 0.f;
+      int64_t px = 0;
+      if (k < np) {
+        float mult;
+        px = entry_pixel(pix_map, k, L, mult);
+        const float sm = mult == 1.f ? 1.f : 1.41421356237309515f;
+        sc = pixel_scale(ctf, i, px, L, inv_l2) * sm;
+        sy = inv_l * sm;
+      }
+      float2 v[16];
 #pragma unroll
-      for (int q = 0; q < 16; ++q)
-        vals[q] = gram_row_value(c, yhat, ctf, pix_map, i, m, r, B, np, k0 + q, L, inv_l, inv_l2);
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        u16x8 hi, lo;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          float2 v = vals[4 * q + e];
-          unsigned short hx = f32_to_bf16_bits(v.x), hy = f32_to_bf16_bits(v.y);
-          hi[2 * e] = hx;
-          hi[2 * e + 1] = hy;
-          lo[2 * e] = f32_to_bf16_bits(v.x - bf16_bits_to_f32(hx));
-          lo[2 * e + 1] = f32_to_bf16_bits(v.y - bf16_bits_to_f32(hy));
+      for (int j = 0; j < 16; ++j) {
+        if (j < nrow2) {
+          const int mm = rp + 2 * j;
+          v[j] = make_float2(0.f, 0.f);
+          if (k < np && mm < r) v[j] = c[((int64_t)mm * B + i) * np + k];
+          else if (k < np && mm == r) v[j] = yhat[i * d + px];
         }
-        bf16x8 ah = __builtin_bit_cast(bf16x8, hi);
-        bf16x8 al = __builtin_bit_cast(bf16x8, lo);
+      }
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        if (j < nrow2) {
+          const int mm = rp + 2 * j;
+          if (mm <= r) {
+            const float s = mm == r ? sy : sc;
+            const float x = v[j].x * s, y = v[j].y * s;
+            const unsigned short hx = f32_to_bf16_bits(x), hy = f32_to_bf16_bits(y);
+            const unsigned short lx = f32_to_bf16_bits(x - bf16_bits_to_f32(hx));
+            const unsigned short ly = f32_to_bf16_bits(y - bf16_bits_to_f32(hy));
+            t_hi[mm * kMlTileLd + p] = (unsigned)hx | ((unsigned)hy << 16);
+            t_lo[mm * kMlTileLd + p] = (unsigned)lx | ((unsigned)ly << 16);
+          }
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+      for (int q = 0; q < kMlChunk / 8; ++q) {
+        const uint4 uh = *reinterpret_cast<const uint4*>(&t_hi[m * kMlTileLd + 8 * q + 4 * h]);
+        const uint4 ul = *reinterpret_cast<const uint4*>(&t_lo[m * kMlTileLd + 8 * q + 4 * h]);
+        const bf16x8 ah = __builtin_bit_cast(bf16x8, uh);
+        const bf16x8 al = __builtin_bit_cast(bf16x8, ul);
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, ah, acc, 0, 0, 0);
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, al, acc, 0, 0, 0);
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, ah, acc, 0, 0, 0);
       }
+      // the next chunk's stores must not overtake this chunk's operand reads
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
+    __syncthreads();  // the tiles are dead: s_part aliases them
     // D layout of the 32x32 tile: lane l, register v -> row 8*(v/4) + 4*(l/32) + v%4, col l%32.
     // Waves are folded into the 4 partial tiles in rounds: round j adds waves 4j .. 4j + 3.
     for (int j = 0; 4 * j < nw; ++j) {
@@ -187,7 +236,7 @@ __global__ void __launch_bounds__(PREC == 1 ? kMlBlockMfma : kMlBlock)
   } else {
     // ---- FP32 Gram (256 threads): stage 64 pixels of all rows in LDS; thread k owns the
     // upper-triangle pairs k, k + 256, k + 512 ((r+1)(r+2)/2 <= 528).
-    __shared__ float2 s_rows[kRmax][64];
+    float2(*s_rows)[64] = reinterpret_cast<float2(*)[64]>(s_raw + kMlPhase2Bytes);
     const int npair = (r + 1) * (r + 2) / 2;
     int pt[3], ps[3];
     float accp[3] = {0.f, 0.f, 0.f};
