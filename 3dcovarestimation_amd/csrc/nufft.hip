@@ -227,13 +227,13 @@ __device__ __forceinline__ int point_bin(const T* x, const T* y, const T* z, int
 
 template <typename T>
 __global__ void k_bin_count(const T* x, const T* y, const T* z, int64_t M, BinGeom g,
-                            int* bin_count, int* pt_bin, int* pt_pos) {
+                            int* bin_count, int* pt_bin) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= M) return;
   double v[3];
   int bin = point_bin(x, y, z, j, g, v);
   pt_bin[j] = bin;
-  pt_pos[j] = atomicAdd(&bin_count[bin], 1);
+  atomicAdd(&bin_count[bin], 1);
 }
 
 // Single-workgroup scans: bin_start (exclusive over bins) and, per super-bin,
@@ -296,21 +296,24 @@ __global__ void k_sub_fill(const int* bin_count, const int* bin_start, const int
   }
 }
 
+// Sorted position p holds point sidx[p] (stable radix sort by bin): its inverse and its
+// super-bin-relative coordinates.
 template <typename T>
-__global__ void k_bin_scatter(const T* x, const T* y, const T* z, int64_t M, BinGeom g,
-                              const int* pt_bin, const int* pt_pos, const int* bin_start,
-                              int* sidx, int* spos, T* sv0, T* sv1, T* sv2) {
-  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= M) return;
+__global__ void k_bin_place(const T* x, const T* y, const T* z, int64_t M, BinGeom g,
+                            const int* __restrict__ sidx, int* spos, T* sv0, T* sv1, T* sv2) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= M) return;
+  const int j = sidx[p];
   double v[3];
   (void)point_bin(x, y, z, j, g, v);
-  int dst = bin_start[pt_bin[j]] + pt_pos[j];
-  sidx[dst] = (int)j;
-  spos[j] = dst;
-  sv0[dst] = (T)v[0];
-  sv1[dst] = (T)v[1];
-  sv2[dst] = (T)v[2];
+  spos[j] = (int)p;
+  sv0[p] = (T)v[0];
+  sv1[p] = (T)v[1];
+  sv2[p] = (T)v[2];
 }
+
+hipError_t sort_points_by_bin(void* tmp, size_t& tmp_bytes, const unsigned* keys_in, unsigned* keys_out,
+                              int* idx_out, int n, int nbits, hipStream_t st);
 
 // Modes -> zero-padded, deconvolved fine grid (one thread per fine-grid cell).
 template <typename T, bool REAL_IN>
@@ -1690,7 +1693,9 @@ struct Plan {
   int* sub_n = nullptr;
   int* nsub = nullptr;
   int* pt_bin = nullptr;
-  int* pt_pos = nullptr;
+  int* pt_pos = nullptr;      // sorted bin keys (radix sort output)
+  void* sort_tmp = nullptr;   // radix sort work area
+  size_t sort_tmp_bytes = 0;
   int* sidx = nullptr;
   int* spos = nullptr;
   int* iord = nullptr;        // bank-aware interpolation order (FP32, two transforms per cell, W <= 8)
@@ -1717,6 +1722,9 @@ static std::once_flag g_rocfft_once;
 static int free_points(Plan* p) {
   hipFree(p->pt_bin);
   hipFree(p->pt_pos);
+  hipFree(p->sort_tmp);
+  p->sort_tmp = nullptr;
+  p->sort_tmp_bytes = 0;
   hipFree(p->sidx);
   hipFree(p->spos);
   hipFree(p->iord);
@@ -2152,8 +2160,23 @@ static int setpts_impl(Plan* p, int64_t M, const T* x, const T* y, const T* z, h
   if (M > 0) {
     dim3 gm((unsigned)ceil_div(M, kBlock));
     hipLaunchKernelGGL((k_bin_count<T>), gm, dim3(kBlock), 0, st, x, y, z, M, p->geom, p->bin_count,
-                       p->pt_bin, p->pt_pos);
+                       p->pt_bin);
     COV3D_LAUNCH_CHECK();
+    int nbits = 1;
+    while ((int64_t)1 << nbits < p->nbins) ++nbits;
+    size_t need = 0;
+    COV3D_HIP(sort_points_by_bin(nullptr, need, (const unsigned*)p->pt_bin, (unsigned*)p->pt_pos, p->sidx,
+                                 (int)M, nbits, st));
+    if (need > p->sort_tmp_bytes) {
+      COV3D_HIP(hipStreamSynchronize(st));
+      hipFree(p->sort_tmp);
+      p->sort_tmp = nullptr;
+      p->sort_tmp_bytes = 0;
+      COV3D_HIP(hipMalloc(&p->sort_tmp, need));
+      p->sort_tmp_bytes = need;
+    }
+    COV3D_HIP(sort_points_by_bin(p->sort_tmp, need, (const unsigned*)p->pt_bin, (unsigned*)p->pt_pos, p->sidx,
+                                 (int)M, nbits, st));
   }
   hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, st, p->bin_count, p->nsuper, p->bin_start,
                      p->sub_start, p->nsub);
@@ -2164,9 +2187,8 @@ static int setpts_impl(Plan* p, int64_t M, const T* x, const T* y, const T* z, h
                        p->sub_n);
     COV3D_LAUNCH_CHECK();
     dim3 gm((unsigned)ceil_div(M, kBlock));
-    hipLaunchKernelGGL((k_bin_scatter<T>), gm, dim3(kBlock), 0, st, x, y, z, M, p->geom, p->pt_bin,
-                       p->pt_pos, p->bin_start, p->sidx, p->spos, (T*)p->sv[0], (T*)p->sv[1],
-                       (T*)p->sv[2]);
+    hipLaunchKernelGGL((k_bin_place<T>), gm, dim3(kBlock), 0, st, x, y, z, M, p->geom, p->sidx, p->spos,
+                       (T*)p->sv[0], (T*)p->sv[1], (T*)p->sv[2]);
     COV3D_LAUNCH_CHECK();
   }
   p->iord_valid = false;

@@ -182,6 +182,27 @@ def test_kernel_variants_match_nudft(dev, monkeypatch, spread_mode, interp_mode,
     assert np.array_equal(out1, plan.execute_adjoint(c).cpu().numpy())
 
 
+def test_setpts_is_deterministic(dev):
+    """The bin sort is stable (points of a bin stay in index order), so two plans given the same
+    points lay them out identically and produce bit-identical adjoints: every per-bin sum runs
+    over the same points in the same order."""
+    npl = pkg("nufft_plan")
+    L, ntrans = 16, 20
+    pts = _points(L, 6, 2, 77, dev, spread=1.2)
+    g = torch.Generator().manual_seed(5)
+    M = pts.shape[0] * pts.shape[2]
+    c = torch.complex(torch.randn((ntrans, M), generator=g), torch.randn((ntrans, M), generator=g)).to(dev)
+    vol = torch.randn((ntrans, L, L, L), generator=g).to(dev)
+    outs = []
+    for _ in range(2):
+        plan = npl.NufftPlan((L,) * 3, batch_size=ntrans, eps=1e-6, device=dev)
+        plan.setpts(pts)
+        outs.append((plan.execute_adjoint(c).cpu(), plan.execute_forward(vol).cpu()))
+        del plan
+    assert torch.equal(outs[0][0], outs[1][0])
+    assert torch.equal(outs[0][1], outs[1][1])
+
+
 @pytest.mark.parametrize("dtype_code,tol", [(0, TOL_F32), (1, 1e-9)])
 @pytest.mark.parametrize("iflag", [-1, 1])
 def test_c_abi_both_signs_real_and_complex(dev, dtype_code, tol, iflag):
