@@ -1,0 +1,341 @@
+// Pruned 3-D real FFTs of the NUFFT fine grid (FP32, cubic grids of NF = 64,
+// 128 or 256 cells per axis, N <= NF/2 + 1 modes per axis).
+//
+// Type 2 turns the real modes into the Hermitian half of the fine-grid
+// spectrum.  The zero-padded grid holds N of NF values on each axis, so the
+// transform runs one axis at a time on the non-zero lines only, with the
+// deconvolution and zero padding done while a line is loaded:
+//   A: z lines of the N^2 mode rows -> (t, i0, i1, k2 <= NF/2)        sign -1
+//   B: y lines (N of NF non-zero)   -> (t, i0, k1, k2)               sign -1
+//   C: x lines (N of NF non-zero)   -> fine (t, k0, k1, k2)          sign -1
+// which is the forward real-to-complex transform of the padded grid (the
+// output of rocFFT's R2C of the same grid, to rounding).
+//
+// Type 1 (real output) folds the spread grid G into its Hermitian part
+// H[k] = (G[k] + conj(G[-k])) / 2 while loading the first pass, and keeps only
+// the N cropped outputs of every line:
+//   X: x lines of H (k2 <= NF/2)    -> (t, i0, k1, k2)               sign +1
+//   Y: y lines                      -> (t, i0, i1, k2)               sign +1
+//   Z: Hermitian z lines -> real, deconvolved modes f (t, i0, i1, i2)
+// which equals rocFFT's complex-to-real transform of H followed by the crop.
+//
+// Every line is transformed in LDS by a radix-4 Stockham FFT (one radix-2
+// stage when log2 NF is odd), 16 lines per workgroup.  The strided passes
+// take 16 consecutive k2 columns per workgroup, so each row of a tile is one
+// 128-byte segment in HBM.
+#include "common.h"
+
+namespace cov3d {
+
+constexpr int kFftThreads = 256;
+constexpr int kFftLines = 16;
+
+// In-place unnormalised FFT (sign -1 or +1) of kFftLines lines of length NF in LDS; line l
+// starts at buf[l * LS].  tw[k] = exp(-2 pi i k / NF).  Stockham radix 4 (and 2): stage with
+// stride Ns reads x[j + r NF/R], applies exp(sign 2 pi i r (j mod Ns) / (Ns R)), takes the R-point
+// DFT and writes y[(j / Ns) Ns R + j mod Ns + r Ns].
+template <int NF, int LS, int R>
+__device__ __forceinline__ void fft_stage(float2* buf, const float2* tw, float sign, int Ns) {
+  constexpr int Q = NF / R;
+  constexpr int PER = kFftLines * Q / kFftThreads;
+  static_assert(PER >= 1 && kFftLines * Q % kFftThreads == 0, "stage does not tile the block");
+  const int tid = threadIdx.x;
+  float2 v[PER][R];
+#pragma unroll
+  for (int b = 0; b < PER; ++b) {
+    const int g = tid + b * kFftThreads;
+    const int l = g / Q, j = g % Q;
+    const float2* x = buf + l * LS;
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[b][r] = x[j + r * Q];
+    const int jm = j & (Ns - 1);
+    const int step = NF / (Ns * R);
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+      const float2 w = tw[(r * jm * step) & (NF - 1)];
+      const float wy = sign < 0.f ? w.y : -w.y;
+      const float2 a = v[b][r];
+      v[b][r] = make_float2(a.x * w.x - a.y * wy, a.x * wy + a.y * w.x);
+    }
+    if constexpr (R == 4) {
+      // omega = exp(sign 2 pi i / 4) = sign * i; omega * (a + ib) = sign * (-b + ia)
+      const float2 s02 = make_float2(v[b][0].x + v[b][2].x, v[b][0].y + v[b][2].y);
+      const float2 d02 = make_float2(v[b][0].x - v[b][2].x, v[b][0].y - v[b][2].y);
+      const float2 s13 = make_float2(v[b][1].x + v[b][3].x, v[b][1].y + v[b][3].y);
+      const float2 d13 = make_float2(v[b][1].x - v[b][3].x, v[b][1].y - v[b][3].y);
+      const float2 od13 = make_float2(-sign * d13.y, sign * d13.x);
+      v[b][0] = make_float2(s02.x + s13.x, s02.y + s13.y);
+      v[b][1] = make_float2(d02.x + od13.x, d02.y + od13.y);
+      v[b][2] = make_float2(s02.x - s13.x, s02.y - s13.y);
+      v[b][3] = make_float2(d02.x - od13.x, d02.y - od13.y);
+    } else {
+      const float2 a = v[b][0], c = v[b][1];
+      v[b][0] = make_float2(a.x + c.x, a.y + c.y);
+      v[b][1] = make_float2(a.x - c.x, a.y - c.y);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < PER; ++b) {
+    const int g = tid + b * kFftThreads;
+    const int l = g / Q, j = g % Q;
+    const int jm = j & (Ns - 1);
+    float2* y = buf + l * LS + (j / Ns) * Ns * R + jm;
+#pragma unroll
+    for (int r = 0; r < R; ++r) y[r * Ns] = v[b][r];
+  }
+  __syncthreads();
+}
+
+template <int NF, int LS>
+__device__ void lds_fft(float2* buf, const float2* tw, float sign) {
+  int Ns = 1;
+  for (; Ns * 4 <= NF; Ns *= 4) fft_stage<NF, LS, 4>(buf, tw, sign, Ns);
+  if (Ns < NF) fft_stage<NF, LS, 2>(buf, tw, sign, Ns);
+}
+
+// Fine-grid index of mode i (centred modes, negative frequencies wrapped to the end).
+__device__ __forceinline__ int wrap_mode(int i, int N, int NF) {
+  const int n = i - N / 2;
+  return n >= 0 ? n : n + NF;
+}
+
+template <int NF>
+__device__ __forceinline__ void load_twiddles(float2* s_tw, const float2* __restrict__ tw) {
+  for (int k = threadIdx.x; k < NF; k += kFftThreads) s_tw[k] = tw[k];
+}
+
+// ---- type 2, pass A: real mode rows (t, i0, i1, :) -> z spectrum (t, i0, i1, k2 <= NF/2).
+// Lines g = (t * N + i0) * N + i1; 16 consecutive lines per workgroup.
+template <int NF>
+__global__ void __launch_bounds__(kFftThreads)
+    k_pfft2_z(const float* __restrict__ f, float2* __restrict__ A, const float2* __restrict__ tw,
+              const float* __restrict__ ip0, const float* __restrict__ ip1, const float* __restrict__ ip2,
+              int64_t nlines, int64_t nlive, int N) {
+  constexpr int LS = NF + 1;
+  constexpr int H = NF / 2 + 1;
+  __shared__ float2 s_buf[kFftLines * LS];
+  __shared__ float2 s_tw[NF];
+  load_twiddles<NF>(s_tw, tw);
+  const int64_t g0 = (int64_t)blockIdx.x * kFftLines;
+  for (int e = threadIdx.x; e < kFftLines * NF; e += kFftThreads) {
+    const int l = e / NF, m = e % NF;
+    s_buf[l * LS + m] = make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < kFftLines * N; e += kFftThreads) {
+    const int l = e / N, i2 = e % N;
+    const int64_t g = g0 + l;
+    if (g < nlines) {
+      const int i1 = (int)(g % N), i0 = (int)((g / N) % N);
+      const float v = g < nlive ? f[g * N + i2] * (ip0[i0] * ip1[i1] * ip2[i2]) : 0.f;
+      s_buf[l * LS + wrap_mode(i2, N, NF)] = make_float2(v, 0.f);
+    }
+  }
+  __syncthreads();
+  lds_fft<NF, LS>(s_buf, s_tw, -1.f);
+  for (int e = threadIdx.x; e < kFftLines * H; e += kFftThreads) {
+    const int l = e / H, k = e % H;
+    const int64_t g = g0 + l;
+    if (g < nlines) A[g * H + k] = s_buf[l * LS + k];
+  }
+}
+
+// ---- strided passes: a workgroup takes 16 consecutive k2 columns [c0, c0 + 16) of one
+// (outer, mid) position.  Input lines run over `nin` rows with row stride `sin` (complex
+// elements); row i of the line goes to position pos(i).  Output rows o < nout come from line
+// position opos(o) and go to row stride `sout`.
+//   type 2, pass B: outer t*N+i0, input rows i1 < N at wrap(i1), output all NF rows
+//   type 2, pass C: outer t, mid k1; input rows i0 < N at wrap(i0), output all NF rows
+//   type 1, pass X: input all NF rows (folded), output rows i0 < N from wrap(i0)
+//   type 1, pass Y: input all NF rows, output rows i1 < N from wrap(i1)
+// MODE 0: pass B, 1: pass C, 2: pass X (with the Hermitian fold), 3: pass Y.
+template <int NF, int MODE>
+__global__ void __launch_bounds__(kFftThreads)
+    k_pfft_cols(const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw,
+                int ntrans, int N, int conj_out) {
+  constexpr int LS = NF + 1;
+  constexpr int H = NF / 2 + 1;
+  constexpr int NCT = (H + kFftLines - 1) / kFftLines;  // column tiles
+  __shared__ float2 s_buf[kFftLines * LS];
+  __shared__ float2 s_tw[NF];
+  load_twiddles<NF>(s_tw, tw);
+  const int ct = blockIdx.x % NCT;
+  const int64_t pos = blockIdx.x / NCT;  // (outer, mid) position
+  const int c0 = ct * kFftLines;
+  const int ncol = min(kFftLines, H - c0);
+  const int col = threadIdx.x % kFftLines, rsub = threadIdx.x / kFftLines;  // 16 columns x 16 rows
+  constexpr float sign = MODE <= 1 ? -1.f : 1.f;
+  const bool in_pruned = MODE <= 1;  // input has N rows at wrapped positions, the rest zero
+  // element (row, col) addresses; rows are complex elements apart by the row stride
+  int64_t in_base, in_stride, out_base, out_stride;
+  int nin = in_pruned ? N : NF;
+  int nout = in_pruned ? NF : N;
+  if constexpr (MODE == 0) {  // pos = t * N + i0; in A (t, i0, i1, k2), out B (t, i0, k1, k2)
+    in_base = pos * N * H;
+    in_stride = H;
+    out_base = pos * NF * H;
+    out_stride = H;
+  } else if constexpr (MODE == 1) {  // pos = t * NF + k1; in B (t, i0, k1, k2), out fine (t, k0, k1, k2)
+    const int64_t t = pos / NF, k1 = pos % NF;
+    in_base = (t * N * NF + k1) * H;
+    in_stride = (int64_t)NF * H;
+    out_base = (t * NF * NF + k1) * H;
+    out_stride = (int64_t)NF * H;
+  } else if constexpr (MODE == 2) {  // pos = t * NF + k1; in G (t, k0, k1, k2 < NF), out P (t, i0, k1, k2)
+    const int64_t t = pos / NF, k1 = pos % NF;
+    in_base = (t * NF * NF + k1) * NF;
+    in_stride = (int64_t)NF * NF;
+    out_base = (t * N * NF + k1) * H;
+    out_stride = (int64_t)NF * H;
+  } else {  // pos = t * N + i0; in P (t, i0, k1, k2), out Q (t, i0, i1, k2)
+    in_base = pos * NF * H;
+    in_stride = H;
+    out_base = pos * N * H;
+    out_stride = H;
+  }
+  if (in_pruned) {
+    for (int e = threadIdx.x; e < kFftLines * NF; e += kFftThreads) {
+      const int l = e / NF, m = e % NF;
+      s_buf[l * LS + m] = make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+  }
+  if (col < ncol) {
+    for (int i = rsub; i < nin; i += kFftThreads / kFftLines) {
+      const int m = in_pruned ? wrap_mode(i, N, NF) : i;
+      float2 v;
+      if constexpr (MODE == 2) {
+        // H[k] = (G[k] + conj(G[-k])) / 2 at k = (i, k1, c0 + col), conjugated when conj_out
+        const int64_t t = pos / NF, k1 = pos % NF;
+        const int k2 = c0 + col;
+        const int n0 = i ? NF - i : 0, n1 = k1 ? NF - (int)k1 : 0, n2 = k2 ? NF - k2 : 0;
+        const float2 a = in[in_base + i * in_stride + k2];
+        const float2 b = in[(t * NF * NF + ((int64_t)n0 * NF + n1)) * NF + n2];
+        const float im = 0.5f * (a.y - b.y);
+        v = make_float2(0.5f * (a.x + b.x), conj_out ? -im : im);
+      } else {
+        v = in[in_base + i * in_stride + c0 + col];
+      }
+      s_buf[col * LS + m] = v;
+    }
+  }
+  __syncthreads();
+  lds_fft<NF, LS>(s_buf, s_tw, sign);
+  if (col < ncol) {
+    for (int o = rsub; o < nout; o += kFftThreads / kFftLines) {
+      const int m = in_pruned ? o : wrap_mode(o, N, NF);
+      out[out_base + o * out_stride + c0 + col] = s_buf[col * LS + m];
+    }
+  }
+}
+
+// ---- type 1, pass Z: Hermitian z lines Q (t, i0, i1, k2 <= NF/2) -> real, cropped, deconvolved
+// modes f (t, i0, i1, i2).  Lines g = (t * N + i0) * N + i1; 16 per workgroup.
+template <int NF>
+__global__ void __launch_bounds__(kFftThreads)
+    k_pfft1_z(const float2* __restrict__ Q, float* __restrict__ f, const float2* __restrict__ tw,
+              const float* __restrict__ ip0, const float* __restrict__ ip1, const float* __restrict__ ip2,
+              int64_t nlines, int N) {
+  constexpr int LS = NF + 1;
+  constexpr int H = NF / 2 + 1;
+  __shared__ float2 s_buf[kFftLines * LS];
+  __shared__ float2 s_tw[NF];
+  load_twiddles<NF>(s_tw, tw);
+  const int64_t g0 = (int64_t)blockIdx.x * kFftLines;
+  for (int e = threadIdx.x; e < kFftLines * H; e += kFftThreads) {
+    const int l = e / H, k = e % H;
+    const int64_t g = g0 + l;
+    const float2 v = g < nlines ? Q[g * H + k] : make_float2(0.f, 0.f);
+    s_buf[l * LS + k] = v;
+    if (k > 0 && k < NF - k) s_buf[l * LS + NF - k] = make_float2(v.x, -v.y);
+  }
+  __syncthreads();
+  lds_fft<NF, LS>(s_buf, s_tw, 1.f);
+  for (int e = threadIdx.x; e < kFftLines * N; e += kFftThreads) {
+    const int l = e / N, i2 = e % N;
+    const int64_t g = g0 + l;
+    if (g < nlines) {
+      const int i1 = (int)(g % N), i0 = (int)((g / N) % N);
+      f[g * N + i2] = s_buf[l * LS + wrap_mode(i2, N, NF)].x * (ip0[i0] * ip1[i1] * ip2[i2]);
+    }
+  }
+}
+
+template <int NF>
+static int pruned_type2_nf(const float* f, float2* fine, float2* work, const float2* tw, const float* ip0,
+                           const float* ip1, const float* ip2, int ntrans, int nrun, int N, hipStream_t st) {
+  constexpr int H = NF / 2 + 1;
+  constexpr int NCT = (H + kFftLines - 1) / kFftLines;
+  float2* A = work;                                   // (ntrans, N, N, H)
+  float2* B = work + (int64_t)ntrans * N * N * H;     // (ntrans, N, NF, H)
+  const int64_t nlines = (int64_t)ntrans * N * N;
+  hipLaunchKernelGGL((k_pfft2_z<NF>), dim3((unsigned)ceil_div(nlines, kFftLines)), dim3(kFftThreads), 0, st, f,
+                     A, tw, ip0, ip1, ip2, nlines, (int64_t)nrun * N * N, N);
+  COV3D_LAUNCH_CHECK();
+  hipLaunchKernelGGL((k_pfft_cols<NF, 0>), dim3((unsigned)((int64_t)ntrans * N * NCT)), dim3(kFftThreads), 0, st,
+                     A, B, tw, ntrans, N, 0);
+  COV3D_LAUNCH_CHECK();
+  hipLaunchKernelGGL((k_pfft_cols<NF, 1>), dim3((unsigned)((int64_t)ntrans * NF * NCT)), dim3(kFftThreads), 0, st,
+                     B, fine, tw, ntrans, N, 0);
+  COV3D_LAUNCH_CHECK();
+  return 0;
+}
+
+template <int NF>
+static int pruned_type1_nf(const float2* G, float* f, float2* work, const float2* tw, const float* ip0,
+                           const float* ip1, const float* ip2, int ntrans, int N, int conj_out, hipStream_t st) {
+  constexpr int H = NF / 2 + 1;
+  constexpr int NCT = (H + kFftLines - 1) / kFftLines;
+  float2* P = work;                                   // (ntrans, N, NF, H)
+  float2* Q = work + (int64_t)ntrans * N * NF * H;    // (ntrans, N, N, H)
+  hipLaunchKernelGGL((k_pfft_cols<NF, 2>), dim3((unsigned)((int64_t)ntrans * NF * NCT)), dim3(kFftThreads), 0, st,
+                     G, P, tw, ntrans, N, conj_out);
+  COV3D_LAUNCH_CHECK();
+  hipLaunchKernelGGL((k_pfft_cols<NF, 3>), dim3((unsigned)((int64_t)ntrans * N * NCT)), dim3(kFftThreads), 0, st,
+                     P, Q, tw, ntrans, N, 0);
+  COV3D_LAUNCH_CHECK();
+  const int64_t nlines = (int64_t)ntrans * N * N;
+  hipLaunchKernelGGL((k_pfft1_z<NF>), dim3((unsigned)ceil_div(nlines, kFftLines)), dim3(kFftThreads), 0, st, Q,
+                     f, tw, ip0, ip1, ip2, nlines, N);
+  COV3D_LAUNCH_CHECK();
+  return 0;
+}
+
+bool pruned_fft_supported(int64_t NF, int64_t N) {
+  return (NF == 64 || NF == 128 || NF == 256) && N >= 1 && 2 * N <= NF + 2;
+}
+
+// Work area (complex values) both pipelines need.
+int64_t pruned_fft_work(int64_t NF, int64_t N, int64_t ntrans) {
+  const int64_t H = NF / 2 + 1;
+  return ntrans * N * H * (N + NF);
+}
+
+int pruned_type2(const float* f, void* fine, void* work, const void* tw, const float* ip0, const float* ip1,
+                 const float* ip2, int ntrans, int nrun, int NF, int N, hipStream_t st) {
+  float2* fo = (float2*)fine;
+  float2* w = (float2*)work;
+  const float2* t = (const float2*)tw;
+  switch (NF) {
+    case 64: return pruned_type2_nf<64>(f, fo, w, t, ip0, ip1, ip2, ntrans, nrun, N, st);
+    case 128: return pruned_type2_nf<128>(f, fo, w, t, ip0, ip1, ip2, ntrans, nrun, N, st);
+    case 256: return pruned_type2_nf<256>(f, fo, w, t, ip0, ip1, ip2, ntrans, nrun, N, st);
+    default: set_error("pruned FFT: unsupported grid %d", NF); return COV3D_ERR_ARG;
+  }
+}
+
+int pruned_type1(const void* G, float* f, void* work, const void* tw, const float* ip0, const float* ip1,
+                 const float* ip2, int ntrans, int NF, int N, int conj_out, hipStream_t st) {
+  const float2* g = (const float2*)G;
+  float2* w = (float2*)work;
+  const float2* t = (const float2*)tw;
+  switch (NF) {
+    case 64: return pruned_type1_nf<64>(g, f, w, t, ip0, ip1, ip2, ntrans, N, conj_out, st);
+    case 128: return pruned_type1_nf<128>(g, f, w, t, ip0, ip1, ip2, ntrans, N, conj_out, st);
+    case 256: return pruned_type1_nf<256>(g, f, w, t, ip0, ip1, ip2, ntrans, N, conj_out, st);
+    default: set_error("pruned FFT: unsupported grid %d", NF); return COV3D_ERR_ARG;
+  }
+}
+
+}  // namespace cov3d

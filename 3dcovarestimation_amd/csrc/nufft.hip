@@ -315,6 +315,14 @@ __global__ void k_bin_place(const T* x, const T* y, const T* z, int64_t M, BinGe
 hipError_t sort_points_by_bin(void* tmp, size_t& tmp_bytes, const unsigned* keys_in, unsigned* keys_out,
                               int* idx_out, int n, int nbits, hipStream_t st);
 
+// fft_pruned.hip
+bool pruned_fft_supported(int64_t NF, int64_t N);
+int64_t pruned_fft_work(int64_t NF, int64_t N, int64_t ntrans);
+int pruned_type2(const float* f, void* fine, void* work, const void* tw, const float* ip0, const float* ip1,
+                 const float* ip2, int ntrans, int nrun, int NF, int N, hipStream_t st);
+int pruned_type1(const void* G, float* f, void* work, const void* tw, const float* ip0, const float* ip1,
+                 const float* ip2, int ntrans, int NF, int N, int conj_out, hipStream_t st);
+
 // Modes -> zero-padded, deconvolved fine grid (one thread per fine-grid cell).
 template <typename T, bool REAL_IN>
 __global__ void k_deconv_pad(const T* __restrict__ f, Cplx<T>* __restrict__ fine, int64_t ntrans,
@@ -1709,6 +1717,10 @@ struct Plan {
   int real_fft = 1;
   void* rbuf = nullptr;       // real grid (type 2) or Hermitian half grid (type 1), allocated on first use
   size_t rbuf_bytes = 0;
+  // pruned FP32 real transforms (fft_pruned.hip) on cubic power-of-two grids instead of
+  // pad + R2C and fold + C2R + crop (COV3D_FFT_MODE=0 keeps rocFFT); rbuf holds their passes
+  bool pruned = false;
+  void* tw = nullptr;         // exp(-2 pi i k / nf), k < nf (complex64)
   int stage_half = 0;         // interpolation reads a Hermitian half grid with this m2 extent (0: full grid)
   int stage_conj = 0;         // ... conjugated
   rocfft_execution_info info = nullptr;
@@ -2030,6 +2042,23 @@ static int exec_type2(Plan* p, const void* f, bool f_real, void* c, int64_t nrun
     // real volume: real fine grid -> R2C (Hermitian half) -> interpolation mirrors m2 > nf2/2
     int rc = ensure_real(p);
     if (rc) return rc;
+    if constexpr (sizeof(T) == 4) {
+      if (p->pruned) {
+        {
+          ScopedTiming tm(kTFft, st);
+          rc = pruned_type2((const float*)f, p->fine, p->rbuf, p->tw, (const float*)p->invp[0],
+                            (const float*)p->invp[1], (const float*)p->invp[2], (int)p->ntrans, (int)nrun,
+                            (int)p->nf[0], (int)p->N[0], st);
+          if (rc) return rc;
+        }
+        p->stage_half = (int)(p->nf[2] / 2 + 1);
+        p->stage_conj = p->iflag2 > 0 ? 1 : 0;  // the transform has the negative sign
+        rc = run_interp(p, c, (int)nrun, st);
+        p->stage_half = 0;
+        p->stage_conj = 0;
+        return rc;
+      }
+    }
     {
       ScopedTiming tm(kTDeconvPad, st);
       const int64_t nrows = p->ntrans * p->nf[0] * p->nf[1];
@@ -2086,6 +2115,14 @@ static int exec_type1(Plan* p, const void* c, void* f, bool f_real, int64_t nrun
     // when type 1 takes the negative sign) -> real fine grid -> deconvolved real modes
     int rc = ensure_real(p);
     if (rc) return rc;
+    if constexpr (sizeof(T) == 4) {
+      if (p->pruned) {
+        ScopedTiming tm(kTFft, st);
+        return pruned_type1(p->fine, (float*)f, p->rbuf, p->tw, (const float*)p->invp[0], (const float*)p->invp[1],
+                            (const float*)p->invp[2], (int)nrun, (int)p->nf[0], (int)p->N[0],
+                            p->iflag2 > 0 ? 1 : 0, st);
+      }
+    }
     const int64_t nh = p->nf[0] * p->nf[1] * (p->nf[2] / 2 + 1) * p->ntrans;
     {
       ScopedTiming tm(kTDeconvCrop, st);
@@ -2227,6 +2264,7 @@ static int destroy_plan(Plan* p) {
   hipFree(p->nsub);
   hipFree(p->fft_work);
   hipFree(p->rbuf);
+  hipFree(p->tw);
   if (p->fft_fwd) rocfft_plan_destroy(p->fft_fwd);
   if (p->fft_bwd) rocfft_plan_destroy(p->fft_bwd);
   if (p->fft_r2c) rocfft_plan_destroy(p->fft_r2c);
@@ -2332,6 +2370,19 @@ static int make_plan(int type, const int64_t* n, int iflag, int64_t ntrans, doub
     const size_t tsz = dtype == 1 ? 8 : 4;
     p->rbuf_bytes = (size_t)ntrans * std::max<size_t>((size_t)nfine * tsz,
                                                         (size_t)p->nf[0] * p->nf[1] * (p->nf[2] / 2 + 1) * 2 * tsz);
+  }
+  if (dtype == 0 && env_int("COV3D_FFT_MODE", 1) == 1 && p->nf[0] == p->nf[1] && p->nf[0] == p->nf[2] &&
+      p->N[0] == p->N[1] && p->N[0] == p->N[2] && pruned_fft_supported(p->nf[0], p->N[0]) &&
+      (size_t)pruned_fft_work(p->nf[0], p->N[0], ntrans) * 8 <= p->rbuf_bytes) {
+    const int64_t NF = p->nf[0];
+    std::vector<float> tw(2 * NF);
+    for (int64_t k = 0; k < NF; ++k) {
+      tw[2 * k] = (float)cos(2.0 * M_PI * (double)k / (double)NF);
+      tw[2 * k + 1] = (float)-sin(2.0 * M_PI * (double)k / (double)NF);
+    }
+    if (hipMalloc(&p->tw, NF * 8) != hipSuccess) return fail(COV3D_ERR_HIP);
+    if (hipMemcpy(p->tw, tw.data(), NF * 8, hipMemcpyHostToDevice) != hipSuccess) return fail(COV3D_ERR_HIP);
+    p->pruned = true;
   }
   p->work_size = std::max(w1, w2);
   if (p->work_size) {

@@ -6,6 +6,8 @@ slices (the production case) and from uniform random positions, including
 values outside [-pi, pi) to exercise the wrap.
 """
 
+import warnings
+
 import numpy as np
 import pytest
 import torch
@@ -180,6 +182,43 @@ def test_kernel_variants_match_nudft(dev, monkeypatch, spread_mode, interp_mode,
     assert rel_l2(out1, ref1) < TOL_F32
     # deterministic: the gather kernels write each cell once
     assert np.array_equal(out1, plan.execute_adjoint(c).cpu().numpy())
+
+
+@pytest.mark.parametrize("L,nrun", [(32, 3), (64, 3), (32, 2)])
+def test_pruned_fft_matches_rocfft_and_nudft(dev, monkeypatch, L, nrun):
+    """The pruned real FFT passes (cubic power-of-two fine grids, the FP32 default there) against
+    rocFFT's R2C / C2R path (COV3D_FFT_MODE=0) of the same plan, and against the NUDFT: type 2 of
+    real volumes and the real part of type 1.  nrun < batch_size leaves transforms that must
+    stay zero."""
+    o = oracle()
+    npl = pkg("nufft_plan")
+    r = 3
+    pts = _points(L, 2, 1, 500 + L, dev, spread=1.2)
+    g = torch.Generator().manual_seed(L + nrun)
+    vol = torch.randn((nrun, L, L, L), generator=g).to(dev)
+    M = pts.shape[0] * pts.shape[2]
+    c = torch.complex(torch.randn((nrun, M), generator=g), torch.randn((nrun, M), generator=g)).to(dev)
+    outs = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("COV3D_FFT_MODE", mode)
+        plan = npl.NufftPlan((L,) * 3, batch_size=r, eps=1e-6, device=dev)
+        plan.setpts(pts)
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            fwd = plan.execute_forward(vol).reshape(r, -1).cpu().numpy()
+            adj = plan.execute_adjoint(c, real_output=True).cpu().numpy()
+        outs[mode] = (fwd, adj, plan.points.cpu().numpy().astype(np.float64))
+        del plan
+    for k in (0, 1):
+        assert rel_l2(outs["1"][k][:nrun], outs["0"][k][:nrun]) < 2e-6
+        assert np.abs(outs["1"][k][nrun:]).max(initial=0.0) == 0.0
+    if L > 32:
+        return  # the direct sums at L = 64 take too long; the rocFFT comparison above covers it
+    P = outs["1"][2]
+    ref2 = o.nudft_type2(vol.cpu().numpy().astype(np.complex128), P)
+    assert rel_l2(outs["1"][0][:nrun], ref2) < TOL_F32
+    ref1 = o.nudft_type1(c.cpu().numpy().astype(np.complex128), P, (L, L, L)).real
+    assert rel_l2(outs["1"][1][:nrun], ref1) < TOL_F32
 
 
 def test_setpts_is_deterministic(dev):
