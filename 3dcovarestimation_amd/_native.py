@@ -38,6 +38,7 @@ _SIGNATURES = {
     "cov3d_ml_grad_images": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_i64, c_i64, c_i64, c_dbl, c_dbl,
                              c_vp],
     "cov3d_pts_rot": [c_vp, c_vp, c_i64, c_i64, c_vp],
+    "cov3d_points_wrap": [c_vp, c_vp, c_i64, c_i64, c_int, c_vp],
     "cov3d_ctf_eval": [c_vp, c_vp, c_i64, c_i64, c_vp],
     "cov3d_vdot_c64_f64": [c_vp, c_vp, c_i64, c_vp, c_vp],
     "cov3d_toeplitz_apply": [c_vp, c_vp, c_vp, c_i64, c_i64, c_vp],

@@ -70,6 +70,35 @@ class VolumeBase(torch.nn.Module):
         return super().load_state_dict(state_dict, *args, **kwargs)
 
 
+class _ScaleRows(torch.autograd.Function):
+    """v * exp(l) per row (covar.py:45-65's vectors * exp(log_sqrt_eigenvals)).
+
+    The gradient of l is r dot products over whole volumes; they are summed in
+    blocks of 4096 first, which spreads the reduction over the device (one
+    reduction per row runs on a handful of workgroups).
+    """
+
+    @staticmethod
+    def forward(ctx, v, l):
+        s = torch.exp(l)
+        ctx.save_for_backward(v, s)
+        return v * s.reshape((-1,) + (1,) * (v.ndim - 1))
+
+    @staticmethod
+    def backward(ctx, g):
+        v, s = ctx.saved_tensors
+        r = v.shape[0]
+        sb = s.reshape((-1,) + (1,) * (v.ndim - 1))
+        gv = g * sb if ctx.needs_input_grad[0] else None
+        gl = None
+        if ctx.needs_input_grad[1]:
+            prod = (g * v).reshape(r, -1)
+            n = prod.shape[1]
+            blk = 4096 if n % 4096 == 0 else 1
+            gl = prod.reshape(r, n // blk, blk).sum(-1).sum(-1) * s
+        return gv, gl
+
+
 class Covar(VolumeBase):
     """Rank-r covariance Sigma = V^T V with V = vectors * exp(log_sqrt_eigenvals) (covar.py:278-458)."""
 
@@ -109,7 +138,7 @@ class Covar(VolumeBase):
         self.log_sqrt_eigenvals.data.copy_(l)
 
     def get_vectors_spatial_domain(self) -> torch.Tensor:
-        return self.vectors * torch.exp(self.log_sqrt_eigenvals).reshape(-1, 1, 1, 1)
+        return _ScaleRows.apply(self.vectors, self.log_sqrt_eigenvals)
 
     def get_vectors_fourier_domain(self) -> torch.Tensor:
         v = self.get_vectors_spatial_domain()

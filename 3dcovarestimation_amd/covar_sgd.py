@@ -280,7 +280,9 @@ class CovarTrainer:
         if self.optim_type == "SGD":
             self.optimizer = torch.optim.SGD(groups, momentum=self.momentum)
         else:
-            self.optimizer = torch.optim.Adam(groups)
+            # the fused Adam (one kernel per step for all parameters) on the device, the same update
+            on_device = all(p.device.type == "cuda" for g in groups for p in g["params"])
+            self.optimizer = torch.optim.Adam(groups, fused=on_device or None)
         self.scheduler = torch.optim.lr_scheduler.ReduceLROnPlateau(self.optimizer, patience=self.scheduler_patiece)
 
     def train_epochs(self, max_epochs: int, restart_optimizer: bool = False) -> None:

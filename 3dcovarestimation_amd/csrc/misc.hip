@@ -42,6 +42,21 @@ __global__ void k_pts_rot(const float* __restrict__ rot, float* __restrict__ pts
   }
 }
 
+// Points (n, 3, P) -> (3, n P) wrapped to [-pi, pi): remainder(x + pi, 2 pi) - pi, the
+// torch.remainder formula x - m floor(x / m), in the points' own precision.
+template <typename T>
+__global__ void k_points_wrap(const T* __restrict__ in, T* __restrict__ out, int64_t n, int64_t P) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * P) return;
+  const int64_t i = idx / P, k = idx - i * P;
+  const T pi = (T)M_PI, twopi = (T)(2.0 * M_PI);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const T p = in[(i * 3 + a) * P + k] + pi;
+    out[a * n * P + idx] = (p - twopi * floor(p / twopi)) - pi;
+  }
+}
+
 // ----------------------------------------------------------------------------
 // CTF (the formula of aspire's CTFFilter._evaluate on evaluate_grid's grid):
 //   omega = pi*[gx, gy]; f = omega / (2 pi pixel); angle = atan2(fy, fx) - ang;
@@ -354,6 +369,22 @@ extern "C" int cov3d_pts_rot(const void* rot, void* pts, int64_t B, int64_t L, v
   hipLaunchKernelGGL(cov3d::k_pts_rot, dim3((unsigned)cov3d::ceil_div(n, cov3d::kMiscBlock)),
                      dim3(cov3d::kMiscBlock), 0, cov3d::as_stream(stream), (const float*)rot,
                      (float*)pts, B, (int)L);
+  COV3D_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int cov3d_points_wrap(const void* in, void* out, int64_t n, int64_t P, int dtype, void* stream) {
+  COV3D_CHECK_ARG(n >= 0 && P >= 0, "bad shape");
+  COV3D_CHECK_ARG(dtype == 0 || dtype == 1, "dtype must be 0 (float32) or 1 (float64)");
+  if (n * P == 0) return 0;
+  COV3D_CHECK_ARG(in && out && in != out, "NULL or aliased pointer");
+  const dim3 grid((unsigned)cov3d::ceil_div(n * P, cov3d::kMiscBlock));
+  if (dtype == 0)
+    hipLaunchKernelGGL(cov3d::k_points_wrap<float>, grid, dim3(cov3d::kMiscBlock), 0, cov3d::as_stream(stream),
+                       (const float*)in, (float*)out, n, P);
+  else
+    hipLaunchKernelGGL(cov3d::k_points_wrap<double>, grid, dim3(cov3d::kMiscBlock), 0, cov3d::as_stream(stream),
+                       (const double*)in, (double*)out, n, P);
   COV3D_LAUNCH_CHECK();
   return 0;
 }

@@ -247,8 +247,10 @@ class NufftPlan(BaseNufftPlan):
         _native.require_device(points, "points")
         self.points_N = points.shape[0]
         self.points_L = int(round(points.shape[-1] ** 0.5))
-        pts = points.transpose(0, 1).reshape(3, -1)
-        pts = (torch.remainder(pts + torch.pi, 2 * torch.pi) - torch.pi).to(self.dtype).contiguous()
+        src = points.to(self.dtype).contiguous()
+        pts = torch.empty((3, src.shape[0] * src.shape[2]), dtype=self.dtype, device=src.device)
+        _native.check(self._lib.cov3d_points_wrap(src.data_ptr(), pts.data_ptr(), src.shape[0], src.shape[2],
+                                                  _dtype_code(self.dtype), _native.stream_ptr(src.device)))
         self.points = pts
         self._pts_synced = False
 

@@ -165,6 +165,12 @@ int cov3d_ml_grad_images(const void* c, const void* ctf, const void* yhat,
 int cov3d_pts_rot(const void* rot, void* pts, int64_t B, int64_t L,
                   void* stream);
 
+/* Points of NufftPlan.setpts (nufft_plan.py:318-336): in (n, 3, P) -> out (3, n*P),
+ * each coordinate wrapped to [-pi, pi) as remainder(x + pi, 2 pi) - pi.  dtype 0 = f32,
+ * 1 = f64; in and out must not overlap. */
+int cov3d_points_wrap(const void* in, void* out, int64_t n, int64_t P, int dtype,
+                      void* stream);
+
 /* Radial CTF on the centred L x L grid (textbook formula, the one aspire's
  * RadialCTFFilter / CTFFilter.evaluate_grid uses, dataset.py:117-125):
  *   params (B, 7) f32 = [defocus_u A, defocus_v A, defocus_ang rad,
