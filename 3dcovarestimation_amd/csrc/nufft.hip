@@ -236,50 +236,6 @@ __global__ void k_bin_count(const T* x, const T* y, const T* z, int64_t M, BinGe
   atomicAdd(&bin_count[bin], 1);
 }
 
-// Single-workgroup scans: bin_start (exclusive over bins) and, per super-bin,
-// the number of interpolation subproblems (ceil(points / kMaxSub)).
-__global__ void __launch_bounds__(1024) k_bin_scan(const int* bin_count, int64_t nsuper,
-                                                   int* bin_start, int* sub_start,
-                                                   int* nsub_total) {
-  __shared__ int s_pts[1024];
-  __shared__ int s_sub[1024];
-  const int tid = threadIdx.x;
-  const int64_t chunk = (nsuper + 1023) / 1024;
-  const int64_t lo = tid * chunk;
-  const int64_t hi = lo + chunk < nsuper ? lo + chunk : nsuper;
-  int sp = 0, ss = 0;
-  for (int64_t s = lo; s < hi; ++s) {
-    int c = 0;
-    for (int k = 0; k < 8; ++k) c += bin_count[8 * s + k];
-    sp += c;
-    ss += (c + kMaxSub - 1) / kMaxSub;
-  }
-  s_pts[tid] = sp;
-  s_sub[tid] = ss;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    int a = tid >= off ? s_pts[tid - off] : 0;
-    int b = tid >= off ? s_sub[tid - off] : 0;
-    __syncthreads();
-    s_pts[tid] += a;
-    s_sub[tid] += b;
-    __syncthreads();
-  }
-  int run_p = s_pts[tid] - sp;
-  int run_s = s_sub[tid] - ss;
-  for (int64_t s = lo; s < hi; ++s) {
-    int c = 0;
-    for (int k = 0; k < 8; ++k) {
-      bin_start[8 * s + k] = run_p + c;
-      c += bin_count[8 * s + k];
-    }
-    sub_start[s] = run_s;
-    run_p += c;
-    run_s += (c + kMaxSub - 1) / kMaxSub;
-  }
-  if (tid == 1023) *nsub_total = s_sub[1023];
-}
-
 __global__ void k_sub_fill(const int* bin_count, const int* bin_start, const int* sub_start,
                            int64_t nsuper, int* sub_sb, int* sub_pt, int* sub_n) {
   int64_t sb = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -314,6 +270,22 @@ __global__ void k_bin_place(const T* x, const T* y, const T* z, int64_t M, BinGe
 
 hipError_t sort_points_by_bin(void* tmp, size_t& tmp_bytes, const unsigned* keys_in, unsigned* keys_out,
                               int* idx_out, int n, int nbits, hipStream_t st);
+hipError_t exclusive_scan_ints(void* tmp, size_t& tmp_bytes, const int* in, int* out, int n, hipStream_t st);
+
+// Subproblems of each super-bin (ceil(points / kMaxSub)), scanned into sub_start afterwards.
+__global__ void k_sub_count(const int* __restrict__ bin_count, int64_t nsuper, int* __restrict__ sub_cnt) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsuper) return;
+  int c = 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) c += bin_count[8 * s + k];
+  sub_cnt[s] = (c + kMaxSub - 1) / kMaxSub;
+}
+
+__global__ void k_sub_total(const int* __restrict__ sub_cnt, const int* __restrict__ sub_start, int64_t nsuper,
+                            int* __restrict__ nsub) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *nsub = nsuper > 0 ? sub_start[nsuper - 1] + sub_cnt[nsuper - 1] : 0;
+}
 
 // fft_pruned.hip
 bool pruned_fft_supported(int64_t NF, int64_t N);
@@ -508,6 +480,34 @@ __global__ void k_to_sorted(const Cplx<T>* __restrict__ c, Cplx<T>* __restrict__
   }
   for (int t = 0; t < ntrans; ++t) dst[t] = c[(int64_t)t * M + j];
   for (int t = ntrans; t < ldc; ++t) dst[t] = Cplx<T>{(T)0, (T)0};
+}
+
+// FP32 point-major staging for rows of even length ldc <= kStageLdc: a workgroup takes 256
+// consecutive points, reads their values coalesced ([t][j]) into LDS, then writes each point's
+// row as consecutive 16-byte pieces, so one wave instruction covers about 6 rows instead of
+// 64 scattered ones.
+constexpr int kStageLdc = 24;
+__global__ void __launch_bounds__(kBlock)
+    k_to_sorted_rows(const float2* __restrict__ c, float2* __restrict__ cs, const int* __restrict__ spos,
+                     int64_t M, int ntrans, int ldc) {
+  __shared__ float2 s_v[kBlock * (kStageLdc + 1)];
+  __shared__ int s_pos[kBlock];
+  const int64_t j0 = (int64_t)blockIdx.x * kBlock;
+  const int tid = threadIdx.x;
+  const int64_t j = j0 + tid;
+  s_pos[tid] = j < M ? spos[j] : -1;
+  for (int t = 0; t < ldc; ++t)
+    s_v[tid * (kStageLdc + 1) + t] = (j < M && t < ntrans) ? c[(int64_t)t * M + j] : make_float2(0.f, 0.f);
+  __syncthreads();
+  const int hp = ldc >> 1;  // 16-byte pieces per row
+  for (int e = tid; e < kBlock * hp; e += kBlock) {
+    const int q = e / hp, h = e - q * hp;
+    const int sp = s_pos[q];
+    if (sp >= 0) {
+      const float2 a = s_v[q * (kStageLdc + 1) + 2 * h], b = s_v[q * (kStageLdc + 1) + 2 * h + 1];
+      reinterpret_cast<float4*>(cs + (int64_t)sp * ldc)[h] = float4{a.x, a.y, b.x, b.y};
+    }
+  }
 }
 
 template <typename T>
@@ -1696,6 +1696,7 @@ struct Plan {
   int* bin_count = nullptr;
   int* bin_start = nullptr;
   int* sub_start = nullptr;
+  int* sub_cnt = nullptr;     // subproblems per super-bin (scanned into sub_start)
   int* sub_sb = nullptr;
   int* sub_pt = nullptr;
   int* sub_n = nullptr;
@@ -1844,8 +1845,12 @@ static int launch_spread(Plan* p, const void* c_in, int ntrans_run, hipStream_t 
   COV3D_CHECK_ARG(ldc <= p->ldc_cap, "sorted row stride %d exceeds %d", ldc, p->ldc_cap);
   {
     ScopedTiming tm(kTPermute, st);
-    hipLaunchKernelGGL((k_to_sorted<T>), dim3((unsigned)ceil_div(p->M, kBlock)), dim3(kBlock), 0, st,
-                       (const Cplx<T>*)c_in, (Cplx<T>*)p->csort, p->spos, p->M, ntrans_run, ldc);
+    if (sizeof(T) == 4 && (ldc & 1) == 0 && ldc <= kStageLdc)
+      hipLaunchKernelGGL(k_to_sorted_rows, dim3((unsigned)ceil_div(p->M, kBlock)), dim3(kBlock), 0, st,
+                         (const float2*)c_in, (float2*)p->csort, p->spos, p->M, ntrans_run, ldc);
+    else
+      hipLaunchKernelGGL((k_to_sorted<T>), dim3((unsigned)ceil_div(p->M, kBlock)), dim3(kBlock), 0, st,
+                         (const Cplx<T>*)c_in, (Cplx<T>*)p->csort, p->spos, p->M, ntrans_run, ldc);
     COV3D_LAUNCH_CHECK();
   }
   dim3 grid((unsigned)p->nbins, (unsigned)ceil_div(ntrans_run, TC));
@@ -2215,9 +2220,28 @@ static int setpts_impl(Plan* p, int64_t M, const T* x, const T* y, const T* z, h
     COV3D_HIP(sort_points_by_bin(p->sort_tmp, need, (const unsigned*)p->pt_bin, (unsigned*)p->pt_pos, p->sidx,
                                  (int)M, nbits, st));
   }
-  hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, st, p->bin_count, p->nsuper, p->bin_start,
-                     p->sub_start, p->nsub);
-  COV3D_LAUNCH_CHECK();
+  {
+    // bin starts and subproblem starts: device-wide exclusive scans
+    size_t need1 = 0, need2 = 0;
+    COV3D_HIP(exclusive_scan_ints(nullptr, need1, p->bin_count, p->bin_start, (int)p->nbins, st));
+    COV3D_HIP(exclusive_scan_ints(nullptr, need2, p->sub_cnt, p->sub_start, (int)p->nsuper, st));
+    const size_t need = std::max(need1, need2);
+    if (need > p->sort_tmp_bytes) {
+      COV3D_HIP(hipStreamSynchronize(st));
+      hipFree(p->sort_tmp);
+      p->sort_tmp = nullptr;
+      p->sort_tmp_bytes = 0;
+      COV3D_HIP(hipMalloc(&p->sort_tmp, need));
+      p->sort_tmp_bytes = need;
+    }
+    COV3D_HIP(exclusive_scan_ints(p->sort_tmp, need1, p->bin_count, p->bin_start, (int)p->nbins, st));
+    hipLaunchKernelGGL(k_sub_count, dim3((unsigned)ceil_div(p->nsuper, kBlock)), dim3(kBlock), 0, st,
+                       p->bin_count, p->nsuper, p->sub_cnt);
+    COV3D_LAUNCH_CHECK();
+    COV3D_HIP(exclusive_scan_ints(p->sort_tmp, need2, p->sub_cnt, p->sub_start, (int)p->nsuper, st));
+    hipLaunchKernelGGL(k_sub_total, dim3(1), dim3(64), 0, st, p->sub_cnt, p->sub_start, p->nsuper, p->nsub);
+    COV3D_LAUNCH_CHECK();
+  }
   if (M > 0) {
     hipLaunchKernelGGL(k_sub_fill, dim3((unsigned)ceil_div(p->nsuper, kBlock)), dim3(kBlock), 0, st,
                        p->bin_count, p->bin_start, p->sub_start, p->nsuper, p->sub_sb, p->sub_pt,
@@ -2261,6 +2285,7 @@ static int destroy_plan(Plan* p) {
   hipFree(p->bin_count);
   hipFree(p->bin_start);
   hipFree(p->sub_start);
+  hipFree(p->sub_cnt);
   hipFree(p->nsub);
   hipFree(p->fft_work);
   hipFree(p->rbuf);
@@ -2345,6 +2370,7 @@ static int make_plan(int type, const int64_t* n, int iflag, int64_t ntrans, doub
   if (hipMalloc(&p->bin_count, p->nbins * sizeof(int)) != hipSuccess ||
       hipMalloc(&p->bin_start, p->nbins * sizeof(int)) != hipSuccess ||
       hipMalloc(&p->sub_start, p->nsuper * sizeof(int)) != hipSuccess ||
+      hipMalloc(&p->sub_cnt, p->nsuper * sizeof(int)) != hipSuccess ||
       hipMalloc(&p->nsub, sizeof(int)) != hipSuccess) {
     set_error("hipMalloc of bin tables failed");
     return fail(COV3D_ERR_HIP);
