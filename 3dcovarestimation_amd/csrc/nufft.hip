@@ -616,11 +616,23 @@ constexpr int kStageGroup = 7;
 // half_h2 > 0: the fine grid holds only the Hermitian half m2 < half_h2 (= nf2/2 + 1) of the
 // transform of a real volume; a cell m with m2 >= half_h2 is conj(F[-m]).  conj_all conjugates
 // every value (the half grid was computed with the opposite sign).
+// Plane stride (cells) of the interpolation tile [e0][e1][e2].  The FP32 tile of two transforms
+// per 16-byte cell with W = 7 pads its planes from 225 to 234 cells: a ds_read_b128 from 16
+// lanes is conflict-free when their cells differ mod 16, and on the slices of a central-slice
+// batch the padded stride spreads the points' base cells over the 16 classes more evenly (a
+// conflict model over L = 128, 1024-image subproblems: 1.99 -> 1.49 cycles per 16-lane read;
+// measured conflicts were half of the LDS-active cycles).
 template <typename T, int W, int TC>
+constexpr int interp_plane() {
+  return (sizeof(T) == 4 && TC == 2 && W == 7) ? 234 : (kSuperEdge + W) * (kSuperEdge + W);
+}
+
+template <typename T, int W, int TC, int PL = (kSuperEdge + W) * (kSuperEdge + W)>
 __device__ __forceinline__ void stage_tile(const Cplx<T>* __restrict__ fine, Cplx<T>* tile, const BinGeom& g,
                                            int o0, int o1, int o2, int t0, int ntrans, int half_h2 = 0,
                                            int conj_all = 0) {
   constexpr int E = kSuperEdge + W;
+  static_assert(PL >= E * E, "tile plane stride below the plane size");
   constexpr int ncell = E * E * E;
   constexpr int NIT = (ncell + kBlock - 1) / kBlock;
   const int64_t nfine = half_h2 ? (int64_t)g.nf[0] * g.nf[1] * half_h2 : (int64_t)g.nf[0] * g.nf[1] * g.nf[2];
@@ -661,8 +673,9 @@ __device__ __forceinline__ void stage_tile(const Cplx<T>* __restrict__ fine, Cpl
     for (int k = 0; k < kStageGroup; ++k) {
       const int cidx = threadIdx.x + (it0 + k) * kBlock;
       if (it0 + k < NIT && cidx < ncell) {
+        const int tidx = cidx + (cidx / (E * E)) * (PL - E * E);
 #pragma unroll
-        for (int tt = 0; tt < TC; ++tt) tile[cidx * TC + tt] = v[k][tt];
+        for (int tt = 0; tt < TC; ++tt) tile[tidx * TC + tt] = v[k][tt];
       }
     }
   }
@@ -698,7 +711,7 @@ __global__ void __launch_bounds__(kBlock)
       const int m0 = min(max((int)ceilf(sv0[p] - halfw) + W / 2, 0), kSuperEdge);
       const int m1 = min(max((int)ceilf(sv1[p] - halfw) + W / 2, 0), kSuperEdge);
       const int m2 = min(max((int)ceilf(sv2[p] - halfw) + W / 2, 0), kSuperEdge);
-      rb[k] = ((m0 * E + m1) * E + m2) & 15;
+      rb[k] = (m0 * interp_plane<float, W, 2>() + m1 * E + m2) & 15;
       rr[k] = atomicAdd(&s_cnt[rb[k]], 1);
     }
   }
@@ -745,7 +758,8 @@ __global__ void __launch_bounds__(kBlock)
   const int np = sd.sub_n[sub];
   // Points are visited in the bank-aware order computed at setpts (k_bank_order), when present.
   // halo cells lie within one period of the grid (nf >= 2W), so one conditional wrap suffices
-  stage_tile<T, W, TC>(fine, tile, g, o0, o1, o2, t0, ntrans, half_h2, conj_all);
+  constexpr int PL = interp_plane<T, W, TC>();
+  stage_tile<T, W, TC, PL>(fine, tile, g, o0, o1, o2, t0, ntrans, half_h2, conj_all);
   __syncthreads();
   for (int slot = threadIdx.x; slot < np; slot += blockDim.x) {
     const int p = p0 + (iord ? iord[p0 + slot] : slot);
@@ -768,7 +782,7 @@ __global__ void __launch_bounds__(kBlock)
       const float4* t4 = reinterpret_cast<const float4*>(tile);
       float4 buf[2][W * H];
       auto load_row = [&](int a0, int a1, float4* b) {
-        const int row = ((l0 + a0) * E + (l1 + a1)) * E + l2;
+        const int row = (l0 + a0) * PL + (l1 + a1) * E + l2;
 #pragma unroll
         for (int a2 = 0; a2 < W; ++a2)
 #pragma unroll
@@ -798,7 +812,7 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
         for (int a1 = 0; a1 < W; ++a1) {
           const T w01 = w0[a0] * w1[a1];
-          const int row = ((l0 + a0) * E + (l1 + a1)) * E + l2;
+          const int row = (l0 + a0) * PL + (l1 + a1) * E + l2;
 #pragma unroll
           for (int a2 = 0; a2 < W; ++a2) {
             const T ww = w01 * w2[a2];
@@ -1769,9 +1783,11 @@ static int free_points(Plan* p) {
 template <typename T, int W, int TC>
 static int launch_interp(Plan* p, void* c_out, int ntrans_run, hipStream_t st) {
   const size_t smem = (size_t)(kSuperEdge + W) * (kSuperEdge + W) * (kSuperEdge + W) * TC * sizeof(Cplx<T>);
-  COV3D_CHECK_ARG(smem <= 160 * 1024, "NUFFT tile of %zu bytes exceeds LDS", smem);
+  // k_interp's tile may pad its planes (interp_plane)
+  const size_t smem_k = (size_t)(kSuperEdge + W) * interp_plane<T, W, TC>() * TC * sizeof(Cplx<T>);
+  COV3D_CHECK_ARG(smem_k <= 160 * 1024, "NUFFT tile of %zu bytes exceeds LDS", smem_k);
   static bool attr_done = false;
-  if (smem > 65536 && !attr_done) {
+  if (std::max(smem, smem_k) > 65536 && !attr_done) {
     COV3D_HIP(hipFuncSetAttribute((const void*)k_interp<T, W, TC>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     COV3D_HIP(hipFuncSetAttribute((const void*)k_interp_tp<T, W, TC>,
@@ -1802,7 +1818,7 @@ static int launch_interp(Plan* p, void* c_out, int ntrans_run, hipStream_t st) {
                          (Cplx<T>*)p->csort, sd, p->geom, (const T*)p->sv[0], (const T*)p->sv[1],
                          (const T*)p->sv[2], p->M, ntrans_run, kc, (T)p->beta);
     else
-      hipLaunchKernelGGL((k_interp<T, W, TC>), grid, dim3(kBlock), smem, st, (const Cplx<T>*)p->fine,
+      hipLaunchKernelGGL((k_interp<T, W, TC>), grid, dim3(kBlock), smem_k, st, (const Cplx<T>*)p->fine,
                          (Cplx<T>*)p->csort, sd, p->geom, (const T*)p->sv[0], (const T*)p->sv[1],
                          (const T*)p->sv[2], p->sidx, p->M, ntrans_run, kc, (T)p->beta, p->stage_half,
                          p->stage_conj, (TC == 2 && p->iord_valid) ? (const int*)p->iord : nullptr);
