@@ -155,16 +155,20 @@ def test_vol_forward_fourier_slice_closed_form(dev):
     assert rel_l2(lhs.cpu().numpy(), rhs.cpu().numpy()) < 1e-5
 
 
-@pytest.mark.parametrize("spread_mode,interp_mode,ntrans", [("0", "2", 3), ("2", "1", 5), ("0", "0", 20), ("2", "2", 21), ("3", "0", 7),
-                                                           ("4", "0", 20), ("4", "0", 3), ("4", "2", 21)])
-def test_kernel_variants_match_nudft(dev, monkeypatch, spread_mode, interp_mode, ntrans):
+@pytest.mark.parametrize("spread_mode,interp_mode,ntrans,direct", [("0", "2", 3, "1"), ("2", "1", 5, "1"),
+                                                                  ("0", "0", 20, "1"), ("2", "2", 21, "1"),
+                                                                  ("3", "0", 7, "0"), ("4", "0", 20, "1"),
+                                                                  ("4", "0", 3, "0"), ("4", "2", 21, "1")])
+def test_kernel_variants_match_nudft(dev, monkeypatch, spread_mode, interp_mode, ntrans, direct):
     """Every selectable spreading / interpolation kernel (read at plan creation from
-    COV3D_SPREAD_MODE / COV3D_INTERP_MODE) against the NUDFT, including transform
+    COV3D_SPREAD_MODE / COV3D_INTERP_MODE; COV3D_INTERP_DIRECT=0 keeps the sorted output and
+    its permute pass) against the NUDFT, including transform
     counts that leave partial chunks (3, 5, 21) and the 20-transform SGD shape."""
     o = oracle()
     npl = pkg("nufft_plan")
     monkeypatch.setenv("COV3D_SPREAD_MODE", spread_mode)
     monkeypatch.setenv("COV3D_INTERP_MODE", interp_mode)
+    monkeypatch.setenv("COV3D_INTERP_DIRECT", direct)
     L = 12
     pts = _points(L, 2, 1, 300 + ntrans, dev, spread=1.2)
     plan = npl.NufftPlan((L,) * 3, batch_size=ntrans, eps=1e-6, device=dev)
