@@ -1431,15 +1431,21 @@ __global__ void __launch_bounds__(kGatherBlock)
 #pragma unroll
     for (int cb = 0; cb < NCB; ++cb) bv[cb] = row[sub + 16 * cb];
   };
+  // A cell block (one x level of the bin) whose x weight is zero for all 4 hits of a step adds
+  // nothing: its 3 MFMAs are skipped (wave-uniform ballot per level).
   auto mma = [&](int slot, const float* bv) {
     const float4 wx = *reinterpret_cast<const float4*>(&s_w[slot * 12]);
     const float wyz = s_w[slot * 12 + 4 + (sub >> 2)] * s_w[slot * 12 + 8 + (sub & 3)];
-    const float a[4] = {wx.x * wyz, wx.y * wyz, wx.z * wyz, wx.w * wyz};
+    const float wxa[4] = {wx.x, wx.y, wx.z, wx.w};
 #pragma unroll
-    for (int c0 = 0; c0 < 4; ++c0)
+    for (int c0 = 0; c0 < 4; ++c0) {
+      if (__ballot(wxa[c0] != 0.f)) {
+        const float a = wxa[c0] * wyz;
 #pragma unroll
-      for (int cb = 0; cb < NCB; ++cb)
-        acc[c0][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[c0], bv[cb], acc[c0][cb], 0, 0, 0);
+        for (int cb = 0; cb < NCB; ++cb)
+          acc[c0][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bv[cb], acc[c0][cb], 0, 0, 0);
+      }
+    }
   };
   // steps over hit slots [0, nfull), nfull a multiple of 8; the B entries of step s + 2 are
   // loaded during step s (two register sets, no copies between them).  The loads are issued
