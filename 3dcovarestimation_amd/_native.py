@@ -30,6 +30,8 @@ _SIGNATURES = {
     "cov3d_nufft_execute": [c_vp, c_vp, c_vp, c_int, c_i64, c_vp],
     "cov3d_nufft_execute_dir": [c_vp, c_int, c_vp, c_vp, c_int, c_i64, c_vp],
     "cov3d_nufft_destroy": [c_vp],
+    "cov3d_nufft_sorted_input": [c_vp, c_i64, ctypes.POINTER(c_vp), ctypes.POINTER(c_i64), ctypes.POINTER(c_vp)],
+    "cov3d_nufft_adjoint_presorted": [c_vp, c_vp, c_int, c_i64, c_vp],
     "cov3d_nufft_info": [c_vp, ctypes.POINTER(c_i64), ctypes.POINTER(c_int), ctypes.POINTER(c_dbl)],
     "cov3d_slice_forward": [c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_i64, c_i64, c_int, c_int, c_vp],
     "cov3d_slice_adjoint": [c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_i64, c_i64, c_int, c_int, c_i64, c_int, c_vp],
@@ -37,6 +39,8 @@ _SIGNATURES = {
                          c_vp, c_vp, c_vp, c_vp],
     "cov3d_ml_grad_images": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_i64, c_i64, c_i64, c_dbl, c_dbl,
                              c_vp],
+    "cov3d_ml_grad_sorted": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_i64, c_i64, c_i64, c_i64,
+                             c_dbl, c_dbl, c_vp],
     "cov3d_pts_rot": [c_vp, c_vp, c_i64, c_i64, c_vp],
     "cov3d_points_wrap": [c_vp, c_vp, c_i64, c_i64, c_int, c_vp],
     "cov3d_ctf_eval": [c_vp, c_vp, c_i64, c_i64, c_vp],

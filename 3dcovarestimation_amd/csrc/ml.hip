@@ -375,6 +375,76 @@ __global__ void __launch_bounds__(kMlBlock)
   }
 }
 
+// Gradient images written straight into the sorted rows a type-1 NUFFT plan spreads from:
+// value t of pixel k of image i goes to gs[spos[i * np + k] * ldc + t] (t < r; zeros up to ldc).
+// Each workgroup stages its 256 pixels' rows in LDS and writes them as consecutive 16-byte pieces.
+constexpr int kGradLdc = 24;  // largest row stride staged
+template <int RM>
+__global__ void __launch_bounds__(kMlBlock)
+    k_ml_grad_sorted(const float2* __restrict__ c, const float* __restrict__ ctf,
+                     const float2* __restrict__ yhat, const float* __restrict__ z,
+                     const float* __restrict__ minv, const int* __restrict__ pix_map, int64_t np,
+                     float2* __restrict__ gs, const int* __restrict__ spos, int ldc, int64_t B, int r, int L,
+                     float scale) {
+  __shared__ float s_minv[RM * RM];
+  __shared__ float s_z[RM];
+  __shared__ float2 s_v[kMlBlock * (kGradLdc + 1)];
+  __shared__ int s_pos[kMlBlock];
+  const int64_t i = blockIdx.y;
+  const int64_t d = (int64_t)L * L;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < RM * RM; e += blockDim.x) {
+    int a = e / RM, b = e % RM;
+    s_minv[e] = (a < r && b < r) ? minv[(i * r + a) * r + b] : 0.f;
+  }
+  if (tid < RM) s_z[tid] = tid < r ? z[i * r + tid] : 0.f;
+  __syncthreads();
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + tid;
+  float2* row = s_v + tid * (kGradLdc + 1);
+  s_pos[tid] = k < np ? spos[i * np + k] : -1;
+  for (int t = r; t < ldc; ++t) row[t] = make_float2(0.f, 0.f);
+  if (k < np) {
+    float mult;
+    const int64_t px = entry_pixel(pix_map, k, L, mult);
+    const float inv_l = 1.f / (float)L;
+    const float ps = pixel_scale(ctf, i, px, L, inv_l * inv_l);
+    float qx[RM], qy[RM];
+    float ex = 0.f, ey = 0.f;
+#pragma unroll
+    for (int s2 = 0; s2 < RM; ++s2) {
+      float2 v = s2 < r ? c[((int64_t)s2 * B + i) * np + k] : make_float2(0.f, 0.f);
+      qx[s2] = v.x * ps;
+      qy[s2] = v.y * ps;
+      ex += s_z[s2] * qx[s2];
+      ey += s_z[s2] * qy[s2];
+    }
+    float2 yv = yhat[i * d + px];
+    ex -= yv.x * inv_l;
+    ey -= yv.y * inv_l;
+    const float gsc = ps * scale * (float)(L * L) * mult;
+    for (int t = 0; t < r; ++t) {
+      float ax = s_z[t] * ex, ay = s_z[t] * ey;
+#pragma unroll
+      for (int s2 = 0; s2 < RM; ++s2) {
+        float mv = s_minv[t * RM + s2];
+        ax += mv * qx[s2];
+        ay += mv * qy[s2];
+      }
+      row[t] = make_float2(ax * gsc, ay * gsc);
+    }
+  }
+  __syncthreads();
+  const int hp = ldc >> 1;  // 16-byte pieces per row
+  for (int e = tid; e < kMlBlock * hp; e += kMlBlock) {
+    const int q = e / hp, h = e - q * hp;
+    const int sp = s_pos[q];
+    if (sp >= 0) {
+      const float2 a = s_v[q * (kGradLdc + 1) + 2 * h], b = s_v[q * (kGradLdc + 1) + 2 * h + 1];
+      reinterpret_cast<float4*>(gs + (int64_t)sp * ldc)[h] = float4{a.x, a.y, b.x, b.y};
+    }
+  }
+}
+
 }  // namespace cov3d
 
 extern "C" int cov3d_ml_forward(const void* c, const void* ctf, const void* yhat,
@@ -441,6 +511,45 @@ extern "C" int cov3d_ml_grad_images(const void* c, const void* ctf, const void* 
   else
     COV3D_GRAD(32);
 #undef COV3D_GRAD
+  COV3D_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int cov3d_ml_grad_sorted(const void* c, const void* ctf, const void* yhat, const void* z,
+                                    const void* minv, const int* pix_map, int64_t npix, void* g_sorted,
+                                    const int* spos, int64_t ldc, int64_t B, int64_t r, int64_t L,
+                                    double noise_var, double grad_scale, void* stream) {
+  COV3D_CHECK_ARG(r >= 1 && r <= 31, "rank r=%lld must be in [1, 31]", (long long)r);
+  COV3D_CHECK_ARG(B >= 0 && L >= 1, "bad batch/size");
+  COV3D_CHECK_ARG(noise_var > 0, "noise_var must be positive");
+  COV3D_CHECK_ARG(ldc >= r && ldc <= cov3d::kGradLdc && ldc % 2 == 0,
+                  "row stride %lld must be even, >= r and <= %d", (long long)ldc, cov3d::kGradLdc);
+  if (B == 0) return 0;
+  COV3D_CHECK_ARG(c && yhat && z && minv && g_sorted && spos, "NULL pointer");
+  COV3D_CHECK_ARG(B < 65536, "batch too large for one launch (B=%lld)", (long long)B);
+  COV3D_CHECK_ARG(pix_map ? (npix >= 1 && npix <= L * L) : (npix == 0 || npix == L * L),
+                  "npix=%lld invalid for L=%lld", (long long)npix, (long long)L);
+  hipStream_t st = cov3d::as_stream(stream);
+  const int64_t np = pix_map ? npix : L * L;
+  const float scale = (float)(grad_scale / ((double)(L * L) * noise_var));
+  dim3 grid((unsigned)cov3d::ceil_div(np, cov3d::kMlBlock), (unsigned)B);
+#define COV3D_GRADS(RM)                                                                                   \
+  hipLaunchKernelGGL(cov3d::k_ml_grad_sorted<RM>, grid, dim3(cov3d::kMlBlock), 0, st, (const float2*)c, \
+                     (const float*)ctf, (const float2*)yhat, (const float*)z, (const float*)minv, pix_map, \
+                     np, (float2*)g_sorted, spos, (int)ldc, B, (int)r, (int)L, scale)
+  if (r <= 4)
+    COV3D_GRADS(4);
+  else if (r <= 8)
+    COV3D_GRADS(8);
+  else if (r <= 12)
+    COV3D_GRADS(12);
+  else if (r <= 16)
+    COV3D_GRADS(16);
+  else if (r <= 20)
+    COV3D_GRADS(20);
+  else
+    COV3D_GRADS(24);
+#undef COV3D_GRADS
   COV3D_LAUNCH_CHECK();
   return 0;
 }

@@ -1713,6 +1713,7 @@ struct Plan {
   int spread_mode = 0;  // 0 = LDS-staged gather, 1 = LDS atomic tiles, 2/3 = LDS-free gather (FP32; 3 with L2 touch), 4 = FP32 MFMA gather
   int interp_mode = 0;  // 0 = thread per point, 1 = wavefront per point, 2 = eight lanes per point
   int interp_direct = 1;  // mode 0 writes the (ntrans, M) output directly instead of sorted + permute
+  bool presorted = false;  // the current type-1 call's values are already in csort (no permute)
   int max_tc_gather = 20;  // transforms per gather workgroup
   int tc_spread = 2;    // transforms per spreading workgroup (tile mode)
   double beta = 0, eps = 1e-6;
@@ -1878,7 +1879,7 @@ static int launch_spread(Plan* p, const void* c_in, int ntrans_run, hipStream_t 
   const bool sc = sizeof(T) == 4 && (p->spread_mode >= 2 && p->spread_mode <= 4);
   const int ldc = sc ? (int)ceil_div(ntrans_run, TC) * TC : ntrans_run;
   COV3D_CHECK_ARG(ldc <= p->ldc_cap, "sorted row stride %d exceeds %d", ldc, p->ldc_cap);
-  {
+  if (!p->presorted) {
     ScopedTiming tm(kTPermute, st);
     if (sizeof(T) == 4 && (ldc & 1) == 0 && ldc <= kStageLdc)
       hipLaunchKernelGGL(k_to_sorted_rows, dim3((unsigned)ceil_div(p->M, kBlock)), dim3(kBlock), 0, st,
@@ -1981,10 +1982,16 @@ static int gather_ldc(int n, int max_tc) {
   return (int)ceil_div(n, tc) * tc;
 }
 
+// Row stride of the sorted values the spreading of n transforms reads (launch_spread / run_spread).
+static int spread_ldc(const Plan* p, int n) {
+  const bool sc = p->dtype == 0 && p->spread_mode >= 2 && p->spread_mode <= 4;
+  return sc ? gather_ldc(n, p->max_tc_gather) : n;
+}
+
 template <typename T>
 static int run_spread(Plan* p, const void* c_in, int ntrans_run, hipStream_t st) {
   if (p->spread_mode == 1) {
-    {
+    if (!p->presorted) {
       ScopedTiming tm(kTPermute, st);
       hipLaunchKernelGGL((k_to_sorted<T>), dim3((unsigned)ceil_div(p->M, kBlock)), dim3(kBlock), 0, st,
                          (const Cplx<T>*)c_in, (Cplx<T>*)p->csort, p->spos, p->M, ntrans_run, ntrans_run);
@@ -2512,6 +2519,45 @@ extern "C" int cov3d_nufft_execute_dir(void* plan, int type, void* c, void* f, i
   }
   if (p->dtype == 1) return cov3d::exec_type1<double>(p, c, f, f_is_real != 0, ntrans_run, st);
   return cov3d::exec_type1<float>(p, c, f, f_is_real != 0, ntrans_run, st);
+}
+
+extern "C" int cov3d_nufft_sorted_input(void* plan, int64_t ntrans_run, void** values, int64_t* ldc,
+                                        void** spos) {
+  COV3D_CHECK_ARG(plan != nullptr && values && ldc && spos, "NULL pointer");
+  Plan* p = static_cast<Plan*>(plan);
+  COV3D_CHECK_ARG(p->type == 0 || p->type == 1, "plan of type %d has no type-1 input", p->type);
+  if (!p->pts_set) {
+    cov3d::set_error("sorted_input called before setpts");
+    return COV3D_ERR_STATE;
+  }
+  if (ntrans_run <= 0) ntrans_run = p->ntrans;
+  COV3D_CHECK_ARG(ntrans_run <= p->ntrans, "ntrans_run %lld > plan ntrans %lld", (long long)ntrans_run,
+                  (long long)p->ntrans);
+  *values = p->csort;
+  *ldc = cov3d::spread_ldc(p, (int)ntrans_run);
+  *spos = p->spos;
+  return 0;
+}
+
+extern "C" int cov3d_nufft_adjoint_presorted(void* plan, void* f, int f_is_real, int64_t ntrans_run,
+                                             void* stream) {
+  COV3D_CHECK_ARG(plan != nullptr, "plan is NULL");
+  Plan* p = static_cast<Plan*>(plan);
+  COV3D_CHECK_ARG(p->type == 0 || p->type == 1, "plan of type %d cannot run type 1", p->type);
+  if (!p->pts_set) {
+    cov3d::set_error("execute called before setpts");
+    return COV3D_ERR_STATE;
+  }
+  if (ntrans_run <= 0) ntrans_run = p->ntrans;
+  COV3D_CHECK_ARG(ntrans_run <= p->ntrans, "ntrans_run %lld > plan ntrans %lld", (long long)ntrans_run,
+                  (long long)p->ntrans);
+  COV3D_CHECK_ARG(f != nullptr, "output pointer must not be NULL");
+  hipStream_t st = cov3d::as_stream(stream);
+  p->presorted = true;
+  const int rc = p->dtype == 1 ? cov3d::exec_type1<double>(p, p->csort, f, f_is_real != 0, ntrans_run, st)
+                               : cov3d::exec_type1<float>(p, p->csort, f, f_is_real != 0, ntrans_run, st);
+  p->presorted = false;
+  return rc;
 }
 
 extern "C" int cov3d_nufft_execute(void* plan, void* c, void* f, int f_is_real, int64_t ntrans_run,

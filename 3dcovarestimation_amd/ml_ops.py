@@ -130,6 +130,33 @@ def ml_grad_images(
     return g
 
 
+def ml_grad_sorted(
+    c: torch.Tensor,
+    yhat: torch.Tensor,
+    z: torch.Tensor,
+    minv: torch.Tensor,
+    L: int,
+    noise_var: float,
+    grad_scale: float,
+    values: int,
+    spos: int,
+    ldc: int,
+    ctf: Optional[torch.Tensor] = None,
+    pix_map: Optional[torch.Tensor] = None,
+) -> None:
+    """ml_grad_images written into a plan's sorted type-1 input (``NufftPlan.sorted_input``)."""
+    r, B = c.shape[0], c.shape[1]
+    ctf_t = ctf.to(torch.float32).contiguous() if ctf is not None else None
+    lib = _native.load()
+    _native.check(
+        lib.cov3d_ml_grad_sorted(
+            c.data_ptr(), _native.ptr(ctf_t), yhat.contiguous().data_ptr(), z.contiguous().data_ptr(),
+            minv.contiguous().data_ptr(), _native.ptr(pix_map), 0 if pix_map is None else pix_map.numel(),
+            values, spos, ldc, B, r, L, float(noise_var), float(grad_scale), _native.stream_ptr(c.device),
+        )
+    )
+
+
 def _half_plan(plan):
     """A companion plan of ``plan`` for the half-plane points (created once, kept on the plan)."""
     hp = getattr(plan, "_half_plan", None)
@@ -184,8 +211,17 @@ class MLCostNufft(torch.autograd.Function):
     def backward(ctx, grad_out):
         c, yhat, z, minv, ctf = ctx.saved_tensors
         L, B, noise_var = ctx.meta
-        g = ml_grad_images(c, yhat, z, minv, L, noise_var, 1.0 / B, ctf=ctf if ctx.has_ctf else None, out=c,
+        r = c.shape[0]
+        plan = ctx.plan
+        ctf_arg = ctf if ctx.has_ctf else None
+        vals, ldc, spos = plan.sorted_input(r) if plan.dtype == torch.float32 else (None, 0, None)
+        if vals is not None and ldc % 2 == 0 and r <= ldc <= 24:
+            # gradient rows go straight to where the type-1 spreading reads them
+            ml_grad_sorted(c, yhat, z, minv, L, noise_var, 1.0 / B, vals, spos, ldc, ctf=ctf_arg,
                            pix_map=ctx.pix)
-        grad_v = ctx.plan.execute_adjoint(g.reshape(g.shape[0], -1), real_output=True)
-        grad_v = grad_v[: g.shape[0]] * grad_out
+            grad_v = plan.execute_adjoint_presorted(r, real_output=True)
+        else:
+            g = ml_grad_images(c, yhat, z, minv, L, noise_var, 1.0 / B, ctf=ctf_arg, out=c, pix_map=ctx.pix)
+            grad_v = plan.execute_adjoint(g.reshape(r, -1), real_output=True)
+        grad_v = grad_v[:r] * grad_out
         return grad_v, None, None, None, None, None, None, None

@@ -350,6 +350,25 @@ class NufftPlan(BaseNufftPlan):
             )
         return out
 
+    def sorted_input(self, n: int) -> Tuple[int, int, int]:
+        """(values, ldc, spos) device pointers of the plan's sorted type-1 input for n transforms:
+        point j's value t goes to values[spos[j] * ldc + t] (complex), rows zero-padded to ldc."""
+        self._sync_points()
+        vals, ldc, spos = ctypes.c_void_p(), ctypes.c_int64(), ctypes.c_void_p()
+        _native.check(self._lib.cov3d_nufft_sorted_input(self._handle, int(n), ctypes.byref(vals),
+                                                          ctypes.byref(ldc), ctypes.byref(spos)))
+        return vals.value, ldc.value, spos.value
+
+    def execute_adjoint_presorted(self, n: int, real_output: bool = False) -> torch.Tensor:
+        """Type-1 NUFFT of n transforms whose values were written to ``sorted_input(n)``."""
+        self._sync_points()
+        out_dtype = self.dtype if real_output else self.complex_dtype
+        out = self._output((self.batch_size, *self.sz), out_dtype, n)
+        with torch.cuda.device(self.device):
+            _native.check(self._lib.cov3d_nufft_adjoint_presorted(
+                self._handle, out.data_ptr(), 1 if real_output else 0, int(n), _native.stream_ptr(self.device)))
+        return out
+
 
 class NufftType(Enum):
     DISCRETIZED = NufftPlanDiscretized

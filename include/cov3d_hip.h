@@ -76,6 +76,21 @@ int cov3d_nufft_execute(void* plan, void* c, void* f, int f_is_real,
 int cov3d_nufft_execute_dir(void* plan, int type, void* c, void* f,
                             int f_is_real, int64_t ntrans_run, void* stream);
 
+/* The sorted input of the type-1 direction: the plan's buffer of point values
+ * in bin order (*values, complex, row stride *ldc in elements, rows zero-padded
+ * past ntrans_run) and the map from point index to row (*spos, int32, M
+ * entries).  Valid after setpts until the next setpts.  No counterpart in the
+ * reference: it lets a producer of type-1 values (cov3d_ml_grad_sorted) write
+ * them where the spreading reads them, so cov3d_nufft_adjoint_presorted skips
+ * the permute of nufft_plan.py:366-392's input. */
+int cov3d_nufft_sorted_input(void* plan, int64_t ntrans_run, void** values,
+                             int64_t* ldc, void** spos);
+
+/* Type-1 direction (as cov3d_nufft_execute_dir with type 1) on values already
+ * written to the plan's sorted input. */
+int cov3d_nufft_adjoint_presorted(void* plan, void* f, int f_is_real,
+                                  int64_t ntrans_run, void* stream);
+
 /* Frees the plan's device buffers and rocFFT plans. */
 int cov3d_nufft_destroy(void* plan);
 
@@ -152,6 +167,17 @@ int cov3d_ml_forward(const void* c, const void* ctf, const void* yhat,
 int cov3d_ml_grad_images(const void* c, const void* ctf, const void* yhat,
                          const void* z, const void* minv, const int* pix_map,
                          int64_t npix, void* g, int64_t B, int64_t r, int64_t L,
+                         double noise_var, double grad_scale, void* stream);
+
+/* cov3d_ml_grad_images with the output written to a plan's sorted type-1
+ * input (cov3d_nufft_sorted_input): value t of pixel k of image i goes to
+ * g_sorted[spos[i * np + k] * ldc + t], zeros for r <= t < ldc (ldc even,
+ * r <= ldc <= 24).  The plan's points must be the images' np pixels in image
+ * order. */
+int cov3d_ml_grad_sorted(const void* c, const void* ctf, const void* yhat,
+                         const void* z, const void* minv, const int* pix_map,
+                         int64_t npix, void* g_sorted, const int* spos,
+                         int64_t ldc, int64_t B, int64_t r, int64_t L,
                          double noise_var, double grad_scale, void* stream);
 
 /* ------------------------------------------------------------------------

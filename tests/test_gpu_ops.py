@@ -135,6 +135,35 @@ def test_ml_grad_images_vs_closed_form(dev):
     assert rel_l2(g.cpu().numpy(), ref) < 1e-4
 
 
+@pytest.mark.parametrize("r", [6, 20])
+def test_ml_grad_sorted_matches_images_then_adjoint(dev, r):
+    """The gradient written straight into a plan's sorted type-1 input, then the presorted adjoint,
+    equals ml_grad_images followed by the plan's ordinary adjoint bit for bit (same arithmetic,
+    same rows, same spreading order); half-plane pixels, CTF, a full batch and r = 6 / 20."""
+    ml = pkg("ml_ops")
+    npl = pkg("nufft_plan")
+    dsm = pkg("dataset")
+    L, s2, B = 16, 0.4, 5
+    g = torch.Generator().manual_seed(53 + r)
+    pix = ml.half_plane_pixels(L, dev)
+    npx = pix.numel()
+    c = (torch.complex(torch.randn((r, B, npx), generator=g), torch.randn((r, B, npx), generator=g)) * 3).to(dev)
+    yhat = (torch.complex(torch.randn((B, L, L), generator=g), torch.randn((B, L, L), generator=g)) * 5).to(dev)
+    ctf = (torch.rand((B, L, L), generator=g) * 2 - 1).to(dev)
+    rots = dsm.haar_rotations(B, g).to(dev)
+    pts = dsm.pts_rot_from_rotations(rots, L)
+    plan = npl.NufftPlan((L,) * 3, batch_size=r, eps=1e-6, device=dev)
+    plan.set_points_flat(pts[:, :, pix.long()].transpose(0, 1).reshape(3, -1).contiguous(), B)
+    res = ml.ml_forward(c, yhat, L, s2, ctf=ctf, precision=1, pix_map=pix)
+    gimg = ml.ml_grad_images(c, yhat, res["z"], res["minv"], L, s2, 1.0 / B, ctf=ctf, pix_map=pix)
+    ref = plan.execute_adjoint(gimg.reshape(r, -1), real_output=True)
+    vals, ldc, spos = plan.sorted_input(r)
+    assert ldc % 2 == 0 and r <= ldc <= 24
+    ml.ml_grad_sorted(c, yhat, res["z"], res["minv"], L, s2, 1.0 / B, vals, spos, ldc, ctf=ctf, pix_map=pix)
+    out = plan.execute_adjoint_presorted(r, real_output=True)
+    assert torch.equal(out, ref)
+
+
 @pytest.mark.parametrize("L", [16, 15])
 def test_half_plane_equals_full_plane_for_hermitian_data(dev, L):
     """With exactly Hermitian projections, the half-plane Gram/gradient equal the full-plane ones."""
