@@ -793,18 +793,32 @@ __global__ void __launch_bounds__(kBlock)
       for (int k = 0; k < W * W; ++k) {
         const int a0 = k / W, a1 = k % W;
         if (k + 1 < W * W) load_row((k + 1) / W, (k + 1) % W, buf[(k + 1) & 1]);
-        const float w01 = w0[a0] * w1[a1];
+        // the row's W taps weighted along z first, then the row weight w0 w1 once
+        float rx[TC], ry[TC];
 #pragma unroll
-        for (int a2 = 0; a2 < W; ++a2) {
-          const float ww = w01 * w2[a2];
+        for (int h = 0; h < H; ++h) {
+          const float4 v = buf[k & 1][h];
+          rx[2 * h] = w2[0] * v.x;
+          ry[2 * h] = w2[0] * v.y;
+          rx[2 * h + 1] = w2[0] * v.z;
+          ry[2 * h + 1] = w2[0] * v.w;
+        }
+#pragma unroll
+        for (int a2 = 1; a2 < W; ++a2) {
 #pragma unroll
           for (int h = 0; h < H; ++h) {
             const float4 v = buf[k & 1][a2 * H + h];
-            ax[2 * h] += ww * v.x;
-            ay[2 * h] += ww * v.y;
-            ax[2 * h + 1] += ww * v.z;
-            ay[2 * h + 1] += ww * v.w;
+            rx[2 * h] += w2[a2] * v.x;
+            ry[2 * h] += w2[a2] * v.y;
+            rx[2 * h + 1] += w2[a2] * v.z;
+            ry[2 * h + 1] += w2[a2] * v.w;
           }
+        }
+        const float w01 = w0[a0] * w1[a1];
+#pragma unroll
+        for (int tt = 0; tt < TC; ++tt) {
+          ax[tt] += w01 * rx[tt];
+          ay[tt] += w01 * ry[tt];
         }
       }
     } else {
