@@ -126,7 +126,7 @@ __device__ void chol_solve(const double* A, double* x, int r) {
   }
 }
 
-template <int PREC>
+template <int PREC, int NR2 = 16>
 __global__ void __launch_bounds__(PREC == 1 ? kMlBlockMfma : kMlBlock)
     k_ml_forward(const float2* __restrict__ c, const float* __restrict__ ctf,
                  const float2* __restrict__ yhat, const float* __restrict__ lam_inv,
@@ -160,45 +160,41 @@ __global__ void __launch_bounds__(PREC == 1 ? kMlBlockMfma : kMlBlock)
     for (int e = (r + 1) * kMlTileLd + lane; e < kRmax * kMlTileLd; e += 64) t_hi[e] = t_lo[e] = 0u;
     const int p = lane & 31, rp = lane >> 5;  // load role: pixel p, rows rp, rp + 2, ...
     const int m = lane & 31, h = lane >> 5;   // MFMA role: row m, pixel quarter h
-    const int nrow2 = (r + 2) >> 1;           // row pairs covering rows 0..r
+    const int nrow2 = (r + 2) >> 1;           // row pairs covering rows 0..r (<= NR2)
     f32x16 acc;
 #pragma unroll
     for (int v = 0; v < 16; ++v) acc[v] = 0.f;
     const int64_t nchunk = (np + kMlChunk - 1) / kMlChunk;
-    for (int64_t ch = wave; ch < nchunk; ch += nw) {
+    // loads of one chunk: row rp + 2j of pixel p (the image row r comes from yhat), the pixel's
+    // scale; addresses are clamped so that every lane issues every load (a fixed count of
+    // loads in flight), and out-of-range values are zeroed when the chunk is stored
+    auto load = [&](int64_t ch, float2 (&v)[NR2], float& sc, float& sy) {
       const int64_t k = ch * kMlChunk + p;
-      float sc = 0.f, sy = 0.f;
-      int64_t px = 0;
-      if (k < np) {
-        float mult;
-        px = entry_pixel(pix_map, k, L, mult);
-        const float sm = mult == 1.f ? 1.f : 1.41421356237309515f;
-        sc = pixel_scale(ctf, i, px, L, inv_l2) * sm;
-        sy = inv_l * sm;
-      }
-      float2 v[16];
+      const int64_t kc = k < np ? k : np - 1;
+      float mult;
+      const int64_t px = entry_pixel(pix_map, kc, L, mult);
+      const float sm = mult == 1.f ? 1.f : 1.41421356237309515f;
+      sc = k < np ? pixel_scale(ctf, i, px, L, inv_l2) * sm : 0.f;
+      sy = k < np ? inv_l * sm : 0.f;
 #pragma unroll
-      for (int j = 0; j < 16; ++j) {
-        if (j < nrow2) {
-          const int mm = rp + 2 * j;
-          v[j] = make_float2(0.f, 0.f);
-          if (k < np && mm < r) v[j] = c[((int64_t)mm * B + i) * np + k];
-          else if (k < np && mm == r) v[j] = yhat[i * d + px];
-        }
+      for (int j = 0; j < NR2; ++j) {
+        const int mm = rp + 2 * j;
+        const float2* src = mm == r ? yhat + (i * d + px) : c + (((int64_t)(mm < r ? mm : 0) * B + i) * np + kc);
+        v[j] = *src;
       }
+    };
+    auto store_mma = [&](const float2 (&v)[NR2], float sc, float sy) {
 #pragma unroll
-      for (int j = 0; j < 16; ++j) {
-        if (j < nrow2) {
-          const int mm = rp + 2 * j;
-          if (mm <= r) {
-            const float s = mm == r ? sy : sc;
-            const float x = v[j].x * s, y = v[j].y * s;
-            const unsigned short hx = f32_to_bf16_bits(x), hy = f32_to_bf16_bits(y);
-            const unsigned short lx = f32_to_bf16_bits(x - bf16_bits_to_f32(hx));
-            const unsigned short ly = f32_to_bf16_bits(y - bf16_bits_to_f32(hy));
-            t_hi[mm * kMlTileLd + p] = (unsigned)hx | ((unsigned)hy << 16);
-            t_lo[mm * kMlTileLd + p] = (unsigned)lx | ((unsigned)ly << 16);
-          }
+      for (int j = 0; j < NR2; ++j) {
+        const int mm = rp + 2 * j;
+        if (j < nrow2 && mm <= r) {
+          const float sv = mm == r ? sy : sc;
+          const float x = v[j].x * sv, y = v[j].y * sv;
+          const unsigned short hx = f32_to_bf16_bits(x), hy = f32_to_bf16_bits(y);
+          const unsigned short lx = f32_to_bf16_bits(x - bf16_bits_to_f32(hx));
+          const unsigned short ly = f32_to_bf16_bits(y - bf16_bits_to_f32(hy));
+          t_hi[mm * kMlTileLd + p] = (unsigned)hx | ((unsigned)hy << 16);
+          t_lo[mm * kMlTileLd + p] = (unsigned)lx | ((unsigned)ly << 16);
         }
       }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -218,6 +214,22 @@ __global__ void __launch_bounds__(PREC == 1 ? kMlBlockMfma : kMlBlock)
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+    // two register sets: the loads of chunk ch + nw are in flight while chunk ch is stored and
+    // multiplied (the last prefetch repeats a valid chunk and is not used)
+    if (wave < nchunk) {
+      float2 va[NR2], vb[NR2];
+      float sca, sya, scb, syb;
+      load(wave, va, sca, sya);
+      for (int64_t ch = wave; ch < nchunk; ch += 2 * nw) {
+        const int64_t c1 = ch + nw < nchunk ? ch + nw : ch;
+        load(c1, vb, scb, syb);
+        store_mma(va, sca, sya);
+        if (ch + nw >= nchunk) break;
+        const int64_t c2 = ch + 2 * nw < nchunk ? ch + 2 * nw : ch + nw;
+        load(c2, va, sca, sya);
+        store_mma(vb, scb, syb);
+      }
     }
     __syncthreads();  // the tiles are dead: s_part aliases them
     // D layout of the 32x32 tile: lane l, register v -> row 8*(v/4) + 4*(l/32) + v%4, col l%32.
@@ -462,16 +474,27 @@ extern "C" int cov3d_ml_forward(const void* c, const void* ctf, const void* yhat
                   "npix=%lld invalid for L=%lld", (long long)npix, (long long)L);
   const int64_t np = pix_map ? npix : L * L;
   hipStream_t st = cov3d::as_stream(stream);
-  if (precision == 1)
-    hipLaunchKernelGGL((cov3d::k_ml_forward<1>), dim3((unsigned)B), dim3(cov3d::kMlBlockMfma), 0, st,
-                       (const float2*)c, (const float*)ctf, (const float2*)yhat,
-                       (const float*)lam_inv, pix_map, np, B, (int)r, (int)L, (float)noise_var, mean_const,
-                       (float*)cost, (float*)z, (float*)m_out, (float*)minv, (float*)b_out);
+  // split-BF16: register sets sized for the (r + 2) / 2 row pairs a lane loads
+#define COV3D_MLF(NR)                                                                                  \
+  hipLaunchKernelGGL((cov3d::k_ml_forward<1, NR>), dim3((unsigned)B), dim3(cov3d::kMlBlockMfma), 0, st, \
+                     (const float2*)c, (const float*)ctf, (const float2*)yhat, (const float*)lam_inv, pix_map, \
+                     np, B, (int)r, (int)L, (float)noise_var, mean_const, (float*)cost, (float*)z,          \
+                     (float*)m_out, (float*)minv, (float*)b_out)
+  const int64_t nr2 = (r + 2) / 2;
+  if (precision == 1 && nr2 <= 4)
+    COV3D_MLF(4);
+  else if (precision == 1 && nr2 <= 8)
+    COV3D_MLF(8);
+  else if (precision == 1 && nr2 <= 11)
+    COV3D_MLF(11);
+  else if (precision == 1)
+    COV3D_MLF(16);
   else
     hipLaunchKernelGGL((cov3d::k_ml_forward<0>), dim3((unsigned)B), dim3(cov3d::kMlBlock), 0, st,
                        (const float2*)c, (const float*)ctf, (const float2*)yhat,
                        (const float*)lam_inv, pix_map, np, B, (int)r, (int)L, (float)noise_var, mean_const,
                        (float*)cost, (float*)z, (float*)m_out, (float*)minv, (float*)b_out);
+#undef COV3D_MLF
   COV3D_LAUNCH_CHECK();
   return 0;
 }
