@@ -686,12 +686,15 @@ __device__ __forceinline__ void stage_tile(const Cplx<T>* __restrict__ fine, Cpl
 // and ds_read_b128 serves 16 lanes per clock from 64 banks (measured), so a read is conflict-free
 // when the 16 lanes' base cells differ mod 16.  Points are bucketed by base cell mod 16 and dealt
 // round-robin over the buckets: bucket b, rank r goes to slot sum_b' min(c_b', r) +
-// #{b' < b : c_b' > r}.  iord[p0 + slot] is the subproblem-local index of the point at that slot.
-// The order depends on the points only, so it is computed once per setpts.
+// #{b' < b : c_b' > r}.  The point at slot s is stored as a record irec[p0 + s] = (its three
+// grid coordinates, its sorted index as int bits) with its caller index iidx[p0 + s] = sidx[p]:
+// k_interp then reads its points with coalesced loads that depend on nothing, issued before the
+// tile is staged.  The order depends on the points only, so it is computed once per setpts.
 template <int W>
 __global__ void __launch_bounds__(kBlock)
     k_bank_order(SubDesc sd, const float* __restrict__ sv0, const float* __restrict__ sv1,
-                 const float* __restrict__ sv2, int* __restrict__ iord) {
+                 const float* __restrict__ sv2, const int* __restrict__ sidx, float4* __restrict__ irec,
+                 int* __restrict__ iidx) {
   const int sub = blockIdx.x;
   if (sub >= *sd.nsub) return;
   constexpr int E = kSuperEdge + W;
@@ -726,7 +729,9 @@ __global__ void __launch_bounds__(kBlock)
         const int cb = s_cnt[b];
         pos += min(cb, rr[k]) + ((b < rb[k] && cb > rr[k]) ? 1 : 0);
       }
-      iord[p0 + pos] = slot;
+      const int ps = p0 + slot;
+      irec[p0 + pos] = make_float4(sv0[ps], sv1[ps], sv2[ps], __int_as_float(ps));
+      iidx[p0 + pos] = sidx[ps];
     }
   }
 }
@@ -741,7 +746,7 @@ __global__ void __launch_bounds__(kBlock)
     k_interp(const Cplx<T>* __restrict__ fine, Cplx<T>* __restrict__ c, SubDesc sd, BinGeom g,
              const T* __restrict__ sv0, const T* __restrict__ sv1, const T* __restrict__ sv2,
              const int* __restrict__ sidx, int64_t M, int ntrans, T kc, T beta, int half_h2, int conj_all,
-             const int* __restrict__ iord, Cplx<T>* __restrict__ direct) {
+             const float4* __restrict__ irec, const int* __restrict__ iidx, Cplx<T>* __restrict__ direct) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Cplx<T>* tile = reinterpret_cast<Cplx<T>*>(smem_raw);
   const int sub = xcd_remap(blockIdx.x, gridDim.x);
@@ -756,18 +761,51 @@ __global__ void __launch_bounds__(kBlock)
   const int o0 = s0 * kSuperEdge - W / 2, o1 = s1 * kSuperEdge - W / 2, o2 = s2 * kSuperEdge - W / 2;
   const int p0 = sd.sub_pt[sub];
   const int np = sd.sub_n[sub];
-  // Points are visited in the bank-aware order computed at setpts (k_bank_order), when present.
+  // Points are visited in the bank-aware order of their records (k_bank_order), when present:
+  // a thread's (at most kMaxSub / kBlock) records are loaded before the tile is staged, so their
+  // latency overlaps the staging loads, and each loop iteration takes the next one.
+  constexpr int kPer = kMaxSub / kBlock;
+  float4 prec[kPer];
+  int pidx[kPer];
+  if (irec) {
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int slot = threadIdx.x + k * kBlock;
+      prec[k] = slot < np ? irec[p0 + slot] : make_float4(0.f, 0.f, 0.f, 0.f);
+      pidx[k] = slot < np ? iidx[p0 + slot] : 0;
+    }
+  }
   // halo cells lie within one period of the grid (nf >= 2W), so one conditional wrap suffices
   constexpr int PL = interp_plane<T, W, TC>();
   stage_tile<T, W, TC, PL>(fine, tile, g, o0, o1, o2, t0, ntrans, half_h2, conj_all);
   __syncthreads();
-  for (int slot = threadIdx.x; slot < np; slot += blockDim.x) {
-    const int p = p0 + (iord ? iord[p0 + slot] : slot);
+  for (int slot = threadIdx.x; slot < np; slot += kBlock) {
+    int p;
+    int64_t j;
+    T u0, u1, u2;
+    if (irec) {
+      u0 = (T)prec[0].x;
+      u1 = (T)prec[0].y;
+      u2 = (T)prec[0].z;
+      p = __float_as_int(prec[0].w);
+      j = pidx[0];
+#pragma unroll
+      for (int k = 0; k + 1 < kPer; ++k) {
+        prec[k] = prec[k + 1];
+        pidx[k] = pidx[k + 1];
+      }
+    } else {
+      p = p0 + slot;
+      u0 = sv0[p];
+      u1 = sv1[p];
+      u2 = sv2[p];
+      j = direct ? sidx[p] : 0;
+    }
     T w0[W], w1[W], w2[W];
     int l0, l1, l2;
-    stencil_weights<T, W>(sv0[p], kc, beta, l0, w0);
-    stencil_weights<T, W>(sv1[p], kc, beta, l1, w1);
-    stencil_weights<T, W>(sv2[p], kc, beta, l2, w2);
+    stencil_weights<T, W>(u0, kc, beta, l0, w0);
+    stencil_weights<T, W>(u1, kc, beta, l1, w1);
+    stencil_weights<T, W>(u2, kc, beta, l2, w2);
     l0 = min(max(l0, 0), kSuperEdge);
     l1 = min(max(l1, 0), kSuperEdge);
     l2 = min(max(l2, 0), kSuperEdge);
@@ -853,8 +891,7 @@ __global__ void __launch_bounds__(kBlock)
       }
     }
     if (direct) {
-      // the caller's (ntrans, M) layout at the point's original index
-      const int64_t j = sidx[p];
+      // the caller's (ntrans, M) layout at the point's original index j
 #pragma unroll
       for (int tt = 0; tt < TC; ++tt)
         if (t0 + tt < ntrans) direct[(int64_t)(t0 + tt) * M + j] = Cplx<T>{ax[tt], ay[tt]};
@@ -1751,8 +1788,9 @@ struct Plan {
   size_t sort_tmp_bytes = 0;
   int* sidx = nullptr;
   int* spos = nullptr;
-  int* iord = nullptr;        // bank-aware interpolation order (FP32, two transforms per cell, W <= 8)
-  bool iord_valid = false;
+  float4* irec = nullptr;     // interpolation point records in bank-aware order (FP32, two
+  int* iidx = nullptr;        // transforms per cell, W <= 8) and their caller indices
+  bool irec_valid = false;
   void* csort = nullptr;  // sorted-order staging of point values (M * ldc_cap complex)
   int ldc_cap = 0;        // row capacity of csort
   void* sv[3] = {nullptr, nullptr, nullptr};
@@ -1784,9 +1822,11 @@ static int free_points(Plan* p) {
   p->sort_tmp_bytes = 0;
   hipFree(p->sidx);
   hipFree(p->spos);
-  hipFree(p->iord);
-  p->iord = nullptr;
-  p->iord_valid = false;
+  hipFree(p->irec);
+  hipFree(p->iidx);
+  p->irec = nullptr;
+  p->iidx = nullptr;
+  p->irec_valid = false;
   hipFree(p->csort);
   for (int k = 0; k < 3; ++k) hipFree(p->sv[k]);
   hipFree(p->sub_sb);
@@ -1853,7 +1893,8 @@ static int launch_interp(Plan* p, void* c_out, int ntrans_run, hipStream_t st) {
       hipLaunchKernelGGL((k_interp<T, W, TC>), grid, dim3(kBlock), smem_k, st, (const Cplx<T>*)p->fine,
                          (Cplx<T>*)p->csort, sd, p->geom, (const T*)p->sv[0], (const T*)p->sv[1],
                          (const T*)p->sv[2], p->sidx, p->M, ntrans_run, kc, (T)p->beta, p->stage_half,
-                         p->stage_conj, (TC == 2 && p->iord_valid) ? (const int*)p->iord : nullptr,
+                         p->stage_conj, (TC == 2 && p->irec_valid) ? (const float4*)p->irec : nullptr,
+                         (TC == 2 && p->irec_valid) ? (const int*)p->iidx : nullptr,
                          direct ? (Cplx<T>*)c_out : nullptr);
   }
   COV3D_LAUNCH_CHECK();
@@ -2230,7 +2271,8 @@ static int setpts_impl(Plan* p, int64_t M, const T* x, const T* y, const T* z, h
     COV3D_HIP(hipMalloc(&p->pt_pos, Ma * sizeof(int)));
     COV3D_HIP(hipMalloc(&p->sidx, Ma * sizeof(int)));
     COV3D_HIP(hipMalloc(&p->spos, Ma * sizeof(int)));
-    COV3D_HIP(hipMalloc(&p->iord, Ma * sizeof(int)));
+    COV3D_HIP(hipMalloc(&p->irec, Ma * sizeof(float4)));
+    COV3D_HIP(hipMalloc(&p->iidx, Ma * sizeof(int)));
     // interpolation writes whole chunks of up to 4 transforms (ntrans rounded up to 4); the
     // spreading input uses row stride gather_ldc(n) for any n <= ntrans.  The matrix-core
     // spreading reads up to 7 values past a row's chunk: a zeroed tail keeps the last row's
@@ -2308,7 +2350,7 @@ static int setpts_impl(Plan* p, int64_t M, const T* x, const T* y, const T* z, h
                        (T*)p->sv[0], (T*)p->sv[1], (T*)p->sv[2]);
     COV3D_LAUNCH_CHECK();
   }
-  p->iord_valid = false;
+  p->irec_valid = false;
   if constexpr (sizeof(T) == 4) {
     if (M > 0 && p->tc == 2 && p->w <= 8) {
       SubDesc sd{p->sub_sb, p->sub_pt, p->sub_n, p->nsub};
@@ -2319,14 +2361,15 @@ static int setpts_impl(Plan* p, int64_t M, const T* x, const T* y, const T* z, h
       switch (p->w) {
 #define COV3D_W(WW)                                                                                      \
   case WW:                                                                                               \
-    hipLaunchKernelGGL((k_bank_order<WW>), gsub, dim3(kBlock), 0, st, sd, s0, s1, s2, p->iord); \
+    hipLaunchKernelGGL((k_bank_order<WW>), gsub, dim3(kBlock), 0, st, sd, s0, s1, s2, p->sidx, p->irec, \
+                       p->iidx);                                                                          \
     break;
         COV3D_W(2) COV3D_W(3) COV3D_W(4) COV3D_W(5) COV3D_W(6) COV3D_W(7) COV3D_W(8)
 #undef COV3D_W
         default: break;
       }
       COV3D_LAUNCH_CHECK();
-      p->iord_valid = true;
+      p->irec_valid = true;
     }
   }
   p->pts_set = true;
