@@ -612,6 +612,8 @@ __device__ __forceinline__ int wrap1(int m, int n) {
 // registers before any is stored, so a thread has up to kStageGroup * TC
 // global loads in flight instead of one iteration's.
 constexpr int kStageGroup = 7;
+// k_interp, whose occupancy is bounded by its tile, issues a whole W = 7 tile (14 iterations) at once
+constexpr int kInterpStageGroup = 14;
 
 // half_h2 > 0: the fine grid holds only the Hermitian half m2 < half_h2 (= nf2/2 + 1) of the
 // transform of a real volume; a cell m with m2 >= half_h2 is conj(F[-m]).  conj_all conjugates
@@ -627,7 +629,7 @@ constexpr int interp_plane() {
   return (sizeof(T) == 4 && TC == 2 && W == 7) ? 234 : (kSuperEdge + W) * (kSuperEdge + W);
 }
 
-template <typename T, int W, int TC, int PL = (kSuperEdge + W) * (kSuperEdge + W)>
+template <typename T, int W, int TC, int PL = (kSuperEdge + W) * (kSuperEdge + W), int SG = kStageGroup>
 __device__ __forceinline__ void stage_tile(const Cplx<T>* __restrict__ fine, Cplx<T>* tile, const BinGeom& g,
                                            int o0, int o1, int o2, int t0, int ntrans, int half_h2 = 0,
                                            int conj_all = 0) {
@@ -637,10 +639,10 @@ __device__ __forceinline__ void stage_tile(const Cplx<T>* __restrict__ fine, Cpl
   constexpr int NIT = (ncell + kBlock - 1) / kBlock;
   const int64_t nfine = half_h2 ? (int64_t)g.nf[0] * g.nf[1] * half_h2 : (int64_t)g.nf[0] * g.nf[1] * g.nf[2];
 #pragma unroll
-  for (int it0 = 0; it0 < NIT; it0 += kStageGroup) {
-    Cplx<T> v[kStageGroup][TC];
+  for (int it0 = 0; it0 < NIT; it0 += SG) {
+    Cplx<T> v[SG][TC];
 #pragma unroll
-    for (int k = 0; k < kStageGroup; ++k) {
+    for (int k = 0; k < SG; ++k) {
       const int cidx = threadIdx.x + (it0 + k) * kBlock;
       if (it0 + k < NIT && cidx < ncell) {
         const int e2 = cidx % E;
@@ -670,7 +672,7 @@ __device__ __forceinline__ void stage_tile(const Cplx<T>* __restrict__ fine, Cpl
       }
     }
 #pragma unroll
-    for (int k = 0; k < kStageGroup; ++k) {
+    for (int k = 0; k < SG; ++k) {
       const int cidx = threadIdx.x + (it0 + k) * kBlock;
       if (it0 + k < NIT && cidx < ncell) {
         const int tidx = cidx + (cidx / (E * E)) * (PL - E * E);
@@ -777,7 +779,7 @@ __global__ void __launch_bounds__(kBlock)
   }
   // halo cells lie within one period of the grid (nf >= 2W), so one conditional wrap suffices
   constexpr int PL = interp_plane<T, W, TC>();
-  stage_tile<T, W, TC, PL>(fine, tile, g, o0, o1, o2, t0, ntrans, half_h2, conj_all);
+  stage_tile<T, W, TC, PL, kInterpStageGroup>(fine, tile, g, o0, o1, o2, t0, ntrans, half_h2, conj_all);
   __syncthreads();
   for (int slot = threadIdx.x; slot < np; slot += kBlock) {
     int p;
