@@ -629,6 +629,22 @@ constexpr int interp_plane() {
   return (sizeof(T) == 4 && TC == 2 && W == 7) ? 234 : (kSuperEdge + W) * (kSuperEdge + W);
 }
 
+// Fine-grid offset of grid cell (m0, m1, m2) (each in [0, nf)); neg: the value is the conjugate
+// of the stored one (the Hermitian half grid stores m2 < half_h2 only; conj_all flips every value).
+__device__ __forceinline__ int64_t fine_cell(const BinGeom& g, int m0, int m1, int m2, int half_h2, int conj_all,
+                                            bool& neg) {
+  neg = false;
+  if (!half_h2) return ((int64_t)m0 * g.nf[1] + m1) * g.nf[2] + m2;
+  if (m2 >= half_h2) {
+    m0 = m0 ? g.nf[0] - m0 : 0;
+    m1 = m1 ? g.nf[1] - m1 : 0;
+    m2 = g.nf[2] - m2;
+    neg = true;
+  }
+  neg = neg != (conj_all != 0);
+  return ((int64_t)m0 * g.nf[1] + m1) * half_h2 + m2;
+}
+
 template <typename T, int W, int TC, int PL = (kSuperEdge + W) * (kSuperEdge + W), int SG = kStageGroup>
 __device__ __forceinline__ void stage_tile(const Cplx<T>* __restrict__ fine, Cplx<T>* tile, const BinGeom& g,
                                            int o0, int o1, int o2, int t0, int ntrans, int half_h2 = 0,
@@ -648,21 +664,9 @@ __device__ __forceinline__ void stage_tile(const Cplx<T>* __restrict__ fine, Cpl
         const int e2 = cidx % E;
         const int e1 = (cidx / E) % E;
         const int e0 = cidx / (E * E);
-        int m0 = wrap1(o0 + e0, g.nf[0]), m1 = wrap1(o1 + e1, g.nf[1]), m2 = wrap1(o2 + e2, g.nf[2]);
-        int64_t off;
-        bool neg = false;
-        if (half_h2) {
-          if (m2 >= half_h2) {
-            m0 = m0 ? g.nf[0] - m0 : 0;
-            m1 = m1 ? g.nf[1] - m1 : 0;
-            m2 = g.nf[2] - m2;
-            neg = true;
-          }
-          neg = neg != (conj_all != 0);
-          off = ((int64_t)m0 * g.nf[1] + m1) * half_h2 + m2;
-        } else {
-          off = ((int64_t)m0 * g.nf[1] + m1) * g.nf[2] + m2;
-        }
+        bool neg;
+        const int64_t off = fine_cell(g, wrap1(o0 + e0, g.nf[0]), wrap1(o1 + e1, g.nf[1]), wrap1(o2 + e2, g.nf[2]),
+                                      half_h2, conj_all, neg);
 #pragma unroll
         for (int tt = 0; tt < TC; ++tt) {
           Cplx<T> x = t0 + tt < ntrans ? fine[(t0 + tt) * nfine + off] : Cplx<T>{(T)0, (T)0};
@@ -734,6 +738,59 @@ __global__ void __launch_bounds__(kBlock)
       const int ps = p0 + slot;
       irec[p0 + pos] = make_float4(sv0[ps], sv1[ps], sv2[ps], __int_as_float(ps));
       iidx[p0 + pos] = sidx[ps];
+    }
+  }
+}
+
+// Taps of one point on a staged FP32 tile of TC/2 16-byte words per cell (TC even, W <= 8):
+// ax/ay[tt] += sum over the W^3 stencil of w0 w1 w2 * tile value of transform tt.  Stencil rows
+// (a0, a1) go in order, double-buffered: the W cells of the next row are read from LDS while the
+// FMAs of the current row run (occupancy is bounded by the tile, so registers are spent on reads
+// in flight).  Each row's W taps are weighted along z first, then by the row weight w0 w1 once.
+template <int W, int TC, int PL>
+__device__ __forceinline__ void interp_taps_f32(const float4* __restrict__ t4, int l0, int l1, int l2,
+                                                const float* w0, const float* w1, const float* w2, float* ax,
+                                                float* ay) {
+  constexpr int E = kSuperEdge + W;
+  constexpr int H = TC / 2;
+  float4 buf[2][W * H];
+  auto load_row = [&](int a0, int a1, float4* b) {
+    const int row = (l0 + a0) * PL + (l1 + a1) * E + l2;
+#pragma unroll
+    for (int a2 = 0; a2 < W; ++a2)
+#pragma unroll
+      for (int h = 0; h < H; ++h) b[a2 * H + h] = t4[(row + a2) * H + h];
+  };
+  load_row(0, 0, buf[0]);
+#pragma unroll
+  for (int k = 0; k < W * W; ++k) {
+    const int a0 = k / W, a1 = k % W;
+    if (k + 1 < W * W) load_row((k + 1) / W, (k + 1) % W, buf[(k + 1) & 1]);
+    float rx[TC], ry[TC];
+#pragma unroll
+    for (int h = 0; h < H; ++h) {
+      const float4 v = buf[k & 1][h];
+      rx[2 * h] = w2[0] * v.x;
+      ry[2 * h] = w2[0] * v.y;
+      rx[2 * h + 1] = w2[0] * v.z;
+      ry[2 * h + 1] = w2[0] * v.w;
+    }
+#pragma unroll
+    for (int a2 = 1; a2 < W; ++a2) {
+#pragma unroll
+      for (int h = 0; h < H; ++h) {
+        const float4 v = buf[k & 1][a2 * H + h];
+        rx[2 * h] += w2[a2] * v.x;
+        ry[2 * h] += w2[a2] * v.y;
+        rx[2 * h + 1] += w2[a2] * v.z;
+        ry[2 * h + 1] += w2[a2] * v.w;
+      }
+    }
+    const float w01 = w0[a0] * w1[a1];
+#pragma unroll
+    for (int tt = 0; tt < TC; ++tt) {
+      ax[tt] += w01 * rx[tt];
+      ay[tt] += w01 * ry[tt];
     }
   }
 }
@@ -815,52 +872,7 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
     for (int tt = 0; tt < TC; ++tt) ax[tt] = ay[tt] = (T)0;
     if constexpr (TC % 2 == 0 && sizeof(T) == 4 && W <= 8) {
-      // Stencil rows (a0, a1) in order, double-buffered: the W cells of the next row are
-      // read from LDS while the FMAs of the current row run (occupancy is bounded by the
-      // tile, so registers are spent on reads in flight).
-      constexpr int H = TC / 2;
-      const float4* t4 = reinterpret_cast<const float4*>(tile);
-      float4 buf[2][W * H];
-      auto load_row = [&](int a0, int a1, float4* b) {
-        const int row = (l0 + a0) * PL + (l1 + a1) * E + l2;
-#pragma unroll
-        for (int a2 = 0; a2 < W; ++a2)
-#pragma unroll
-          for (int h = 0; h < H; ++h) b[a2 * H + h] = t4[(row + a2) * H + h];
-      };
-      load_row(0, 0, buf[0]);
-#pragma unroll
-      for (int k = 0; k < W * W; ++k) {
-        const int a0 = k / W, a1 = k % W;
-        if (k + 1 < W * W) load_row((k + 1) / W, (k + 1) % W, buf[(k + 1) & 1]);
-        // the row's W taps weighted along z first, then the row weight w0 w1 once
-        float rx[TC], ry[TC];
-#pragma unroll
-        for (int h = 0; h < H; ++h) {
-          const float4 v = buf[k & 1][h];
-          rx[2 * h] = w2[0] * v.x;
-          ry[2 * h] = w2[0] * v.y;
-          rx[2 * h + 1] = w2[0] * v.z;
-          ry[2 * h + 1] = w2[0] * v.w;
-        }
-#pragma unroll
-        for (int a2 = 1; a2 < W; ++a2) {
-#pragma unroll
-          for (int h = 0; h < H; ++h) {
-            const float4 v = buf[k & 1][a2 * H + h];
-            rx[2 * h] += w2[a2] * v.x;
-            ry[2 * h] += w2[a2] * v.y;
-            rx[2 * h + 1] += w2[a2] * v.z;
-            ry[2 * h + 1] += w2[a2] * v.w;
-          }
-        }
-        const float w01 = w0[a0] * w1[a1];
-#pragma unroll
-        for (int tt = 0; tt < TC; ++tt) {
-          ax[tt] += w01 * rx[tt];
-          ay[tt] += w01 * ry[tt];
-        }
-      }
+      interp_taps_f32<W, TC, PL>(reinterpret_cast<const float4*>(tile), l0, l1, l2, w0, w1, w2, ax, ay);
     } else {
       for (int a0 = 0; a0 < W; ++a0) {
 #pragma unroll
@@ -903,6 +915,138 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
       for (int tt = 0; tt < TC; ++tt) dst[tt] = Cplx<T>{ax[tt], ay[tt]};
     }
+  }
+}
+
+// Type-2 interpolation, persistent and software-pipelined (FP32, two transforms per 16-byte cell,
+// W <= 8, point records from k_bank_order).  Same arithmetic and visiting order as k_interp; the
+// grid holds as many workgroups as are resident at once (the tile bounds occupancy to two per CU),
+// a multiple of 8.  The items (transform pair, subproblem), item = pair * nsub + sub, are dealt
+// to the XCDs as xcd_remap deals subproblems: runs of kSubChunk consecutive items, run q to XCD
+// q % 8.  Workgroup b runs on XCD b % 8 and takes that XCD's items in order from the XCD's
+// counter ctr[b % 8] (zero at launch), one item ahead: while the taps of one item run, the tile
+// and point records of the next are loaded into registers, so an item's staging latency hides
+// behind the previous item's arithmetic, and the counter balances the uneven subproblems.
+template <int W>
+__global__ void __launch_bounds__(kBlock)
+    k_interp_pipe(const Cplx<float>* __restrict__ fine, Cplx<float>* __restrict__ c, SubDesc sd, BinGeom g,
+                  const float4* __restrict__ irec, const int* __restrict__ iidx, int64_t M, int ntrans, float kc,
+                  float beta, int half_h2, int conj_all, Cplx<float>* __restrict__ direct, int* __restrict__ ctr) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  float4* t4 = reinterpret_cast<float4*>(smem_raw);
+  constexpr int E = kSuperEdge + W;
+  constexpr int ncell = E * E * E;
+  constexpr int NIT = (ncell + kBlock - 1) / kBlock;
+  constexpr int PL = interp_plane<float, W, 2>();
+  constexpr int kPer = kMaxSub / kBlock;
+  const int nsub = *sd.nsub;
+  const int64_t nitems = (int64_t)nsub * ((ntrans + 1) / 2);
+  const int xcd = blockIdx.x & 7;
+  const int64_t nfine = half_h2 ? (int64_t)g.nf[0] * g.nf[1] * half_h2 : (int64_t)g.nf[0] * g.nf[1] * g.nf[2];
+  // the XCD's v-th item; increasing in v, so the first one past the end ends the walk
+  auto item_of = [&](int64_t v) { return ((v / kSubChunk) * 8 + xcd) * kSubChunk + v % kSubChunk; };
+  __shared__ int s_v[2];
+  __shared__ float4 s_rec[kPer][kBlock];
+  __shared__ int s_idx[kPer][kBlock];
+  if (threadIdx.x == 0) {
+    s_v[0] = atomicAdd(&ctr[xcd], 1);
+    s_v[1] = atomicAdd(&ctr[xcd], 1);
+  }
+  __syncthreads();
+  int64_t item = item_of(s_v[0]);
+  if (item >= nitems) return;
+  float4 tv[NIT];  // next tile: cell threadIdx.x + k kBlock, transforms t0 and t0 + 1
+  float4 nrec[kPer];
+  int nidx[kPer];
+  // records first: waiting for the tile's loads then covers them too
+  auto fetch = [&](int64_t it) {
+    const int sub = (int)(it % nsub);
+    const int t0 = (int)(it / nsub) * 2;
+    const int p0 = sd.sub_pt[sub];
+    const int np = sd.sub_n[sub];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int slot = threadIdx.x + k * kBlock;
+      nrec[k] = slot < np ? irec[p0 + slot] : make_float4(0.f, 0.f, 0.f, 0.f);
+      nidx[k] = slot < np ? iidx[p0 + slot] : 0;
+    }
+    const int sb = sd.sub_sb[sub];
+    const int s2 = sb % g.nsb[2];
+    const int s1 = (sb / g.nsb[2]) % g.nsb[1];
+    const int s0 = sb / (g.nsb[2] * g.nsb[1]);
+    const int o0 = s0 * kSuperEdge - W / 2, o1 = s1 * kSuperEdge - W / 2, o2 = s2 * kSuperEdge - W / 2;
+    const bool two = t0 + 1 < ntrans;
+#pragma unroll
+    for (int k = 0; k < NIT; ++k) {
+      const int cidx = threadIdx.x + k * kBlock;
+      if (cidx < ncell) {
+        const int e2 = cidx % E;
+        const int e1 = (cidx / E) % E;
+        const int e0 = cidx / (E * E);
+        bool neg;
+        const int64_t off = fine_cell(g, wrap1(o0 + e0, g.nf[0]), wrap1(o1 + e1, g.nf[1]), wrap1(o2 + e2, g.nf[2]),
+                                      half_h2, conj_all, neg);
+        const Cplx<float> x0 = fine[(int64_t)t0 * nfine + off];
+        const Cplx<float> x1 = two ? fine[(int64_t)(t0 + 1) * nfine + off] : Cplx<float>{0.f, 0.f};
+        tv[k] = neg ? make_float4(x0.x, -x0.y, x1.x, -x1.y) : make_float4(x0.x, x0.y, x1.x, x1.y);
+      }
+    }
+  };
+  fetch(item);
+  while (true) {
+    const int sub = (int)(item % nsub);
+    const int pair = (int)(item / nsub);
+    const int t0 = pair * 2;
+    const int np = sd.sub_n[sub];
+    __syncthreads();  // the previous item's taps are done with the tile
+#pragma unroll
+    for (int k = 0; k < NIT; ++k) {
+      const int cidx = threadIdx.x + k * kBlock;
+      if (cidx < ncell) t4[cidx + (cidx / (E * E)) * (PL - E * E)] = tv[k];
+    }
+    // the current item's records move to LDS (each thread reads back only its own slots)
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      s_rec[k][threadIdx.x] = nrec[k];
+      s_idx[k][threadIdx.x] = nidx[k];
+    }
+    // s_v[1] (the next item) was written before the barrier above and is rewritten only after
+    // the barrier below
+    const int64_t nxt = item_of(s_v[1]);
+    __syncthreads();
+    // the item after next: its index is needed only at the next iteration's fetch, so the
+    // atomic's latency hides behind this item's taps
+    int v2 = 0;
+    if (threadIdx.x == 0 && nxt < nitems) v2 = atomicAdd(&ctr[xcd], 1);
+    if (nxt < nitems) fetch(nxt);
+    for (int slot = threadIdx.x, k = 0; slot < np; slot += kBlock, ++k) {
+      const float4 rec = s_rec[k][threadIdx.x];
+      const float u0 = rec.x, u1 = rec.y, u2 = rec.z;
+      const int p = __float_as_int(rec.w);
+      const int64_t j = s_idx[k][threadIdx.x];
+      float w0[W], w1[W], w2[W];
+      int l0, l1, l2;
+      stencil_weights<float, W>(u0, kc, beta, l0, w0);
+      stencil_weights<float, W>(u1, kc, beta, l1, w1);
+      stencil_weights<float, W>(u2, kc, beta, l2, w2);
+      l0 = min(max(l0, 0), kSuperEdge);
+      l1 = min(max(l1, 0), kSuperEdge);
+      l2 = min(max(l2, 0), kSuperEdge);
+      float ax[2] = {0.f, 0.f}, ay[2] = {0.f, 0.f};
+      interp_taps_f32<W, 2, PL>(t4, l0, l1, l2, w0, w1, w2, ax, ay);
+      if (direct) {
+#pragma unroll
+        for (int tt = 0; tt < 2; ++tt)
+          if (t0 + tt < ntrans) direct[(int64_t)(t0 + tt) * M + j] = Cplx<float>{ax[tt], ay[tt]};
+      } else {
+        Cplx<float>* dst = c + ((int64_t)pair * M + p) * 2;
+        dst[0] = Cplx<float>{ax[0], ay[0]};
+        dst[1] = Cplx<float>{ax[1], ay[1]};
+      }
+    }
+    if (nxt >= nitems) break;
+    if (threadIdx.x == 0) s_v[1] = v2;
+    item = nxt;
   }
 }
 
@@ -1766,6 +1910,9 @@ struct Plan {
   int spread_mode = 0;  // 0 = LDS-staged gather, 1 = LDS atomic tiles, 2/3 = LDS-free gather (FP32; 3 with L2 touch), 4 = FP32 MFMA gather
   int interp_mode = 0;  // 0 = thread per point, 1 = wavefront per point, 2 = eight lanes per point
   int interp_direct = 1;  // mode 0 writes the (ntrans, M) output directly instead of sorted + permute
+  int interp_pipe = 1;    // mode 0, FP32 with point records: k_interp_pipe instead of k_interp
+  int pipe_grid = 0;      // k_interp_pipe's grid: resident workgroups on the device, a multiple of 8
+  int* pipe_ctr = nullptr;  // k_interp_pipe's per-XCD item counters (8 ints)
   bool presorted = false;  // the current type-1 call's values are already in csort (no permute)
   int max_tc_gather = 20;  // transforms per gather workgroup
   int tc_spread = 2;    // transforms per spreading workgroup (tile mode)
@@ -1883,6 +2030,30 @@ static int launch_interp(Plan* p, void* c_out, int ntrans_run, hipStream_t st) {
                            (Cplx<float>*)p->csort, sd, p->geom, (const float*)p->sv[0], (const float*)p->sv[1],
                            (const float*)p->sv[2], p->M, ntrans_run, (float)kc, (float)p->beta, p->stage_half,
                            p->stage_conj);
+        done = true;
+      }
+    }
+    if constexpr (sizeof(T) == 4 && W <= 8 && TC == 2) {
+      if (!done && p->interp_mode == 0 && p->interp_pipe && p->irec_valid) {
+        if (p->pipe_grid == 0) {
+          COV3D_HIP(hipFuncSetAttribute((const void*)k_interp_pipe<W>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)smem_k));
+          int dev = 0, ncu = 0, per_cu = 0;
+          COV3D_HIP(hipGetDevice(&dev));
+          COV3D_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+          COV3D_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_interp_pipe<W>, kBlock,
+                                                                 smem_k));
+          p->pipe_grid = std::max(8, (ncu * std::max(per_cu, 1)) / 8 * 8);
+          COV3D_HIP(hipMalloc(&p->pipe_ctr, 8 * sizeof(int)));
+        }
+        // the XCD counters start at zero (a kernel, not a memset node, so that the launch
+        // sequence stays capturable)
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, p->pipe_ctr, (int64_t)8, 0);
+        hipLaunchKernelGGL((k_interp_pipe<W>), dim3((unsigned)p->pipe_grid), dim3(kBlock), smem_k, st,
+                           (const Cplx<float>*)p->fine, (Cplx<float>*)p->csort, sd, p->geom,
+                           (const float4*)p->irec, (const int*)p->iidx, p->M, ntrans_run, (float)kc,
+                           (float)p->beta, p->stage_half, p->stage_conj,
+                           direct ? (Cplx<float>*)c_out : nullptr, p->pipe_ctr);
         done = true;
       }
     }
@@ -2382,6 +2553,7 @@ static int destroy_plan(Plan* p) {
   if (!p) return 0;
   free_points(p);
   hipFree(p->fine);
+  hipFree(p->pipe_ctr);
   for (int k = 0; k < 3; ++k) hipFree(p->invp[k]);
   hipFree(p->bin_count);
   hipFree(p->bin_start);
@@ -2432,6 +2604,8 @@ static int make_plan(int type, const int64_t* n, int iflag, int64_t ntrans, doub
     p->interp_mode = (im == 1 || im == 2) ? im : 0;
     // measured at L=128, r=20, 1024 images: interp +0.5 ms, the 0.9 ms permute pass saved
     p->interp_direct = env_int("COV3D_INTERP_DIRECT", 1) != 0 ? 1 : 0;
+    // thread-per-point FP32 plans with point records run the persistent pipelined kernel
+    p->interp_pipe = env_int("COV3D_INTERP_PIPE", 1) != 0 ? 1 : 0;
   }
   p->max_tc_gather = std::min(std::max(env_int("COV3D_GATHER_TC", 20), 1), 20);
   p->ldc_cap = (int)ceil_div(ntrans, 4) * 4;
