@@ -135,6 +135,46 @@ class CovarDataset:
             self.noise_var /= self.resolution**2
             self._in_spatial_domain = True
 
+    def preprocess(self, mean_volume: torch.Tensor, mask: Optional[torch.Tensor] = None,
+                   batch_size: int = 1024, softening_radius: float = 5.0) -> None:
+        """Removes the translations and the projected mean from every image, and applies the
+        soft projected mask when ``mask`` is given (dataset.py:232-281, preprocess_from_modules).
+
+        Per batch, on the device: the images' spectra times exp(-i (g_x o_x + g_y o_y)) for the
+        stored offsets o (the pose module negates them, poses.py:263-264), minus the CTF-weighted
+        projection of ``mean_volume`` (1, L, L, L); with a mask, the residual image times the
+        projected mask thresholded at ``get_mask_threshold`` (computed on the first batch's
+        poses) and blurred by the soft-edged disc of radius ``softening_radius``.  Offsets are
+        zero afterwards.  The images stay in their current domain.
+        """
+        from .nufft_plan import NufftPlan
+        from .poses import get_phase_shift_grid, offset_to_phase_shift
+        from .projection_funcs import get_mask_threshold, preprocess_image_batch
+        from .utils import soft_edged_kernel
+
+        L = self.resolution
+        dev = self.images.device
+        if mean_volume.ndim == 3:
+            mean_volume = mean_volume.unsqueeze(0)
+        if mask is not None and mask.ndim == 3:
+            mask = mask.unsqueeze(0)
+        plan = NufftPlan((L,) * 3, batch_size=1, device=dev)
+        grid = get_phase_shift_grid(L, dtype=self.dtype, device=dev)
+        soft = soft_edged_kernel(radius=softening_radius, L=L, dim=2, in_fourier=True).to(dev)
+        soft = soft.to(torch.complex64 if self.dtype == torch.float32 else torch.complex128)
+        with torch.no_grad():
+            thr = 0.0
+            if mask is not None:
+                plan.setpts(self[slice(0, min(batch_size, len(self)))][1])
+                thr = get_mask_threshold(mask, plan)
+            for i in range(0, len(self), batch_size):
+                sl = slice(i, min(i + batch_size, len(self)))
+                images, pts, filters, _ = self[sl]
+                phase = offset_to_phase_shift(-self.offsets[sl].to(self.dtype), phase_shift_grid=grid)
+                self.images[sl] = preprocess_image_batch(images, plan, filters, (pts, phase), mean_volume, mask,
+                                                         thr, soft, fourier_domain=not self._in_spatial_domain)
+        self.offsets.zero_()
+
     def _subset(self, idx: torch.Tensor) -> "CovarDataset":
         sub = CovarDataset(
             self.images[idx], self.rotations[idx], self.noise_var,

@@ -218,11 +218,26 @@ def preprocess_image_batch(
         images = centered_ifft2(images).real
         with torch.no_grad():
             mproj = vol_forward(mask, nufft_plan, filters=None, fourier_domain=False).squeeze(1)
-            mproj = mproj > mask_threshold
-            soft = centered_ifft2(centered_fft2(mproj) * softening_kernel_fourier).real
+            mproj = (mproj > mask_threshold).to(images.dtype)
+            soft = centered_ifft2(centered_fft2(mproj) * softening_kernel_fourier.to(mproj.device)).real.to(images.dtype)
         images = images * soft
         if fourier_domain:
             images = centered_fft2(images)
     elif not fourier_domain:
         images = centered_ifft2(images).real
     return images
+
+
+def get_mask_threshold(mask: torch.Tensor, nufft_plan: BaseNufftPlan) -> float:
+    """10th percentile (linear interpolation, as numpy's default) of the projected mask values
+    above 10^-1.5 (projection_funcs.py:255-269); computed on the device."""
+    vals = vol_forward(mask, nufft_plan).reshape(-1)
+    vals = vals[vals > 10 ** (-1.5)]
+    if vals.numel() == 0:
+        raise ValueError("the projected mask has no value above 10^-1.5")
+    s = torch.sort(vals.double()).values
+    pos = 0.1 * (s.numel() - 1)
+    lo = int(math.floor(pos))
+    hi = min(lo + 1, s.numel() - 1)
+    return float(s[lo] + (s[hi] - s[lo]) * (pos - lo))
+
