@@ -386,24 +386,33 @@ def trainCovar(covar_model, dataset, batch_size, optimize_pose=False, mean_model
 
 
 def update_fourier_reg(trainer1: CovarTrainer, trainer2: CovarTrainer) -> None:
-    """Half-set shell regulariser from the covariance FSC (covar_sgd.py:1121-1187)."""
+    """Half-set shell regulariser from the covariance FSC (covar_sgd.py:1121-1156).  The gain is
+    the mean of the two half-sets' shell x shell covariance gains (the trainer's filter_gain,
+    covar_sgd.py:175-181, dataset.py:358-421)."""
     L = trainer1.covar.resolution
-    gain = (trainer1.dataset.filter_gain() + trainer2.dataset.filter_gain()) / 2
+    gain = (trainer1.dataset.get_total_covar_gain() + trainer2.dataset.get_total_covar_gain()) / 2
     e1, l1 = trainer1.covar.eigenvecs
     e2, l2 = trainer2.covar.eigenvecs
     reg, fsc = compute_updated_fourier_reg(e1 * (l1**0.5).reshape(-1, 1, 1, 1), e2 * (l2**0.5).reshape(-1, 1, 1, 1),
-                                           gain, L, trainer1.optimize_in_fourier_domain)
+                                           gain, getattr(trainer1, "fourier_reg", None), L,
+                                           trainer1.optimize_in_fourier_domain, mask=trainer1.dataset.mask)
     trainer1.update_fourier_reg_halfsets(reg)
     trainer2.update_fourier_reg_halfsets(reg)
     trainer1.training_log["covariance_fsc_halfset"] = fsc
 
 
-def compute_updated_fourier_reg(eigenvecs1, eigenvecs2, filter_gain, L, optimize_in_fourier_domain, mask=None):
-    """reg = 1 / ((FSC / (1 - FSC)) * shell gain^-1), rank-1 shell approximation (covar_sgd.py:1159-1187)."""
+def compute_updated_fourier_reg(eigenvecs1, eigenvecs2, filter_gain, current_fourier_reg, L,
+                                optimize_in_fourier_domain, mask=None):
+    """reg = 1 / ((FSC / (1 - FSC)) * shell gain^-1), rank-1 shell approximation (covar_sgd.py:1159-1187).
+
+    ``filter_gain`` is the (S, S) shell x shell covariance gain; its inverse is held constant
+    beyond shell L // 2 (both axes).  ``current_fourier_reg`` is accepted for the reference's
+    signature and, as there, does not enter the result."""
     corr = 1 / filter_gain
     corr[:, L // 2:] = corr[L // 2, L // 2]
     corr[L // 2:, :] = corr[L // 2, L // 2]
     if mask is not None:
+        mask = mask.to(eigenvecs1.device)
         eigenvecs1 = eigenvecs1 * mask
         eigenvecs2 = eigenvecs2 * mask
     fsc = covar_fsc(eigenvecs1, eigenvecs2)

@@ -175,6 +175,64 @@ class CovarDataset:
                                                          thr, soft, fourier_domain=not self._in_spatial_domain)
         self.offsets.zero_()
 
+    def get_total_gain(self, batch_size: int = 4096) -> torch.Tensor:
+        """diag(sum_i P_i^T P_i) on the (L,)^3 grid (dataset.py:322-356): the nearest-neighbour
+        (upsampling 1) back-projection of CTF^2 with the Nyquist row and column masked, summed
+        over the stack, |.|, divided by L^2."""
+        from .nufft_plan import NufftPlanDiscretized
+        from .projection_funcs import im_backward
+
+        L = self.resolution
+        plan = NufftPlanDiscretized((L,) * 3, upsample_factor=1, mode="nearest")
+        gain = torch.zeros((L,) * 3, device=self.images.device, dtype=self.dtype)
+        for i in range(0, len(self), batch_size):
+            _, pts, filters, _ = self[slice(i, min(i + batch_size, len(self)))]
+            plan.setpts(pts)
+            cf = torch.complex(filters, torch.zeros_like(filters))
+            gain += im_backward(cf, plan, filters, fourier_domain=True).squeeze().abs()
+        return gain / L
+
+    def get_total_covar_gain(self, batch_size: int = 16384) -> torch.Tensor:
+        """Shell x shell gain of the covariance least-squares problem (dataset.py:358-421):
+        sum_i a_i a_i^T, a_i the Fourier-shell average of image i's nearest-neighbour
+        (upsampling 1) back-projection of CTF_i^2 / L^2 (Nyquist row and column zeroed for even L).
+
+        The reference forms each image's L^3 back-projection (execute_adjoint_unaggregated) and
+        averages it over shells.  Every contribution is non-negative, so the shell average is
+        the sum of CTF_i^2 / L^2 over the image's pixels whose nearest voxel lies in the shell,
+        divided by the shell size: one pass over the pixels, no per-image volume.  The nearest
+        voxel is rounded exactly as the slice kernels do (grid_sample, align_corners=True, half
+        to even; points outside the volume are dropped).
+        """
+        from .fsc_utils import shell_index
+        from .nufft_plan import NufftPlanDiscretized
+
+        L = self.resolution
+        dev = self.images.device
+        shell_of, size, n = shell_index(L, 3, dev)
+        plan = NufftPlanDiscretized((L,) * 3, upsample_factor=1, mode="nearest")
+        acc = torch.zeros((n, n), device=dev, dtype=torch.float64)
+        for i in range(0, len(self), batch_size):
+            _, pts, filters, _ = self[slice(i, min(i + batch_size, len(self)))]
+            B = filters.shape[0]
+            plan.setpts(pts)
+            grid = plan._grid_flat.to(device=dev, dtype=torch.float32)
+            f2 = (filters.to(torch.float32) ** 2).clone()
+            if L % 2 == 0:
+                f2[:, 0, :] = 0
+                f2[:, :, 0] = 0
+            vox = torch.round(((grid + 1) / 2) * float(L - 1)).to(torch.int64)  # (x, y, z) per point
+            ok = ((vox >= 0) & (vox < L)).all(dim=1)
+            flat = (vox[:, 2] * L + vox[:, 1]) * L + vox[:, 0]
+            sh = shell_of[torch.where(ok, flat, torch.zeros_like(flat))]
+            img = torch.arange(B, device=dev).repeat_interleave(L * L)
+            w = torch.where(ok, f2.reshape(-1).double() / L**2, torch.zeros((), dtype=torch.float64, device=dev))
+            a = torch.zeros((B * n,), device=dev, dtype=torch.float64)
+            a.index_add_(0, img * n + sh, w)
+            a = a.reshape(B, n) / size
+            acc += a.T @ a
+        return acc.to(self.dtype)
+
     def _subset(self, idx: torch.Tensor) -> "CovarDataset":
         sub = CovarDataset(
             self.images[idx], self.rotations[idx], self.noise_var,
