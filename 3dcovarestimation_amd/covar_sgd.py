@@ -338,9 +338,50 @@ class CovarTrainer:
             fourier_reg = upsample_and_expand_fourier_shell(radial, self.covar.resolution * us, 3)
         self.fourier_reg = fourier_reg
 
+    @property
+    def vectorsGT(self) -> Optional[torch.Tensor]:
+        """Ground-truth eigenvectors when ``gt_data`` carries them (covar_sgd.py:158-172)."""
+        gt = self.gt_data
+        return getattr(gt, "eigenvecs", None) if gt is not None else None
+
+    def results_dict(self) -> dict:
+        """The saved result, laid out as the reference's ``training_results.bin``
+        (covar_sgd.py:524-535): the Covar state dict (``vectors``, ``log_sqrt_eigenvals``,
+        ``_in_spatial_domain``, ``grid_correction``), ``vectorsGT``, ``fourier_reg`` and the
+        training-log lists at the top level.  Tensors are moved to the host, so the file loads
+        with ``torch.load(path, weights_only=True)`` on any machine."""
+        def host(x):
+            return x.detach().cpu() if isinstance(x, torch.Tensor) else x
+
+        ckp = {k: host(v) for k, v in self.covar.state_dict().items()}
+        ckp["vectorsGT"] = host(self.vectorsGT)
+        ckp["fourier_reg"] = host(self.fourier_reg)
+        ckp.update({k: [host(x) for x in v] if isinstance(v, list) else host(v) for k, v in self.training_log.items()})
+        return ckp
+
     def save_result(self) -> None:
-        state = {"covar": self.covar.state_dict(), "fourier_reg": self.fourier_reg, "log": self.training_log}
-        torch.save(state, self.save_path)
+        """torch.save of ``results_dict()`` to ``save_path``, creating its folder (covar_sgd.py:537-545)."""
+        import os
+
+        folder = os.path.split(self.save_path)[0]
+        if folder:
+            os.makedirs(folder, exist_ok=True)
+        torch.save(self.results_dict(), self.save_path)
+
+
+def load_training_results(path: str) -> dict:
+    """Reads a ``training_results.bin`` written by ``CovarTrainer.save_result`` (tensors, lists,
+    numbers, None only), with the loader that executes nothing from the file."""
+    return torch.load(path, map_location="cpu", weights_only=True)
+
+
+def covar_from_results(results: dict, dtype: torch.dtype = torch.float32) -> Covar:
+    """Rebuilds the ``Covar`` module from a results dictionary (its state-dict entries)."""
+    keys = ("vectors", "log_sqrt_eigenvals", "_in_spatial_domain", "grid_correction")
+    v = results["vectors"]
+    covar = Covar(v.shape[-1], v.shape[0], dtype=dtype)
+    covar.load_state_dict({k: results[k] for k in keys})
+    return covar
 
 
 def trainCovar(covar_model, dataset, batch_size, optimize_pose=False, mean_model=None, pose=None, savepath=None,
