@@ -1569,7 +1569,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int kHitCap = 128;
 constexpr int kCsortSlack = 64;  // complex values of slack after the sorted value rows
 
-template <int TC>
+template <int TC, bool OFF32>
 __global__ void __launch_bounds__(kGatherBlock)
     k_spread_mfma(const Cplx<float>* __restrict__ cs, Cplx<float>* __restrict__ fine, BinGeom g,
                   const int* __restrict__ bin_start, const int* __restrict__ bin_count,
@@ -1579,6 +1579,8 @@ __global__ void __launch_bounds__(kGatherBlock)
   constexpr int NCB = (NC + 15) / 16;
   constexpr int kNb = 27;
   __shared__ float4 s_rec[kHitCap];  // p0, p1, p2, packed block offsets q0 | q1 << 10 | q2 << 20 (+512 each)
+  // per hit: OFF32 -> the float index of its value row's chunk in cs (uint32 bits; the
+  // address is formed once per hit, lane-parallel, instead of once per step), else its point
   __shared__ int s_idx[kHitCap];
   __shared__ float s_w[kHitCap * 12];  // per hit: wx[0..3], wy[0..3], wz[0..3]
   const int t0 = blockIdx.y * TC;
@@ -1624,7 +1626,8 @@ __global__ void __launch_bounds__(kGatherBlock)
   const int total = __builtin_amdgcn_readlane(incl, kNb - 1);
 
   auto load_b = [&](int slot, float* bv) {
-    const float* row = reinterpret_cast<const float*>(cs + (int64_t)s_idx[slot] * ldc + t0);
+    const float* row = OFF32 ? reinterpret_cast<const float*>(cs) + (uint32_t)s_idx[slot]
+                             : reinterpret_cast<const float*>(cs + (int64_t)s_idx[slot] * ldc + t0);
 #pragma unroll
     for (int cb = 0; cb < NCB; ++cb) bv[cb] = row[sub + 16 * cb];
   };
@@ -1713,7 +1716,7 @@ __global__ void __launch_bounds__(kGatherBlock)
       const int r = nh + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
                                                         __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
       s_rec[r] = make_float4(v0, v1, v2, __int_as_float(pk));
-      s_idx[r] = pi;
+      s_idx[r] = OFF32 ? (int)((uint32_t)pi * (uint32_t)(2 * ldc) + (uint32_t)(2 * t0)) : pi;
     }
     __syncthreads();
     for (int f = lane; f < 12 * nnew; f += kGatherBlock) {
@@ -2122,7 +2125,10 @@ static int launch_spread(Plan* p, const void* c_in, int ntrans_run, hipStream_t 
   ScopedTiming tm(kTSpread, st);
   if constexpr (sizeof(T) == 4) {
     if (p->spread_mode == 4 && p->w <= 8) {
-      hipLaunchKernelGGL((k_spread_mfma<TC>), grid, dim3(kGatherBlock), 0, st, (const Cplx<float>*)p->csort,
+      // 32-bit row offsets when every float index of the sorted input fits in 32 bits
+      const bool off32 = ((int64_t)p->M * ldc + kCsortSlack) * 2 < ((int64_t)1 << 32);
+      auto k = off32 ? k_spread_mfma<TC, true> : k_spread_mfma<TC, false>;
+      hipLaunchKernelGGL(k, grid, dim3(kGatherBlock), 0, st, (const Cplx<float>*)p->csort,
                          (Cplx<float>*)p->fine, p->geom, p->bin_start, p->bin_count, (const float*)p->sv[0],
                          (const float*)p->sv[1], (const float*)p->sv[2], ntrans_run, ldc, p->w, (float)kc,
                          (float)(p->beta * 1.4426950408889634));
