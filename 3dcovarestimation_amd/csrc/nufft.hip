@@ -225,15 +225,33 @@ __device__ __forceinline__ int point_bin(const T* x, const T* y, const T* z, int
   return bin_index(g, b[0], b[1], b[2]);
 }
 
+// Bin of every point (the sort key) and its super-bin-relative coordinates, kept as a 4-wide
+// record pv[4j..4j+2] for k_bin_place.  Consecutive points (neighbouring pixels of one image)
+// often share a bin: each run of equal bins among consecutive lanes adds its length with one
+// atomic.
 template <typename T>
 __global__ void k_bin_count(const T* x, const T* y, const T* z, int64_t M, BinGeom g,
-                            int* bin_count, int* pt_bin) {
-  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= M) return;
-  double v[3];
-  int bin = point_bin(x, y, z, j, g, v);
-  pt_bin[j] = bin;
-  atomicAdd(&bin_count[bin], 1);
+                            int* bin_count, int* pt_bin, T* __restrict__ pv) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool ok = j < M;
+  int bin = -1;
+  if (ok) {
+    double v[3];
+    bin = point_bin(x, y, z, j, g, v);
+    pt_bin[j] = bin;
+    pv[4 * j] = (T)v[0];
+    pv[4 * j + 1] = (T)v[1];
+    pv[4 * j + 2] = (T)v[2];
+  }
+  const int lane = threadIdx.x & 63;
+  const int prev = __shfl_up(bin, 1);
+  const bool start = ok && (lane == 0 || prev != bin);
+  const unsigned long long ends = __ballot(start || !ok);  // a lane past the end closes a run
+  if (start) {
+    const unsigned long long above = lane == 63 ? 0ull : ends >> (lane + 1);
+    const int len = above ? __ffsll((long long)above) : 64 - lane;
+    atomicAdd(&bin_count[bin], len);
+  }
 }
 
 __global__ void k_sub_fill(const int* bin_count, const int* bin_start, const int* sub_start,
@@ -253,19 +271,30 @@ __global__ void k_sub_fill(const int* bin_count, const int* bin_start, const int
 }
 
 // Sorted position p holds point sidx[p] (stable radix sort by bin): its inverse and its
-// super-bin-relative coordinates.
+// super-bin-relative coordinates (from k_bin_count's records).
 template <typename T>
-__global__ void k_bin_place(const T* x, const T* y, const T* z, int64_t M, BinGeom g,
-                            const int* __restrict__ sidx, int* spos, T* sv0, T* sv1, T* sv2) {
+__global__ void k_bin_place(const T* __restrict__ pv, int64_t M, const int* __restrict__ sidx, int* spos, T* sv0,
+                            T* sv1, T* sv2) {
   int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= M) return;
   const int j = sidx[p];
-  double v[3];
-  (void)point_bin(x, y, z, j, g, v);
+  T v[4];
+  if constexpr (sizeof(T) == 4) {
+    const float4 r = reinterpret_cast<const float4*>(pv)[j];
+    v[0] = r.x;
+    v[1] = r.y;
+    v[2] = r.z;
+  } else {
+    const double2 a = reinterpret_cast<const double2*>(pv)[2 * (int64_t)j];
+    const double2 b = reinterpret_cast<const double2*>(pv)[2 * (int64_t)j + 1];
+    v[0] = a.x;
+    v[1] = a.y;
+    v[2] = b.x;
+  }
   spos[j] = (int)p;
-  sv0[p] = (T)v[0];
-  sv1[p] = (T)v[1];
-  sv2[p] = (T)v[2];
+  sv0[p] = v[0];
+  sv1[p] = v[1];
+  sv2[p] = v[2];
 }
 
 hipError_t sort_points_by_bin(void* tmp, size_t& tmp_bytes, const unsigned* keys_in, unsigned* keys_out,
@@ -1935,6 +1964,7 @@ struct Plan {
   int* sub_n = nullptr;
   int* nsub = nullptr;
   int* pt_bin = nullptr;
+  void* pv = nullptr;  // setpts: per-point (coordinates relative to the super-bin, pad), 4 x T
   int* pt_pos = nullptr;      // sorted bin keys (radix sort output)
   void* sort_tmp = nullptr;   // radix sort work area
   size_t sort_tmp_bytes = 0;
@@ -1981,6 +2011,8 @@ static int free_points(Plan* p) {
   p->irec_valid = false;
   hipFree(p->csort);
   for (int k = 0; k < 3; ++k) hipFree(p->sv[k]);
+  hipFree(p->pv);
+  p->pv = nullptr;
   hipFree(p->sub_sb);
   hipFree(p->sub_pt);
   hipFree(p->sub_n);
@@ -2463,6 +2495,7 @@ static int setpts_impl(Plan* p, int64_t M, const T* x, const T* y, const T* z, h
       COV3D_HIP(hipMemset((char*)p->csort + body, 0, kCsortSlack * csz));
     }
     for (int k = 0; k < 3; ++k) COV3D_HIP(hipMalloc(&p->sv[k], Ma * sizeof(T)));
+    COV3D_HIP(hipMalloc(&p->pv, Ma * 4 * sizeof(T)));
     const int64_t ca = cap > 8 ? cap : 8;
     COV3D_HIP(hipMalloc(&p->sub_sb, ca * sizeof(int)));
     COV3D_HIP(hipMalloc(&p->sub_pt, ca * sizeof(int)));
@@ -2479,7 +2512,7 @@ static int setpts_impl(Plan* p, int64_t M, const T* x, const T* y, const T* z, h
   if (M > 0) {
     dim3 gm((unsigned)ceil_div(M, kBlock));
     hipLaunchKernelGGL((k_bin_count<T>), gm, dim3(kBlock), 0, st, x, y, z, M, p->geom, p->bin_count,
-                       p->pt_bin);
+                       p->pt_bin, (T*)p->pv);
     COV3D_LAUNCH_CHECK();
     int nbits = 1;
     while ((int64_t)1 << nbits < p->nbins) ++nbits;
@@ -2525,7 +2558,7 @@ static int setpts_impl(Plan* p, int64_t M, const T* x, const T* y, const T* z, h
                        p->sub_n);
     COV3D_LAUNCH_CHECK();
     dim3 gm((unsigned)ceil_div(M, kBlock));
-    hipLaunchKernelGGL((k_bin_place<T>), gm, dim3(kBlock), 0, st, x, y, z, M, p->geom, p->sidx, p->spos,
+    hipLaunchKernelGGL((k_bin_place<T>), gm, dim3(kBlock), 0, st, (const T*)p->pv, M, p->sidx, p->spos,
                        (T*)p->sv[0], (T*)p->sv[1], (T*)p->sv[2]);
     COV3D_LAUNCH_CHECK();
   }
