@@ -964,8 +964,6 @@ __global__ void __launch_bounds__(kBlock)
   extern __shared__ __align__(16) unsigned char smem_raw[];
   float4* t4 = reinterpret_cast<float4*>(smem_raw);
   constexpr int E = kSuperEdge + W;
-  constexpr int ncell = E * E * E;
-  constexpr int NIT = (ncell + kBlock - 1) / kBlock;
   constexpr int PL = interp_plane<float, W, 2>();
   constexpr int kPer = kMaxSub / kBlock;
   const int nsub = *sd.nsub;
@@ -984,7 +982,14 @@ __global__ void __launch_bounds__(kBlock)
   __syncthreads();
   int64_t item = item_of(s_v[0]);
   if (item >= nitems) return;
-  float4 tv[NIT];  // next tile: cell threadIdx.x + k kBlock, transforms t0 and t0 + 1
+  // Tile staging: thread c < E^2 owns the tile column (e1, e2) = (c / E, c % E) and loads its E
+  // cells e0 = 0 .. E-1; the fine-grid row of e0 is wave-uniform, so a cell costs a select and
+  // an add beyond its loads.
+  static_assert(E * E <= kBlock, "one tile column per thread");
+  const int col = threadIdx.x;
+  const bool has_col = col < E * E;
+  const int ce1 = col / E, ce2 = col % E;
+  float4 tv[E];  // next tile: cells (e0, ce1, ce2), transforms t0 and t0 + 1
   float4 nrec[kPer];
   int nidx[kPer];
   // records first: waiting for the tile's loads then covers them too
@@ -1005,18 +1010,25 @@ __global__ void __launch_bounds__(kBlock)
     const int s0 = sb / (g.nsb[2] * g.nsb[1]);
     const int o0 = s0 * kSuperEdge - W / 2, o1 = s1 * kSuperEdge - W / 2, o2 = s2 * kSuperEdge - W / 2;
     const bool two = t0 + 1 < ntrans;
+    // the column's offset within a fine-grid x row, and whether its values are mirrored
+    // (fine_cell's Hermitian-half rule: m2 >= half_h2 -> all three indices negated, conjugate)
+    const int m1 = wrap1(o1 + ce1, g.nf[1]), m2 = wrap1(o2 + ce2, g.nf[2]);
+    const bool mir = half_h2 && m2 >= half_h2;
+    const bool neg = half_h2 && (mir != (conj_all != 0));
+    const int ld2 = half_h2 ? half_h2 : g.nf[2];
+    const int colo = mir ? (m1 ? g.nf[1] - m1 : 0) * ld2 + (g.nf[2] - m2) : m1 * ld2 + m2;
+    const int rowst = g.nf[1] * ld2;
+    const Cplx<float>* f0 = fine + (int64_t)t0 * nfine;
+    const Cplx<float>* f1 = two ? f0 + nfine : f0;
 #pragma unroll
-    for (int k = 0; k < NIT; ++k) {
-      const int cidx = threadIdx.x + k * kBlock;
-      if (cidx < ncell) {
-        const int e2 = cidx % E;
-        const int e1 = (cidx / E) % E;
-        const int e0 = cidx / (E * E);
-        bool neg;
-        const int64_t off = fine_cell(g, wrap1(o0 + e0, g.nf[0]), wrap1(o1 + e1, g.nf[1]), wrap1(o2 + e2, g.nf[2]),
-                                      half_h2, conj_all, neg);
-        const Cplx<float> x0 = fine[(int64_t)t0 * nfine + off];
-        const Cplx<float> x1 = two ? fine[(int64_t)(t0 + 1) * nfine + off] : Cplx<float>{0.f, 0.f};
+    for (int k = 0; k < E; ++k) {
+      if (has_col) {
+        const int r0 = wrap1(o0 + k, g.nf[0]);
+        const int r0m = r0 ? g.nf[0] - r0 : 0;
+        const uint32_t off = (uint32_t)((mir ? r0m : r0) * rowst + colo);
+        const Cplx<float> x0 = f0[off];
+        Cplx<float> x1 = f1[off];
+        if (!two) x1 = Cplx<float>{0.f, 0.f};
         tv[k] = neg ? make_float4(x0.x, -x0.y, x1.x, -x1.y) : make_float4(x0.x, x0.y, x1.x, x1.y);
       }
     }
@@ -1029,10 +1041,8 @@ __global__ void __launch_bounds__(kBlock)
     const int np = sd.sub_n[sub];
     __syncthreads();  // the previous item's taps are done with the tile
 #pragma unroll
-    for (int k = 0; k < NIT; ++k) {
-      const int cidx = threadIdx.x + k * kBlock;
-      if (cidx < ncell) t4[cidx + (cidx / (E * E)) * (PL - E * E)] = tv[k];
-    }
+    for (int k = 0; k < E; ++k)
+      if (has_col) t4[k * PL + col] = tv[k];
     // the current item's records move to LDS (each thread reads back only its own slots)
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
