@@ -1621,7 +1621,7 @@ __global__ void __launch_bounds__(kGatherBlock)
   // per hit: OFF32 -> the float index of its value row's chunk in cs (uint32 bits; the
   // address is formed once per hit, lane-parallel, instead of once per step), else its point
   __shared__ int s_idx[kHitCap];
-  __shared__ float s_w[kHitCap * 12];  // per hit: wx[0..3], wy[0..3], wz[0..3]
+  __shared__ __align__(16) float s_w[kHitCap * 12];  // per hit: wx[0..3], wy[0..3], wz[0..3]
   const int t0 = blockIdx.y * TC;
   const int lane = threadIdx.x;
   int b0, b1, b2;
@@ -1758,13 +1758,22 @@ __global__ void __launch_bounds__(kGatherBlock)
       s_idx[r] = OFF32 ? (int)((uint32_t)pi * (uint32_t)(2 * ldc) + (uint32_t)(2 * t0)) : pi;
     }
     __syncthreads();
-    for (int f = lane; f < 12 * nnew; f += kGatherBlock) {
-      const int slot = nh + f / 12, e = f % 12, ax = e >> 2, c = e & 3;
+    // one lane per (hit, axis): the axis's 4 weights (the bin's 4 cells), stored as one float4
+    for (int f = lane; f < 3 * nnew; f += kGatherBlock) {
+      const int hit = f / 3, ax = f - 3 * hit;
+      const int slot = nh + hit;
       const float4 rec = s_rec[slot];
       const float p = ax == 0 ? rec.x : (ax == 1 ? rec.y : rec.z);
-      const int qa = ((__float_as_int(rec.w) >> (10 * ax)) & 1023) - 512;
-      const int x = qa + c - (int)ceilf(p - halfw);
-      s_w[slot * 12 + e] = (x >= 0 && x < W) ? es_eval_native((float)(qa + c) - p, kc, beta_l2) : 0.f;
+      const int sh = ax == 0 ? 0 : (ax == 1 ? 10 : 20);
+      const int qa = ((__float_as_int(rec.w) >> sh) & 1023) - 512;
+      const int j0 = (int)ceilf(p - halfw);
+      float w4[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int x = qa + c - j0;
+        w4[c] = (x >= 0 && x < W) ? es_eval_native((float)(qa + c) - p, kc, beta_l2) : 0.f;
+      }
+      *reinterpret_cast<float4*>(&s_w[slot * 12 + 4 * ax]) = make_float4(w4[0], w4[1], w4[2], w4[3]);
     }
     __syncthreads();
     nh += nnew;
