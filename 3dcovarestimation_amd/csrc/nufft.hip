@@ -86,6 +86,48 @@ static inline double es_eval_host(double t, int w, double beta) {
   return exp(beta * (sqrt(s) - 1.0));
 }
 
+// Degree-8 polynomial form of the stencil weights.  With j0 = ceil(u - W/2) the first tap of a
+// point at u, tap a's weight is phi(x + a - W/2) for x = j0 - u + W/2 in [0, 1).  In y = x - 1/2,
+// tap a is P_a(y) = E_a(y^2) + y O_a(y^2), and phi being even gives tap W-1-a = E_a(y^2) - y O_a(y^2):
+// one pair of short Horner chains per mirrored pair of taps, no exp or sqrt.  P_a interpolates phi
+// at the 9 Chebyshev points of [-1/2, 1/2] (solved here in FP64); evaluated in FP32 it is within
+// 1.2e-7 of phi for W = 7 (the FP32 floor; the native exp/sqrt path is at the same level).
+struct EsPoly {
+  float e[4][5];  // E_a: coefficients of y^0, y^2, ..., y^8
+  float o[4][4];  // O_a: coefficients of y^1, y^3, ..., y^7
+};
+
+static EsPoly es_poly(int w, double beta) {
+  EsPoly P{};
+  constexpr int n = 9;
+  for (int a = 0; a < (w + 1) / 2 && a < 4; ++a) {
+    double A[n][n + 1];
+    for (int k = 0; k < n; ++k) {
+      const double y = 0.5 * cos(M_PI * (2 * k + 1) / (2.0 * n));
+      double pw = 1.0;
+      for (int d = 0; d < n; ++d, pw *= y) A[k][d] = pw;
+      A[k][n] = es_eval_host(y + 0.5 + a - 0.5 * w, w, beta);
+    }
+    for (int col = 0; col < n; ++col) {  // Gaussian elimination, partial pivoting
+      int piv = col;
+      for (int r = col + 1; r < n; ++r)
+        if (fabs(A[r][col]) > fabs(A[piv][col])) piv = r;
+      for (int d = 0; d <= n; ++d) std::swap(A[col][d], A[piv][d]);
+      for (int r = 0; r < n; ++r) {
+        if (r == col) continue;
+        const double f = A[r][col] / A[col][col];
+        for (int d = col; d <= n; ++d) A[r][d] -= f * A[col][d];
+      }
+    }
+    for (int d = 0; d < n; ++d) {
+      const float cf = (float)(A[d][n] / A[d][d]);
+      if (d % 2 == 0) P.e[a][d / 2] = cf;
+      else P.o[a][d / 2] = cf;
+    }
+  }
+  return P;
+}
+
 // 1/p(n) for n in [-floor(N/2), ceil(N/2)-1], p(n) = int phi(t) cos(2 pi n t / nf) dt.
 static std::vector<double> deconv_factors(int64_t N, int64_t nf, int w, double beta) {
   std::vector<double> gx, gw;
@@ -630,6 +672,33 @@ __device__ __forceinline__ void stencil_weights(T u, T c, T beta, int& l0, T* w)
   l0 = j0 + W / 2;
 }
 
+// Stencil weights from the polynomial form (EsPoly); l0 as stencil_weights.  y is exact: j0 - u
+// is a multiple of u's ulp below 2^23 of them.
+template <int W>
+__device__ __forceinline__ void stencil_weights_poly(float u, const EsPoly& P, int& l0, float* w) {
+  static_assert(W <= 8, "four mirrored tap pairs at most");
+  const float halfw = 0.5f * (float)W;
+  const int j0 = (int)ceilf(u - halfw);
+  const float y = ((float)j0 - u + halfw) - 0.5f;
+  const float y2 = y * y;
+#pragma unroll
+  for (int a = 0; a < (W + 1) / 2; ++a) {
+    float e = P.e[a][4];
+#pragma unroll
+    for (int d = 3; d >= 0; --d) e = fmaf(e, y2, P.e[a][d]);
+    if (2 * a + 1 == W) {
+      w[a] = e;
+    } else {
+      float o = P.o[a][3];
+#pragma unroll
+      for (int d = 2; d >= 0; --d) o = fmaf(o, y2, P.o[a][d]);
+      w[a] = fmaf(y, o, e);
+      w[W - 1 - a] = fmaf(-y, o, e);
+    }
+  }
+  l0 = j0 + W / 2;
+}
+
 // Fine-grid index of an unwrapped coordinate within one period of [0, n).
 __device__ __forceinline__ int wrap1(int m, int n) {
   m += m < 0 ? n : 0;
@@ -834,7 +903,8 @@ __global__ void __launch_bounds__(kBlock)
     k_interp(const Cplx<T>* __restrict__ fine, Cplx<T>* __restrict__ c, SubDesc sd, BinGeom g,
              const T* __restrict__ sv0, const T* __restrict__ sv1, const T* __restrict__ sv2,
              const int* __restrict__ sidx, int64_t M, int ntrans, T kc, T beta, int half_h2, int conj_all,
-             const float4* __restrict__ irec, const int* __restrict__ iidx, Cplx<T>* __restrict__ direct) {
+             const float4* __restrict__ irec, const int* __restrict__ iidx, Cplx<T>* __restrict__ direct,
+             EsPoly poly) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Cplx<T>* tile = reinterpret_cast<Cplx<T>*>(smem_raw);
   const int sub = xcd_remap(blockIdx.x, gridDim.x);
@@ -891,9 +961,15 @@ __global__ void __launch_bounds__(kBlock)
     }
     T w0[W], w1[W], w2[W];
     int l0, l1, l2;
-    stencil_weights<T, W>(u0, kc, beta, l0, w0);
-    stencil_weights<T, W>(u1, kc, beta, l1, w1);
-    stencil_weights<T, W>(u2, kc, beta, l2, w2);
+    if constexpr (sizeof(T) == 4 && W <= 8) {
+      stencil_weights_poly<W>(u0, poly, l0, w0);
+      stencil_weights_poly<W>(u1, poly, l1, w1);
+      stencil_weights_poly<W>(u2, poly, l2, w2);
+    } else {
+      stencil_weights<T, W>(u0, kc, beta, l0, w0);
+      stencil_weights<T, W>(u1, kc, beta, l1, w1);
+      stencil_weights<T, W>(u2, kc, beta, l2, w2);
+    }
     l0 = min(max(l0, 0), kSuperEdge);
     l1 = min(max(l1, 0), kSuperEdge);
     l2 = min(max(l2, 0), kSuperEdge);
@@ -960,7 +1036,8 @@ template <int W>
 __global__ void __launch_bounds__(kBlock)
     k_interp_pipe(const Cplx<float>* __restrict__ fine, Cplx<float>* __restrict__ c, SubDesc sd, BinGeom g,
                   const float4* __restrict__ irec, const int* __restrict__ iidx, int64_t M, int ntrans, float kc,
-                  float beta, int half_h2, int conj_all, Cplx<float>* __restrict__ direct, int* __restrict__ ctr) {
+                  float beta, int half_h2, int conj_all, Cplx<float>* __restrict__ direct, int* __restrict__ ctr,
+                  EsPoly poly) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   float4* t4 = reinterpret_cast<float4*>(smem_raw);
   constexpr int E = kSuperEdge + W;
@@ -1065,9 +1142,9 @@ __global__ void __launch_bounds__(kBlock)
       const int64_t j = s_idx[k][threadIdx.x];
       float w0[W], w1[W], w2[W];
       int l0, l1, l2;
-      stencil_weights<float, W>(u0, kc, beta, l0, w0);
-      stencil_weights<float, W>(u1, kc, beta, l1, w1);
-      stencil_weights<float, W>(u2, kc, beta, l2, w2);
+      stencil_weights_poly<W>(u0, poly, l0, w0);
+      stencil_weights_poly<W>(u1, poly, l1, w1);
+      stencil_weights_poly<W>(u2, poly, l2, w2);
       l0 = min(max(l0, 0), kSuperEdge);
       l1 = min(max(l1, 0), kSuperEdge);
       l2 = min(max(l2, 0), kSuperEdge);
@@ -1964,6 +2041,7 @@ struct Plan {
   int interp_pipe = 1;    // mode 0, FP32 with point records: k_interp_pipe instead of k_interp
   int pipe_grid = 0;      // k_interp_pipe's grid: resident workgroups on the device, a multiple of 8
   int* pipe_ctr = nullptr;  // k_interp_pipe's per-XCD item counters (8 ints)
+  EsPoly poly{};            // polynomial stencil weights of the FP32 interpolation kernels
   bool presorted = false;  // the current type-1 call's values are already in csort (no permute)
   int max_tc_gather = 20;  // transforms per gather workgroup
   int tc_spread = 2;    // transforms per spreading workgroup (tile mode)
@@ -2107,7 +2185,7 @@ static int launch_interp(Plan* p, void* c_out, int ntrans_run, hipStream_t st) {
                            (const Cplx<float>*)p->fine, (Cplx<float>*)p->csort, sd, p->geom,
                            (const float4*)p->irec, (const int*)p->iidx, p->M, ntrans_run, (float)kc,
                            (float)p->beta, p->stage_half, p->stage_conj,
-                           direct ? (Cplx<float>*)c_out : nullptr, p->pipe_ctr);
+                           direct ? (Cplx<float>*)c_out : nullptr, p->pipe_ctr, p->poly);
         done = true;
       }
     }
@@ -2122,7 +2200,7 @@ static int launch_interp(Plan* p, void* c_out, int ntrans_run, hipStream_t st) {
                          (const T*)p->sv[2], p->sidx, p->M, ntrans_run, kc, (T)p->beta, p->stage_half,
                          p->stage_conj, (TC == 2 && p->irec_valid) ? (const float4*)p->irec : nullptr,
                          (TC == 2 && p->irec_valid) ? (const int*)p->iidx : nullptr,
-                         direct ? (Cplx<T>*)c_out : nullptr);
+                         direct ? (Cplx<T>*)c_out : nullptr, p->poly);
   }
   COV3D_LAUNCH_CHECK();
   if (direct) return 0;
@@ -2648,6 +2726,7 @@ static int make_plan(int type, const int64_t* n, int iflag, int64_t ntrans, doub
   p->eps = eps < emin ? emin : eps;
   p->w = es_width(p->eps);
   p->beta = es_beta(p->w);
+  p->poly = es_poly(p->w, p->beta);
   const int tc_env = env_int("COV3D_NUFFT_TC", 2);
   p->tc = dtype == 1 ? 1 : (tc_env == 1 || tc_env == 4 ? tc_env : 2);
   {
