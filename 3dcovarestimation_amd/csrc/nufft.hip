@@ -1686,13 +1686,22 @@ constexpr int kHitCap = 128;
 constexpr int kCsortSlack = 64;  // complex values of slack after the sorted value rows
 
 template <int TC, bool OFF32>
-__global__ void __launch_bounds__(kGatherBlock)
+__global__ void __launch_bounds__(kGatherBlock) __attribute__((amdgpu_waves_per_eu(4)))
     k_spread_mfma(const Cplx<float>* __restrict__ cs, Cplx<float>* __restrict__ fine, BinGeom g,
                   const int* __restrict__ bin_start, const int* __restrict__ bin_count,
                   const float* __restrict__ sv0, const float* __restrict__ sv1,
                   const float* __restrict__ sv2, int ntrans, int ldc, int W, float kc, float beta_l2) {
   constexpr int NC = 2 * TC;
-  constexpr int NCB = (NC + 15) / 16;
+  // Columns go in blocks of 16 on v_mfma_f32_16x16x4_f32; a tail of at most 8 columns goes in
+  // groups of 4 on v_mfma_f32_4x4x1_16b_f32 (measured 10.6 against 36 cycles per instruction),
+  // whose 16 blocks are (hit grp, cell row c1) with rows c2 and columns 4 of the tail, so its A
+  // operand is the 16x16x4 one unchanged.  Its accumulators hold per-hit-slot partial sums,
+  // added across the 4 lane groups at the end.
+  constexpr int NR = NC % 16;
+  constexpr bool kTail4 = NR > 0 && NR <= 8;
+  constexpr int NCB = kTail4 ? NC / 16 : (NC + 15) / 16;
+  constexpr int NCG = kTail4 ? (NR + 3) / 4 : 0;
+  constexpr int NB = NCB + NCG;  // B registers per hit
   constexpr int kNb = 27;
   __shared__ float4 s_rec[kHitCap];  // p0, p1, p2, packed block offsets q0 | q1 << 10 | q2 << 20 (+512 each)
   // per hit: OFF32 -> the float index of its value row's chunk in cs (uint32 bits; the
@@ -1705,11 +1714,15 @@ __global__ void __launch_bounds__(kGatherBlock)
   gather_bin(blockIdx.x, gridDim.x, g, b0, b1, b2);
   const float halfw = 0.5f * (float)W;
   const int grp = lane >> 4, sub = lane & 15;
-  f32x4 acc[4][NCB];
+  f32x4 acc[4][NCB > 0 ? NCB : 1];
+  f32x4 acc4[4][NCG > 0 ? NCG : 1];
 #pragma unroll
-  for (int c0 = 0; c0 < 4; ++c0)
+  for (int c0 = 0; c0 < 4; ++c0) {
 #pragma unroll
     for (int cb = 0; cb < NCB; ++cb) acc[c0][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int cg = 0; cg < NCG; ++cg) acc4[c0][cg] = f32x4{0.f, 0.f, 0.f, 0.f};
+  }
 
   // ---- neighbour table: lane n < 27 holds neighbour n's first sorted point, the exclusive
   // prefix of the point counts and the packed offset of this block in the neighbour's frame
@@ -1746,6 +1759,8 @@ __global__ void __launch_bounds__(kGatherBlock)
                              : reinterpret_cast<const float*>(cs + (int64_t)s_idx[slot] * ldc + t0);
 #pragma unroll
     for (int cb = 0; cb < NCB; ++cb) bv[cb] = row[sub + 16 * cb];
+#pragma unroll
+    for (int cg = 0; cg < NCG; ++cg) bv[NCB + cg] = row[16 * NCB + 4 * cg + (sub & 3)];
   };
   // A cell block (one x level of the bin) whose x weight is zero for all 4 hits of a step adds
   // nothing: its 3 MFMAs are skipped (wave-uniform ballot per level).
@@ -1760,6 +1775,9 @@ __global__ void __launch_bounds__(kGatherBlock)
 #pragma unroll
         for (int cb = 0; cb < NCB; ++cb)
           acc[c0][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bv[cb], acc[c0][cb], 0, 0, 0);
+#pragma unroll
+        for (int cg = 0; cg < NCG; ++cg)
+          acc4[c0][cg] = __builtin_amdgcn_mfma_f32_4x4x1f32(a, bv[NCB + cg], acc4[c0][cg], 0, 0, 0);
       }
     }
   };
@@ -1768,7 +1786,7 @@ __global__ void __launch_bounds__(kGatherBlock)
   // unconditionally (the last ones repeat valid slots) so that each wait covers exactly the
   // set it needs.
   auto steps = [&](int nfull) {
-    float ba[NCB], bb[NCB];
+    float ba[NB], bb[NB];
     load_b(grp, ba);
     load_b(4 + grp, bb);
     for (int s = 0; s < nfull; s += 8) {
@@ -1910,6 +1928,35 @@ __global__ void __launch_bounds__(kGatherBlock)
       if (tt < TC && t < ntrans) {
         const int64_t off = ((int64_t)(kBinEdge * b0 + c0) * g.nf[1] + g1) * g.nf[2] + g2;
         *reinterpret_cast<float4*>(fine + t * nfine + off) = out;
+      }
+    }
+  }
+  // 4x4 tail: lane l, register r -> cell (c1 = (l / 4) % 4, c2 = r), tail column l % 4 (+ 4 cg),
+  // one partial sum per lane group; after the sums over the groups, lanes l < 16 with an even
+  // column write (re, im) pairs.
+#pragma unroll
+  for (int cg = 0; cg < NCG; ++cg) {
+#pragma unroll
+    for (int c0 = 0; c0 < 4; ++c0) {
+      f32x4 d = acc4[c0][cg];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        d[r] += __shfl_xor(d[r], 16);
+        d[r] += __shfl_xor(d[r], 32);
+      }
+      const int jc = lane & 3;
+      const int colt = 16 * NCB + 4 * cg + jc;  // real column within the chunk
+      const int tt = colt >> 1;
+      const int t = t0 + tt;
+      float im[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) im[r] = __shfl_xor(d[r], 1);
+      if (lane < 16 && (jc & 1) == 0 && tt < TC && t < ntrans) {
+        const int c1 = (lane >> 2) & 3;
+        const int64_t base = t * nfine + ((int64_t)(kBinEdge * b0 + c0) * g.nf[1] + kBinEdge * b1 + c1) * g.nf[2] +
+                             kBinEdge * b2;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) fine[base + r] = Cplx<float>{d[r], im[r]};
       }
     }
   }
