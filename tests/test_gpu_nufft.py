@@ -190,6 +190,30 @@ def test_kernel_variants_match_nudft(dev, monkeypatch, spread_mode, interp_mode,
     assert np.array_equal(out1, plan.execute_adjoint(c).cpu().numpy())
 
 
+@pytest.mark.parametrize("eps", [1e-2, 1e-3, 1e-4, 1e-5])
+def test_lower_accuracy_widths_match_nudft(dev, eps):
+    """FP32 plans at tolerances that give kernel widths 3 to 6 (odd and even: the polynomial
+    stencil weights with and without a middle tap, the matrix-core spreading's weight tables)
+    stay within 10 eps of the NUDFT in both directions, with 20 transforms (the 4x4 tail)."""
+    o = oracle()
+    npl = pkg("nufft_plan")
+    L, ntrans = 16, 20
+    pts = _points(L, 3, 1, 400, dev, spread=1.2)
+    plan = npl.NufftPlan((L,) * 3, batch_size=ntrans, eps=eps, device=dev)
+    plan.setpts(pts)
+    g = torch.Generator().manual_seed(7)
+    vol = torch.randn((ntrans, L, L, L), generator=g).to(dev)
+    P = plan.points.cpu().numpy().astype(np.float64)
+    out = plan.execute_forward(vol).reshape(ntrans, -1).cpu().numpy()
+    ref = o.nudft_type2(vol.cpu().numpy().astype(np.complex128), P)
+    assert rel_l2(out, ref) < 10 * eps
+    M = P.shape[1]
+    c = torch.complex(torch.randn((ntrans, M), generator=g), torch.randn((ntrans, M), generator=g)).to(dev)
+    out1 = plan.execute_adjoint(c).cpu().numpy()
+    ref1 = o.nudft_type1(c.cpu().numpy(), P, (L, L, L))
+    assert rel_l2(out1, ref1) < 10 * eps
+
+
 @pytest.mark.parametrize("ntrans,direct", [(20, "1"), (5, "0")])
 def test_pipelined_interp_is_bitwise_equal(dev, monkeypatch, ntrans, direct):
     """The persistent pipelined interpolation runs the same arithmetic in the same order as the
