@@ -1,5 +1,5 @@
 // Pruned 3-D real FFTs of the NUFFT fine grid (FP32, cubic grids of NF = 64,
-// 128 or 256 cells per axis, N <= NF/2 + 1 modes per axis).
+// 128, 256 or 512 cells per axis, N <= NF/2 + 1 modes per axis).
 //
 // Type 2 turns the real modes into the Hermitian half of the fine-grid
 // spectrum.  The zero-padded grid holds N of NF values on each axis, so the
@@ -303,7 +303,7 @@ static int pruned_type1_nf(const float2* G, float* f, float2* work, const float2
 }
 
 bool pruned_fft_supported(int64_t NF, int64_t N) {
-  return (NF == 64 || NF == 128 || NF == 256) && N >= 1 && 2 * N <= NF + 2;
+  return (NF == 64 || NF == 128 || NF == 256 || NF == 512) && N >= 1 && 2 * N <= NF + 2;
 }
 
 // Work area (complex values) both pipelines need.
@@ -321,6 +321,7 @@ int pruned_type2(const float* f, void* fine, void* work, const void* tw, const f
     case 64: return pruned_type2_nf<64>(f, fo, w, t, ip0, ip1, ip2, ntrans, nrun, N, st);
     case 128: return pruned_type2_nf<128>(f, fo, w, t, ip0, ip1, ip2, ntrans, nrun, N, st);
     case 256: return pruned_type2_nf<256>(f, fo, w, t, ip0, ip1, ip2, ntrans, nrun, N, st);
+    case 512: return pruned_type2_nf<512>(f, fo, w, t, ip0, ip1, ip2, ntrans, nrun, N, st);
     default: set_error("pruned FFT: unsupported grid %d", NF); return COV3D_ERR_ARG;
   }
 }
@@ -334,6 +335,7 @@ int pruned_type1(const void* G, float* f, void* work, const void* tw, const floa
     case 64: return pruned_type1_nf<64>(g, f, w, t, ip0, ip1, ip2, ntrans, N, conj_out, st);
     case 128: return pruned_type1_nf<128>(g, f, w, t, ip0, ip1, ip2, ntrans, N, conj_out, st);
     case 256: return pruned_type1_nf<256>(g, f, w, t, ip0, ip1, ip2, ntrans, N, conj_out, st);
+    case 512: return pruned_type1_nf<512>(g, f, w, t, ip0, ip1, ip2, ntrans, N, conj_out, st);
     default: set_error("pruned FFT: unsupported grid %d", NF); return COV3D_ERR_ARG;
   }
 }

@@ -238,7 +238,7 @@ def test_pipelined_interp_is_bitwise_equal(dev, monkeypatch, ntrans, direct):
     assert outs[0].abs().max() > 0
 
 
-@pytest.mark.parametrize("L,nrun", [(32, 3), (64, 3), (32, 2)])
+@pytest.mark.parametrize("L,nrun", [(32, 3), (64, 3), (32, 2), (256, 1)])
 def test_pruned_fft_matches_rocfft_and_nudft(dev, monkeypatch, L, nrun):
     """The pruned real FFT passes (cubic power-of-two fine grids, the FP32 default there) against
     rocFFT's R2C / C2R path (COV3D_FFT_MODE=0) of the same plan, and against the NUDFT: type 2 of
