@@ -267,20 +267,26 @@ __device__ __forceinline__ int point_bin(const T* x, const T* y, const T* z, int
   return bin_index(g, b[0], b[1], b[2]);
 }
 
-// Bin of every point (the sort key) and its super-bin-relative coordinates, kept as a 4-wide
-// record pv[4j..4j+2] for k_bin_place.  Consecutive points (neighbouring pixels of one image)
-// often share a bin: each run of equal bins among consecutive lanes adds its length with one
-// atomic.
+// Bin of every point, its sort key and its super-bin-relative coordinates, kept as a 4-wide
+// record pv[4j..4j+2] for k_bin_place.  The key is bin * 8 + the point's first stencil cell
+// along axis 0 relative to its bin (ceil(v0 - W/2) - bin start + 4, clamped to [0, 7]): within a
+// bin the sorted points then run in groups of one axis-0 stencil position, so consecutive hits of
+// the matrix-core spreading share their zero axis-0 weights and skip whole cell blocks together.
+// Consecutive points (neighbouring pixels of one image) often share a bin: each run of equal bins
+// among consecutive lanes adds its length with one atomic.
 template <typename T>
 __global__ void k_bin_count(const T* x, const T* y, const T* z, int64_t M, BinGeom g,
-                            int* bin_count, int* pt_bin, T* __restrict__ pv) {
+                            int* bin_count, int* pt_bin, T* __restrict__ pv, int W) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool ok = j < M;
   int bin = -1;
   if (ok) {
     double v[3];
     bin = point_bin(x, y, z, j, g, v);
-    pt_bin[j] = bin;
+    int b0, b1, b2;
+    bin_coords(g, bin, b0, b1, b2);
+    const int xs = (int)ceil(v[0] - 0.5 * W) - kBinEdge * (b0 & 1) + 4;
+    pt_bin[j] = bin * 8 + min(max(xs, 0), 7);
     pv[4 * j] = (T)v[0];
     pv[4 * j + 1] = (T)v[1];
     pv[4 * j + 2] = (T)v[2];
@@ -2109,7 +2115,7 @@ struct Plan {
   int* nsub = nullptr;
   int* pt_bin = nullptr;
   void* pv = nullptr;  // setpts: per-point (coordinates relative to the super-bin, pad), 4 x T
-  int* pt_pos = nullptr;      // sorted bin keys (radix sort output)
+  int* pt_pos = nullptr;      // sorted keys (radix sort output)
   void* sort_tmp = nullptr;   // radix sort work area
   size_t sort_tmp_bytes = 0;
   int* sidx = nullptr;
@@ -2656,10 +2662,11 @@ static int setpts_impl(Plan* p, int64_t M, const T* x, const T* y, const T* z, h
   if (M > 0) {
     dim3 gm((unsigned)ceil_div(M, kBlock));
     hipLaunchKernelGGL((k_bin_count<T>), gm, dim3(kBlock), 0, st, x, y, z, M, p->geom, p->bin_count,
-                       p->pt_bin, (T*)p->pv);
+                       p->pt_bin, (T*)p->pv, p->w);
     COV3D_LAUNCH_CHECK();
     int nbits = 1;
     while ((int64_t)1 << nbits < p->nbins) ++nbits;
+    nbits += 3;  // k_bin_count's key: bin * 8 + axis-0 stencil position
     size_t need = 0;
     COV3D_HIP(sort_points_by_bin(nullptr, need, (const unsigned*)p->pt_bin, (unsigned*)p->pt_pos, p->sidx,
                                  (int)M, nbits, st));
