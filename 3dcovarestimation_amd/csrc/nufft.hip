@@ -1041,9 +1041,8 @@ __global__ void __launch_bounds__(kBlock)
 template <int W>
 __global__ void __launch_bounds__(kBlock)
     k_interp_pipe(const Cplx<float>* __restrict__ fine, Cplx<float>* __restrict__ c, SubDesc sd, BinGeom g,
-                  const float4* __restrict__ irec, const int* __restrict__ iidx, int64_t M, int ntrans, float kc,
-                  float beta, int half_h2, int conj_all, Cplx<float>* __restrict__ direct, int* __restrict__ ctr,
-                  EsPoly poly) {
+                  const float4* __restrict__ irec, const int* __restrict__ iidx, int64_t M, int ntrans,
+                  int half_h2, int conj_all, Cplx<float>* __restrict__ direct, int* __restrict__ ctr, EsPoly poly) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   float4* t4 = reinterpret_cast<float4*>(smem_raw);
   constexpr int E = kSuperEdge + W;
@@ -2236,8 +2235,8 @@ static int launch_interp(Plan* p, void* c_out, int ntrans_run, hipStream_t st) {
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, p->pipe_ctr, (int64_t)8, 0);
         hipLaunchKernelGGL((k_interp_pipe<W>), dim3((unsigned)p->pipe_grid), dim3(kBlock), smem_k, st,
                            (const Cplx<float>*)p->fine, (Cplx<float>*)p->csort, sd, p->geom,
-                           (const float4*)p->irec, (const int*)p->iidx, p->M, ntrans_run, (float)kc,
-                           (float)p->beta, p->stage_half, p->stage_conj,
+                           (const float4*)p->irec, (const int*)p->iidx, p->M, ntrans_run, p->stage_half,
+                           p->stage_conj,
                            direct ? (Cplx<float>*)c_out : nullptr, p->pipe_ctr, p->poly);
         done = true;
       }
