@@ -414,7 +414,7 @@ __global__ void __launch_bounds__(kMlBlock)
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + tid;
   float2* row = s_v + tid * (kGradLdc + 1);
   s_pos[tid] = k < np ? spos[i * np + k] : -1;
-  for (int t = r; t < ldc; ++t) row[t] = make_float2(0.f, 0.f);
+  if (r & 1) row[r] = make_float2(0.f, 0.f);
   if (k < np) {
     float mult;
     const int64_t px = entry_pixel(pix_map, k, L, mult);
@@ -446,7 +446,8 @@ __global__ void __launch_bounds__(kMlBlock)
     }
   }
   __syncthreads();
-  const int hp = ldc >> 1;  // 16-byte pieces per row
+  // 16-byte pieces carrying the r values (the row padding up to ldc is never read)
+  const int hp = (r + 1) >> 1;
   for (int e = tid; e < kMlBlock * hp; e += kMlBlock) {
     const int q = e / hp, h = e - q * hp;
     const int sp = s_pos[q];
