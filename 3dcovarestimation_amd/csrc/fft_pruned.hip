@@ -29,6 +29,7 @@ namespace cov3d {
 
 constexpr int kFftThreads = 256;
 constexpr int kFftLines = 16;
+constexpr int kColGroup = 8;  // strided-pass loads in flight per thread
 
 // In-place unnormalised FFT (sign -1 or +1) of kFftLines lines of length NF in LDS; line l
 // starts at buf[l * LS].  tw[k] = exp(-2 pi i k / NF).  Stockham radix 4 (and 2): stage with
@@ -123,13 +124,29 @@ __global__ void __launch_bounds__(kFftThreads)
     s_buf[l * LS + m] = make_float2(0.f, 0.f);
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < kFftLines * N; e += kFftThreads) {
-    const int l = e / N, i2 = e % N;
-    const int64_t g = g0 + l;
-    if (g < nlines) {
-      const int i1 = (int)(g % N), i0 = (int)((g / N) % N);
-      const float v = g < nlive ? f[g * N + i2] * (ip0[i0] * ip1[i1] * ip2[i2]) : 0.f;
-      s_buf[l * LS + wrap_mode(i2, N, NF)] = make_float2(v, 0.f);
+  // all of a thread's (at most NE) mode loads are issued before any is stored
+  constexpr int NE = (kFftLines * H + kFftThreads - 1) / kFftThreads;
+  float vin[NE];
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    const int e = threadIdx.x + k * kFftThreads;
+    vin[k] = 0.f;
+    if (e < kFftLines * N) {
+      const int l = e / N, i2 = e % N;
+      const int64_t g = g0 + l;
+      if (g < nlines && g < nlive) vin[k] = f[g * N + i2];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    const int e = threadIdx.x + k * kFftThreads;
+    if (e < kFftLines * N) {
+      const int l = e / N, i2 = e % N;
+      const int64_t g = g0 + l;
+      if (g < nlines) {
+        const int i1 = (int)(g % N), i0 = (int)((g / N) % N);
+        s_buf[l * LS + wrap_mode(i2, N, NF)] = make_float2(vin[k] * (ip0[i0] * ip1[i1] * ip2[i2]), 0.f);
+      }
     }
   }
   __syncthreads();
@@ -202,22 +219,44 @@ __global__ void __launch_bounds__(kFftThreads)
     __syncthreads();
   }
   if (col < ncol) {
-    for (int i = rsub; i < nin; i += kFftThreads / kFftLines) {
-      const int m = in_pruned ? wrap_mode(i, N, NF) : i;
-      float2 v;
-      if constexpr (MODE == 2) {
-        // H[k] = (G[k] + conj(G[-k])) / 2 at k = (i, k1, c0 + col), conjugated when conj_out
-        const int64_t t = pos / NF, k1 = pos % NF;
-        const int k2 = c0 + col;
-        const int n0 = i ? NF - i : 0, n1 = k1 ? NF - (int)k1 : 0, n2 = k2 ? NF - k2 : 0;
-        const float2 a = in[in_base + i * in_stride + k2];
-        const float2 b = in[(t * NF * NF + ((int64_t)n0 * NF + n1)) * NF + n2];
-        const float im = 0.5f * (a.y - b.y);
-        v = make_float2(0.5f * (a.x + b.x), conj_out ? -im : im);
-      } else {
-        v = in[in_base + i * in_stride + c0 + col];
+    // rows rsub, rsub + 16, ...: the loads of kColGroup rows are issued before any is stored.
+    // Measured: the passes whose rows are H apart (B, Y) gain (0.40 -> 0.37, 0.39 -> 0.26 ms at
+    // NF = 256), the passes whose rows are NF H apart (C, X) lose (0.93 -> 1.06, 0.98 -> 1.05 ms)
+    // and keep one row per iteration.
+    constexpr int RS = kFftThreads / kFftLines;
+    constexpr int NIT = NF / RS;
+    constexpr int GB = (MODE == 0 || MODE == 3) ? kColGroup : 1;
+    for (int g0 = 0; g0 < NIT; g0 += GB) {
+      float2 va[GB], vb[GB];
+#pragma unroll
+      for (int k = 0; k < GB; ++k) {
+        const int i = rsub + (g0 + k) * RS;
+        if (g0 + k < NIT && i < nin) {
+          if constexpr (MODE == 2) {
+            // H[k] = (G[k] + conj(G[-k])) / 2 at k = (i, k1, c0 + col), conjugated when conj_out
+            const int64_t t = pos / NF, k1 = pos % NF;
+            const int k2 = c0 + col;
+            const int n0 = i ? NF - i : 0, n1 = k1 ? NF - (int)k1 : 0, n2 = k2 ? NF - k2 : 0;
+            va[k] = in[in_base + i * in_stride + k2];
+            vb[k] = in[(t * NF * NF + ((int64_t)n0 * NF + n1)) * NF + n2];
+          } else {
+            va[k] = in[in_base + i * in_stride + c0 + col];
+          }
+        }
       }
-      s_buf[col * LS + m] = v;
+#pragma unroll
+      for (int k = 0; k < GB; ++k) {
+        const int i = rsub + (g0 + k) * RS;
+        if (g0 + k < NIT && i < nin) {
+          const int m = in_pruned ? wrap_mode(i, N, NF) : i;
+          float2 v = va[k];
+          if constexpr (MODE == 2) {
+            const float im = 0.5f * (v.y - vb[k].y);
+            v = make_float2(0.5f * (v.x + vb[k].x), conj_out ? -im : im);
+          }
+          s_buf[col * LS + m] = v;
+        }
+      }
     }
   }
   __syncthreads();
@@ -243,12 +282,24 @@ __global__ void __launch_bounds__(kFftThreads)
   __shared__ float2 s_tw[NF];
   load_twiddles<NF>(s_tw, tw);
   const int64_t g0 = (int64_t)blockIdx.x * kFftLines;
-  for (int e = threadIdx.x; e < kFftLines * H; e += kFftThreads) {
-    const int l = e / H, k = e % H;
-    const int64_t g = g0 + l;
-    const float2 v = g < nlines ? Q[g * H + k] : make_float2(0.f, 0.f);
-    s_buf[l * LS + k] = v;
-    if (k > 0 && k < NF - k) s_buf[l * LS + NF - k] = make_float2(v.x, -v.y);
+  // all of a thread's (at most NE) loads are issued before any is stored
+  constexpr int NE = (kFftLines * H + kFftThreads - 1) / kFftThreads;
+  float2 vq[NE];
+#pragma unroll
+  for (int j = 0; j < NE; ++j) {
+    const int e = threadIdx.x + j * kFftThreads;
+    vq[j] = make_float2(0.f, 0.f);
+    if (e < kFftLines * H && g0 + e / H < nlines) vq[j] = Q[(g0 + e / H) * H + e % H];
+  }
+#pragma unroll
+  for (int j = 0; j < NE; ++j) {
+    const int e = threadIdx.x + j * kFftThreads;
+    if (e < kFftLines * H) {
+      const int l = e / H, k = e % H;
+      const float2 v = vq[j];
+      s_buf[l * LS + k] = v;
+      if (k > 0 && k < NF - k) s_buf[l * LS + NF - k] = make_float2(v.x, -v.y);
+    }
   }
   __syncthreads();
   lds_fft<NF, LS>(s_buf, s_tw, 1.f);
