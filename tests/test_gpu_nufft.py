@@ -214,6 +214,29 @@ def test_lower_accuracy_widths_match_nudft(dev, eps):
     assert rel_l2(out1, ref1) < 10 * eps
 
 
+@pytest.mark.parametrize("eps", [1e-6, 1e-3])
+def test_polynomial_weights_match_nudft(dev, monkeypatch, eps):
+    """COV3D_ES_POLY=1: the FP32 interpolation weights from the degree-8 polynomial form of the ES
+    kernel (odd and even kernel widths) against the NUDFT at the plans' bars, and against the
+    default exp/sqrt form."""
+    o = oracle()
+    npl = pkg("nufft_plan")
+    L, ntrans = 16, 20
+    pts = _points(L, 3, 1, 410, dev, spread=1.2)
+    g = torch.Generator().manual_seed(9)
+    vol = torch.randn((ntrans, L, L, L), generator=g).to(dev)
+    outs = []
+    for flag in ("1", "0"):
+        monkeypatch.setenv("COV3D_ES_POLY", flag)
+        plan = npl.NufftPlan((L,) * 3, batch_size=ntrans, eps=eps, device=dev)
+        plan.setpts(pts)
+        outs.append(plan.execute_forward(vol).reshape(ntrans, -1).cpu().numpy())
+    ref = o.nudft_type2(vol.cpu().numpy().astype(np.complex128), plan.points.cpu().numpy().astype(np.float64))
+    bar = TOL_F32 if eps < 1e-5 else 10 * eps
+    assert rel_l2(outs[0], ref) < bar
+    assert rel_l2(outs[0], outs[1]) < max(1e-6, eps / 10)
+
+
 @pytest.mark.parametrize("ntrans,direct", [(20, "1"), (5, "0")])
 def test_pipelined_interp_is_bitwise_equal(dev, monkeypatch, ntrans, direct):
     """The persistent pipelined interpolation runs the same arithmetic in the same order as the
