@@ -350,13 +350,19 @@ class SyntheticData:
 
 def make_synthetic(cfg: SyntheticConfig, device, n_images: Optional[int] = None, subtract_mean: bool = True,
                    batch: int = 1024, store_filters: bool = True) -> SyntheticData:
-    """Generates images y_i = CTF_i * A_i(mean + sum_k z_ik u_k) + noise on the device.
+    """Generates images y_i = T_{o_i}[CTF_i * A_i(mean + sum_k z_ik u_k)] + noise on the device.
 
-    Projections use the gfx950 type-2 NUFFT (vol_forward on a NufftPlan).
-    With ``subtract_mean`` the projected mean is removed, as the reference's
-    preprocessing does before covariance estimation.
+    Projections use the gfx950 type-2 NUFFT (vol_forward on a NufftPlan).  With
+    ``cfg.shift_px > 0`` each clean image is translated by an offset o_i uniform in
+    [-shift_px, shift_px]^2 (its centred spectrum times ``offset_to_phase_shift(o_i)``), and o_i is
+    stored in ``dataset.offsets``: the reference's preprocessing and mean back-projection multiply
+    by ``offset_to_phase_shift(-o_i)`` to undo it (mean.py:200-204, poses.py:263-264).
+    With ``subtract_mean`` the projected mean is left out of the images, as the reference's
+    preprocessing removes it before covariance estimation; it is left out before the shift, so
+    such a dataset only needs the phase step of ``CovarDataset.preprocess`` (a zero mean volume).
     """
     from .nufft_plan import NufftPlan
+    from .poses import get_phase_shift_grid, offset_to_phase_shift
     from .projection_funcs import vol_forward
 
     dev = torch.device(device)
@@ -380,6 +386,11 @@ def make_synthetic(cfg: SyntheticConfig, device, n_images: Optional[int] = None,
          torch.full((n,), cfg.pixel_size, dtype=torch.float64)], dim=1).to(torch.float32).to(dev)
     latents = (torch.randn((n, r), generator=gen) * eigvals.sqrt()).to(dev) if r > 0 else None
     amp = (1.0 + cfg.amp_jitter * torch.randn(n, generator=gen)).to(dev) if cfg.amp_jitter > 0 else None
+    offsets = None
+    if cfg.shift_px > 0:
+        offsets = ((torch.rand((n, 2), generator=gen, dtype=torch.float64) * 2 - 1) * cfg.shift_px).to(
+            torch.float32).to(dev)
+        shift_grid = get_phase_shift_grid(L, dtype=torch.float32, device=dev)
     vols = torch.cat([mean.unsqueeze(0), eigvecs], dim=0).to(dev)
     plan = NufftPlan((L,) * 3, batch_size=r + 1, device=dev)
     images = torch.empty((n, L, L), dtype=torch.float32, device=dev)
@@ -396,6 +407,8 @@ def make_synthetic(cfg: SyntheticConfig, device, n_images: Optional[int] = None,
             img = img + proj[:, 0]
         if amp is not None:
             img = img * amp[i:j, None, None]
+        if offsets is not None:
+            img = centered_ifft2(centered_fft2(img) * offset_to_phase_shift(offsets[i:j], phase_shift_grid=shift_grid)).real
         images[i:j] = img
         if filters is not None:
             filters[i:j] = f
@@ -405,6 +418,6 @@ def make_synthetic(cfg: SyntheticConfig, device, n_images: Optional[int] = None,
     for i in range(0, n, 8 * batch):
         j = min(i + 8 * batch, n)
         images[i:j] += math.sqrt(noise_var) * torch.randn((j - i, L, L), device=dev, generator=g)
-    ds = CovarDataset(images, rots, noise_var, filters=filters, ctf_params=ctf_params)
+    ds = CovarDataset(images, rots, noise_var, filters=filters, ctf_params=ctf_params, offsets=offsets)
     return SyntheticData(ds, mean.to(dev), eigvecs.to(dev), eigvals.to(dev), latents,
                          extra={} if amp is None else {"amplitudes": amp})

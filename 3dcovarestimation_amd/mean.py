@@ -155,8 +155,13 @@ def _sphere_mask(L: int, device) -> torch.Tensor:
 
 
 def backproject_lhs_rhs(dataset, idx=None, upsampling_factor: int = 2, batch_size: int = 1024):
-    """rhs = sum im_backward(images), lhs = sum im_backward(CTF)/L, both (us*L)^3 (mean.py:185-216)."""
+    """rhs = sum im_backward(images), lhs = sum im_backward(CTF)/L, both (us*L)^3 (mean.py:185-216).
+
+    Each image is first multiplied by the phase ramp of its negated stored offset
+    (mean.py:200-204), which undoes the particle's in-plane translation; contrasts scale the CTF.
+    """
     from .dataset import create_dataloader
+    from .poses import get_phase_shift_grid, offset_to_phase_shift
 
     L = dataset.resolution
     plan = NufftPlanDiscretized((L,) * 3, upsample_factor=upsampling_factor, mode="nearest")
@@ -166,8 +171,11 @@ def backproject_lhs_rhs(dataset, idx=None, upsampling_factor: int = 2, batch_siz
     vL = L * upsampling_factor
     rhs = torch.zeros((vL,) * 3, dtype=torch.complex64, device=dev)
     lhs = torch.zeros((vL,) * 3, dtype=torch.complex64, device=dev)
+    grid = get_phase_shift_grid(L, dtype=dataset.dtype, device=dev)
     for images, pts_rot, filters, bidx in create_dataloader(dataset, batch_size, idx):
         filters = filters * dataset.contrasts[bidx].reshape(-1, 1, 1)
+        offsets = dataset.offsets[bidx].to(device=dev, dtype=dataset.dtype).reshape(-1, 2)
+        images = images * offset_to_phase_shift(-offsets, phase_shift_grid=grid).reshape(images.shape)
         plan.setpts(pts_rot)
         rhs += im_backward(images, plan, filters, fourier_domain=True)[0]
         lhs += im_backward(torch.complex(filters, torch.zeros_like(filters)), plan, filters, fourier_domain=True)[0]

@@ -50,6 +50,10 @@ def _load():
             fn = getattr(lib, name)
             fn.argtypes = [ctypes.c_int] * 3 + [ctypes.c_int64, dp, dp, dp, dp, ctypes.c_int, ctypes.c_int, dp]
             fn.restype = None
+        fn = lib.oracle_nudft3d_type1_modes
+        fn.argtypes = [ctypes.c_int] * 3 + [ctypes.c_int64, dp, dp, dp, dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                            ctypes.POINTER(ctypes.c_int64), dp]
+        fn.restype = None
         for name in ("oracle_es_interp", "oracle_es_spread"):
             fn = getattr(lib, name)
             fn.argtypes = [ctypes.c_int] * 3 + [ctypes.c_int64, dp, dp, dp, dp, ctypes.c_int, ctypes.c_int,
@@ -104,6 +108,84 @@ def nudft_type1(c: np.ndarray, pts: np.ndarray, N, iflag: int = 1) -> np.ndarray
     _load().oracle_nudft3d_type1(N0, N1, N2, M, _dp(x), _dp(y), _dp(z), _dp(c.view(np.float64)), nt, iflag,
                                  _dp(f.view(np.float64)))
     return f
+
+
+def nudft_type1_modes(c: np.ndarray, pts: np.ndarray, N, modes: np.ndarray, iflag: int = 1) -> np.ndarray:
+    """Type 1 evaluated only at the flat output indices ``modes`` (C order over N):
+    c (ntrans, M), pts (3, M) -> (ntrans, len(modes)) complex128; equals
+    ``nudft_type1(c, pts, N).reshape(ntrans, -1)[:, modes]``."""
+    c = np.ascontiguousarray(c, dtype=np.complex128)
+    if c.ndim == 1:
+        c = c[None]
+    nt, M = c.shape
+    N0, N1, N2 = N
+    modes = np.ascontiguousarray(modes, dtype=np.int64)
+    pts = np.ascontiguousarray(pts, dtype=np.float64)
+    out = np.zeros((nt, modes.size), dtype=np.complex128)
+    x, y, z = (np.ascontiguousarray(pts[k]) for k in range(3))
+    _load().oracle_nudft3d_type1_modes(N0, N1, N2, M, _dp(x), _dp(y), _dp(z), _dp(c.view(np.float64)), nt, iflag,
+                                       modes.size, modes.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                       _dp(out.view(np.float64)))
+    return out
+
+
+def lattice_points(k: np.ndarray, delta: np.ndarray, N: int) -> np.ndarray:
+    """Points 2 pi (k + delta) / N of one shifted integer lattice: k (3, M) integers, delta (3,)."""
+    return 2 * np.pi * (np.asarray(k, dtype=np.float64) + np.asarray(delta, dtype=np.float64)[:, None]) / N
+
+
+def _fft3(x: np.ndarray, inverse: bool) -> np.ndarray:
+    """Unnormalised 3-D DFT over the last three axes, sign - (forward) or + (inverse), FP64."""
+    import scipy.fft
+
+    if inverse:
+        return scipy.fft.ifftn(x, axes=(-3, -2, -1), norm="forward", workers=os.cpu_count())
+    return scipy.fft.fftn(x, axes=(-3, -2, -1), workers=os.cpu_count())
+
+
+def _mode_phase(delta: np.ndarray, N: int, iflag: int) -> np.ndarray:
+    """exp(iflag * i * 2 pi n . delta / N) on the centred (N, N, N) mode grid."""
+    n = np.arange(N) - N // 2
+    ph = [np.exp(1j * iflag * 2 * np.pi * n * float(d) / N) for d in delta]
+    return ph[0][:, None, None] * ph[1][None, :, None] * ph[2][None, None, :]
+
+
+def lattice_type2(f: np.ndarray, k: np.ndarray, delta: np.ndarray, iflag: int = -1) -> np.ndarray:
+    """Exact type 2 at the points ``lattice_points(k, delta, N)``, by FFT instead of a direct sum.
+
+    With p = 2 pi (k + delta) / N, sum_n f[n] e^{iflag i n.p} = sum_n U[n] e^{iflag 2 pi i n.k / N},
+    U = f * exp(iflag i 2 pi n.delta / N): one length-N 3-D DFT of U (modes centred, so shifted to
+    index n mod N first) read at k mod N.  f (ntrans, N, N, N) -> (ntrans, M) complex128.
+    """
+    f = np.asarray(f, dtype=np.complex128)
+    if f.ndim == 3:
+        f = f[None]
+    N = f.shape[-1]
+    U = np.fft.ifftshift(f * _mode_phase(delta, N, iflag)[None], axes=(1, 2, 3))
+    X = _fft3(U, inverse=iflag > 0)
+    km = np.mod(np.asarray(k, dtype=np.int64), N)
+    return X[:, km[0], km[1], km[2]]
+
+
+def lattice_type1(c: np.ndarray, k: np.ndarray, delta: np.ndarray, N: int, iflag: int = 1) -> np.ndarray:
+    """Exact type 1 from the points ``lattice_points(k, delta, N)`` by FFT.
+
+    sum_j c_j e^{iflag i n.p_j} = exp(iflag i 2 pi n.delta / N) sum_m A[m] e^{iflag 2 pi i n.m / N},
+    A[m] = sum of c_j over the points with k_j = m (mod N).  c (ntrans, M) -> (ntrans, N, N, N).
+    """
+    c = np.asarray(c, dtype=np.complex128)
+    if c.ndim == 1:
+        c = c[None]
+    nt = c.shape[0]
+    km = np.mod(np.asarray(k, dtype=np.int64), N)
+    flat = (km[0] * N + km[1]) * N + km[2]
+    A = np.zeros((nt, N**3), dtype=np.complex128)
+    for t in range(nt):
+        A[t] = np.bincount(flat, weights=c[t].real, minlength=N**3) + 1j * np.bincount(
+            flat, weights=c[t].imag, minlength=N**3)
+    A = A.reshape(nt, N, N, N)
+    X = _fft3(A, inverse=iflag > 0)
+    return np.fft.fftshift(X, axes=(1, 2, 3)) * _mode_phase(delta, N, iflag)[None]
 
 
 def nudft_type2_bruteforce(f: np.ndarray, pts: np.ndarray, iflag: int = -1) -> np.ndarray:

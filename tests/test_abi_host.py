@@ -78,3 +78,25 @@ def test_centered_fft_pad_crop_identity(L, pad):
     y = pf.centered_ifft3(pf.centered_fft3(x, padding_size=(pad,) * 3), cropping_size=(L,) * 3)
     assert torch.allclose(y.real, x, atol=1e-12)
     assert torch.allclose(pf.crop_tensor(pf.pad_tensor(x, [pad] * 3), [L] * 3), x)
+
+
+@pytest.mark.parametrize("dtype,code", [(np.complex64, 0), (np.dtype("complex64"), 0), ("complex64", 0),
+                                        (torch.complex64, 0), (np.float32, 0), (np.complex128, 1),
+                                        (np.dtype("complex128"), 1), ("complex128", 1), (torch.complex128, 1),
+                                        (torch.float64, 1)])
+def test_reference_side_stub_dtype_mapping(dtype, code):
+    """The cufinufft.Plan stand-in maps every spelling of the dtype the reference passes
+    (np.complex64 at nufft_plan.py:299-316) to the library's dtype code, and refuses others."""
+    import importlib
+
+    stub = importlib.import_module("tests.finufft_stub")
+    assert stub.dtype_code(dtype) == code
+
+
+@pytest.mark.parametrize("bad", [np.int32, "complex32", torch.int64, object])
+def test_reference_side_stub_refuses_unknown_dtypes(bad):
+    import importlib
+
+    stub = importlib.import_module("tests.finufft_stub")
+    with pytest.raises(ValueError):
+        stub.dtype_code(bad)

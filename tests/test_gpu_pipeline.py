@@ -180,3 +180,98 @@ def test_mean_reconstruction_recovers_volume(dev):
     # nearest-neighbour back-projection on the 2x grid; the reference accepts 2e-2 at L=128 (its
     # test_reconstruction.py NUFFT case) and the discretisation error shrinks with L (0.10 at L=16)
     assert err < 8e-2
+
+
+def test_mean_reconstruction_undoes_offsets(dev):
+    """Images translated by stored offsets (U[-2, 2] px, the config-2 shifts) give the same mean as
+    the untranslated images: the back-projection multiplies each image by the phase ramp of its
+    negated offset (reference mean.py:200-204).  Ignoring the offsets gives a visibly wrong mean,
+    so the check is sensitive to the step."""
+    ds = pkg("dataset")
+    mean = pkg("mean")
+    base = dict(L=32, n_images=4000, rank=0, eigenvalues=[], pixel_size=3.0, snr=1e6, defocus_um=(1.0, 2.5), seed=23)
+    ref = ds.make_synthetic(ds.SyntheticConfig(**base), dev, subtract_mean=False)
+    shifted = ds.make_synthetic(ds.SyntheticConfig(**base, shift_px=2.0), dev, subtract_mean=False)
+    off = shifted.dataset.offsets
+    assert float(off.abs().max()) > 1.5 and float(off.abs().max()) <= 2.0
+    assert float((shifted.dataset.images - ref.dataset.images).norm() / ref.dataset.images.norm()) > 0.3
+    v_ref = mean.reconstruct_mean(ref.dataset, batch_size=1024)
+    v_shift = mean.reconstruct_mean(shifted.dataset, batch_size=1024)
+    err = float((v_shift - v_ref).norm() / v_ref.norm())
+    shifted.dataset.offsets = torch.zeros_like(off)
+    v_blind = mean.reconstruct_mean(shifted.dataset, batch_size=1024)
+    blind = float((v_blind - v_ref).norm() / v_ref.norm())
+    print(f"shifted vs unshifted mean: {err:.2e}; offsets ignored: {blind:.2e}")
+    # both solves stop at the CG's relative residual 1e-6 on a system whose weakly sampled corner
+    # frequencies amplify that residual ~100x in the solution; ignoring the offsets costs 20%
+    assert err < 1e-3
+    assert blind > 1e-1
+
+
+def test_halfset_mean_on_noiseless_data(dev):
+    """reconstruct_mean_from_halfsets (mean.py:262-291, :359-421) on noiseless shifted images: the
+    FSC of the two halves is ~1, so the shell regulariser vanishes and the result equals the
+    full-set reconstruction and the ground truth to the accuracy of the nearest-neighbour grid."""
+    ds = pkg("dataset")
+    mean = pkg("mean")
+    pf = pkg("projection_funcs")
+    cfg = ds.SyntheticConfig(L=32, n_images=8000, rank=0, eigenvalues=[], pixel_size=3.0, snr=1e6,
+                             defocus_um=(1.0, 2.5), shift_px=1.0, seed=24)
+    data = ds.make_synthetic(cfg, dev, subtract_mean=False)
+    full = mean.reconstruct_mean(data.dataset, batch_size=1024)
+    halves = mean.reconstruct_mean_from_halfsets(data.dataset, batch_size=1024)
+    gt = pf.centered_ifft3(pf.centered_fft3(data.mean) * mean._sphere_mask(32, dev)).real
+    d_full = float((halves - full).norm() / full.norm())
+    d_gt = float((halves - gt).norm() / gt.norm())
+    print(f"half-set mean vs full-set mean {d_full:.2e}, vs ground truth {d_gt:.2e}")
+    assert d_full < 1e-2
+    assert d_gt < 8e-2
+
+
+def test_config2_shifted_batch_gradient(dev):
+    """Config 2 (L=64, r=10, shifts U[-2, 2] px, SNR 0.05): images with their projected mean and
+    translations are preprocessed (phase shifts of the stored offsets, mean subtraction), then the
+    chosen form's gradient on a 64-image batch is checked against the FP64 autograd gradient and
+    against central finite differences of the FP64 cost (step 1e-3) along random directions."""
+    ds = pkg("dataset")
+    npl = pkg("nufft_plan")
+    sgd = pkg("covar_sgd")
+    cfg = ds.config_by_name("L64")
+    assert cfg.shift_px == 2.0 and cfg.rank == 10
+    B = 64
+    data = ds.make_synthetic(cfg, dev, n_images=B, subtract_mean=False, batch=B)
+    data.dataset.preprocess(data.mean)
+    assert float(data.dataset.offsets.abs().max()) == 0.0
+    images, pts, ctf, _ = data.dataset[0:B]
+    nv = data.dataset.noise_var
+    L, r = cfg.L, cfg.rank
+    vols = (data.eigenvecs * data.eigenvals.sqrt().reshape(-1, 1, 1, 1)).float()
+    vols = vols + 0.3 * vols.std() * torch.randn(vols.shape, generator=torch.Generator().manual_seed(8)).to(dev)
+    plan = npl.NufftPlan((L,) * 3, batch_size=r, device=dev)
+    plan.setpts(pts)
+    v = vols.clone().requires_grad_(True)
+    cost = sgd.cost_maximum_liklihood(v, images, plan, ctf, nv, form="fused")
+    cost.backward()
+    p64 = npl.NufftPlan((L,) * 3, batch_size=r, eps=1e-12, dtype=torch.float64, device=dev)
+    p64.setpts(pts.double())
+
+    def c64(x):
+        return sgd.cost_maximum_liklihood(x, images.double(), p64, ctf.double(), nv, form="autograd")
+
+    v64 = vols.double().clone().requires_grad_(True)
+    ref = c64(v64)
+    ref.backward()
+    err_c = abs(cost.item() - ref.item()) / abs(ref.item())
+    err_g = rel_l2(v.grad.cpu().numpy(), v64.grad.cpu().numpy())
+    print(f"config 2 batch: cost {err_c:.2e}, gradient {err_g:.2e} vs FP64 autograd")
+    assert err_c < 1e-3 and err_g < 2e-3
+    g = torch.Generator(device=dev).manual_seed(9)
+    h = 1e-3
+    with torch.no_grad():
+        for _ in range(3):
+            d = torch.randn(vols.shape, device=dev, dtype=torch.float64, generator=g)
+            d = d / d.norm() * float(vols.norm())
+            fd = (c64(vols.double() + h * d).item() - c64(vols.double() - h * d).item()) / (2 * h)
+            an = float((v.grad.double() * d).sum())
+            print(f"directional derivative: analytic {an:.6e}, finite difference {fd:.6e}")
+            assert abs(an - fd) < 2e-3 * abs(fd) + 1e-9

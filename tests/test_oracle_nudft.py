@@ -74,3 +74,26 @@ def test_vol_forward_im_backward_adjoint():
     lhs = np.sum(o.vol_forward_nudft(V, pts, F)[:, 0] * y)
     rhs = np.sum(V[0] * o.im_backward_nudft(y, pts, F))
     assert abs(lhs - rhs) < 1e-9 * abs(lhs)
+
+
+@pytest.mark.parametrize("N", [8, 9, 10])
+@pytest.mark.parametrize("iflag", [-1, 1])
+def test_lattice_closed_forms_match_direct_sums(N, iflag):
+    """The FFT closed forms for points on a shifted integer lattice, p = 2 pi (k + delta) / N (used
+    at sizes where direct sums are too slow), against the direct sums of both types and the
+    explicit exp matrix; plus the mode-subset type 1 against the full one."""
+    o = oracle()
+    rng = np.random.default_rng(N + 10 * (iflag + 1))
+    f = rng.standard_normal((2, N, N, N)) + 1j * rng.standard_normal((2, N, N, N))
+    k = rng.integers(-(N // 2), N - N // 2, (3, 60))
+    k[:, :5] = k[:, 5:10]  # repeated lattice sites accumulate in type 1
+    delta = rng.uniform(0, 1, 3)
+    P = o.lattice_points(k, delta, N)
+    ref2 = o.nudft_type2_bruteforce(f, P, iflag)
+    assert rel_l2(o.lattice_type2(f, k, delta, iflag), ref2) < 1e-12
+    assert rel_l2(o.nudft_type2(f, P, iflag), ref2) < 1e-12
+    c = rng.standard_normal((2, 60)) + 1j * rng.standard_normal((2, 60))
+    ref1 = o.nudft_type1(c, P, (N, N, N), iflag)
+    assert rel_l2(o.lattice_type1(c, k, delta, N, iflag), ref1) < 1e-12
+    modes = rng.choice(N**3, 23, replace=False)
+    assert rel_l2(o.nudft_type1_modes(c, P, (N, N, N), modes, iflag), ref1.reshape(2, -1)[:, modes]) < 1e-12
