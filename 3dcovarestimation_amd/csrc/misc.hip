@@ -1,8 +1,7 @@
 // Data-feed helpers (rotated slice frequencies, CTF), FP64 complex dot
-// products and the Toeplitz normal operator of the mean-volume CG, plus the
-// small MLP of the learned CG preconditioner.
+// products of the mean-volume CG, and the small MLP of the learned CG
+// preconditioner.  The CG's Toeplitz operator lives in toeplitz.hip.
 #include <math.h>
-#include <rocfft/rocfft.h>
 
 #include <map>
 #include <mutex>
@@ -121,102 +120,6 @@ __global__ void __launch_bounds__(kMiscBlock)
     atomicAdd(out, sr[0]);
     atomicAdd(out + 1, si[0]);
   }
-}
-
-// ----------------------------------------------------------------------------
-// Toeplitz operator y = crop_L(ifft3c(lhs * fft3c(pad_nf(x)))) with rocFFT.
-// Index maps (per axis): pad  u[(i - L/2) mod nf] = x[i];
-//                        mult U[k] *= lhs[(k + nf/2) mod nf];
-//                        crop y[i] = V[(i - L/2) mod nf] / nf^3.
-// ----------------------------------------------------------------------------
-__global__ void k_tpl_pad(const float2* __restrict__ x, float2* __restrict__ u, int L, int nf) {
-  const int64_t n3 = (int64_t)nf * nf * nf;
-  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= n3) return;
-  int j2 = (int)(idx % nf), j1 = (int)((idx / nf) % nf), j0 = (int)(idx / ((int64_t)nf * nf));
-  auto src = [&](int j) -> int {
-    int s = j < nf - L / 2 ? j : j - nf;  // signed offset
-    int i = s + L / 2;
-    return (i >= 0 && i < L) ? i : -1;
-  };
-  int i0 = src(j0), i1 = src(j1), i2 = src(j2);
-  float2 v = make_float2(0.f, 0.f);
-  if (i0 >= 0 && i1 >= 0 && i2 >= 0) v = x[((int64_t)i0 * L + i1) * L + i2];
-  u[idx] = v;
-}
-
-__global__ void k_tpl_mult(float2* __restrict__ u, const float2* __restrict__ lhs, int nf) {
-  const int64_t n3 = (int64_t)nf * nf * nf;
-  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= n3) return;
-  int k2 = (int)(idx % nf), k1 = (int)((idx / nf) % nf), k0 = (int)(idx / ((int64_t)nf * nf));
-  int c0 = (k0 + nf / 2) % nf, c1 = (k1 + nf / 2) % nf, c2 = (k2 + nf / 2) % nf;
-  float2 a = u[idx], b = lhs[((int64_t)c0 * nf + c1) * nf + c2];
-  u[idx] = make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-
-__global__ void k_tpl_crop(const float2* __restrict__ v, float2* __restrict__ y, int L, int nf,
-                           float scale) {
-  const int64_t n3 = (int64_t)L * L * L;
-  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= n3) return;
-  int i2 = (int)(idx % L), i1 = (int)((idx / L) % L), i0 = (int)(idx / ((int64_t)L * L));
-  auto dst = [&](int i) -> int {
-    int s = i - L / 2;
-    return s >= 0 ? s : s + nf;
-  };
-  float2 a = v[((int64_t)dst(i0) * nf + dst(i1)) * nf + dst(i2)];
-  y[idx] = make_float2(a.x * scale, a.y * scale);
-}
-
-struct TplCtx {
-  rocfft_plan fwd = nullptr, bwd = nullptr;
-  rocfft_execution_info info = nullptr;
-  void* work = nullptr;
-  void* buf = nullptr;
-};
-
-static std::mutex g_tpl_mu;
-static std::map<std::pair<int64_t, int64_t>, TplCtx*> g_tpl;
-static std::once_flag g_rocfft_once_misc;
-
-static int tpl_get(int64_t L, int64_t nf, TplCtx** out) {
-  std::lock_guard<std::mutex> lk(g_tpl_mu);
-  auto it = g_tpl.find({L, nf});
-  if (it != g_tpl.end()) {
-    *out = it->second;
-    return 0;
-  }
-  std::call_once(g_rocfft_once_misc, []() { rocfft_setup(); });
-  TplCtx* c = new TplCtx();
-  size_t len[3] = {(size_t)nf, (size_t)nf, (size_t)nf};
-  if (rocfft_plan_create(&c->fwd, rocfft_placement_inplace, rocfft_transform_type_complex_forward,
-                         rocfft_precision_single, 3, len, 1, nullptr) != rocfft_status_success ||
-      rocfft_plan_create(&c->bwd, rocfft_placement_inplace, rocfft_transform_type_complex_inverse,
-                         rocfft_precision_single, 3, len, 1, nullptr) != rocfft_status_success ||
-      rocfft_execution_info_create(&c->info) != rocfft_status_success) {
-    set_error("rocfft plan creation failed for Toeplitz operator nf=%lld", (long long)nf);
-    delete c;
-    return COV3D_ERR_FFT;
-  }
-  size_t w1 = 0, w2 = 0;
-  rocfft_plan_get_work_buffer_size(c->fwd, &w1);
-  rocfft_plan_get_work_buffer_size(c->bwd, &w2);
-  size_t ws = w1 > w2 ? w1 : w2;
-  if (ws) {
-    if (hipMalloc(&c->work, ws) != hipSuccess) {
-      set_error("hipMalloc of rocFFT work buffer failed");
-      return COV3D_ERR_HIP;
-    }
-    rocfft_execution_info_set_work_buffer(c->info, c->work, ws);
-  }
-  if (hipMalloc(&c->buf, (size_t)nf * nf * nf * 8) != hipSuccess) {
-    set_error("hipMalloc of Toeplitz buffer failed");
-    return COV3D_ERR_HIP;
-  }
-  g_tpl[{L, nf}] = c;
-  *out = c;
-  return 0;
 }
 
 // ----------------------------------------------------------------------------
@@ -414,39 +317,6 @@ extern "C" int cov3d_vdot_c64_f64(const void* x, const void* y, int64_t n, void*
   if (nb > 2048) nb = 2048;
   hipLaunchKernelGGL(cov3d::k_vdot, dim3((unsigned)nb), dim3(cov3d::kMiscBlock), 0, st,
                      (const float2*)x, (const float2*)y, n, (double*)out);
-  COV3D_LAUNCH_CHECK();
-  return 0;
-}
-
-extern "C" int cov3d_toeplitz_apply(const void* x, const void* lhs, void* y, int64_t L, int64_t nf,
-                                    void* stream) {
-  COV3D_CHECK_ARG(L >= 1 && nf >= L, "need nf >= L >= 1");
-  COV3D_CHECK_ARG(x && lhs && y, "NULL pointer");
-  cov3d::TplCtx* c = nullptr;
-  int rc = cov3d::tpl_get(L, nf, &c);
-  if (rc) return rc;
-  hipStream_t st = cov3d::as_stream(stream);
-  const int64_t n3 = nf * nf * nf, l3 = L * L * L;
-  const int blk = cov3d::kMiscBlock;
-  hipLaunchKernelGGL(cov3d::k_tpl_pad, dim3((unsigned)cov3d::ceil_div(n3, blk)), dim3(blk), 0, st,
-                     (const float2*)x, (float2*)c->buf, (int)L, (int)nf);
-  COV3D_LAUNCH_CHECK();
-  void* bufs[1] = {c->buf};
-  if (rocfft_execution_info_set_stream(c->info, st) != rocfft_status_success ||
-      rocfft_execute(c->fwd, bufs, nullptr, c->info) != rocfft_status_success) {
-    cov3d::set_error("rocfft forward failed in Toeplitz operator");
-    return COV3D_ERR_FFT;
-  }
-  hipLaunchKernelGGL(cov3d::k_tpl_mult, dim3((unsigned)cov3d::ceil_div(n3, blk)), dim3(blk), 0, st,
-                     (float2*)c->buf, (const float2*)lhs, (int)nf);
-  COV3D_LAUNCH_CHECK();
-  if (rocfft_execute(c->bwd, bufs, nullptr, c->info) != rocfft_status_success) {
-    cov3d::set_error("rocfft inverse failed in Toeplitz operator");
-    return COV3D_ERR_FFT;
-  }
-  hipLaunchKernelGGL(cov3d::k_tpl_crop, dim3((unsigned)cov3d::ceil_div(l3, blk)), dim3(blk), 0, st,
-                     (const float2*)c->buf, (float2*)y, (int)L, (int)nf,
-                     (float)(1.0 / (double)n3));
   COV3D_LAUNCH_CHECK();
   return 0;
 }

@@ -64,7 +64,8 @@ def toeplitz_apply(x: torch.Tensor, lhs: torch.Tensor, L: int) -> torch.Tensor:
 
 @dataclass
 class CGInfo:
-    iterations: int = 0
+    iterations: int = 0  # first iteration whose residual met the tolerance (the reference's count)
+    iterations_run: int = 0  # iterations executed (up to check_every - 1 more)
     residuals: List[float] = field(default_factory=list)
     preconditioned: bool = False
     fallback: Optional[str] = None
@@ -74,15 +75,17 @@ class CGInfo:
 def complex_conjugate_gradient(A_mv: Callable, b: torch.Tensor, x0: Optional[torch.Tensor] = None,
                                rtol: float = 1e-6, maxiter: Optional[int] = None,
                                precond: Optional[Callable] = None, return_info: bool = False,
-                               stall_window: int = 1000):
+                               stall_window: int = 1000, check_every: int = 16):
     """(Preconditioned) CG for Hermitian positive-definite A (mean.py:43-93).
 
-    Stops when the recursively updated residual satisfies ||r|| < rtol ||b||,
-    as the reference does; the true residual ||b - A x|| / ||b|| is recomputed
-    at the end (``CGInfo.final_residual``).  alpha and beta are FP64 device
-    scalars.  If ``precond`` stagnates (the best residual improves by less than
-    10% over ``stall_window`` iterations) plain CG resumes from the current
-    iterate.
+    Stops when the recursively updated residual satisfies ||r|| < rtol ||b||, as the reference
+    does; the true residual ||b - A x|| / ||b|| is recomputed at the end
+    (``CGInfo.final_residual``).  alpha and beta are FP64 device scalars and stay on the device:
+    the residual norms are read back every ``check_every`` iterations, so the host never waits for
+    the device inside a block and the solve may run up to ``check_every - 1`` iterations past the
+    first one that meets the tolerance (``CGInfo.residuals`` records every iteration).  If
+    ``precond`` stagnates (the best residual improves by less than 10% over ``stall_window``
+    iterations) plain CG resumes from the current iterate.
     """
     if not b.is_complex():
         raise ValueError("Input vector b must be a complex tensor.")
@@ -94,38 +97,44 @@ def complex_conjugate_gradient(A_mv: Callable, b: torch.Tensor, x0: Optional[tor
     z = precond(r) if precond is not None else r
     p = z.clone()
     rz = vdot_f64(r, z).real
-    rr = vdot_f64(r, r).real.item()
-    info.residuals.append(math.sqrt(rr) / b_norm)
+    info.residuals.append(math.sqrt(vdot_f64(r, r).real.item()) / b_norm)
     best = [info.residuals[0]]  # best residual so far, per iteration
+    done = info.residuals[0] < rtol
+    pending = []
     it = 0
-    while it < maxiter:
-        if math.sqrt(rr) < rtol * b_norm:
-            break
+    while not done and it < maxiter:
         Ap = A_mv(p)
-        alpha = rz / vdot_f64(p, Ap)
-        a32 = alpha.to(torch.complex64)
-        x = x + a32 * p
-        r = r - a32 * Ap
-        rr = vdot_f64(r, r).real.item()
+        alpha = (rz / vdot_f64(p, Ap)).to(torch.complex64)
+        x.add_(alpha * p)
+        r.sub_(alpha * Ap)
+        rr = vdot_f64(r, r).real
+        pending.append(rr)
         it += 1
-        info.residuals.append(math.sqrt(rr) / b_norm)
-        best.append(min(best[-1], info.residuals[-1]))
-        if precond is not None and it > stall_window and best[-1] > 0.9 * best[-1 - stall_window]:
-            info.fallback = "stall"
-            precond = None
+        if precond is not None:
+            z = precond(r)
+            rz_new = vdot_f64(r, z).real
+        else:
             z = r
-            p = r.clone()
-            rz = vdot_f64(r, r).real
-            continue
-        z = precond(r) if precond is not None else r
-        rz_new = vdot_f64(r, z).real
-        beta = (rz_new / rz).to(torch.complex64)
-        p = z + beta * p
+            rz_new = rr
+        p.mul_((rz_new / rz).to(torch.complex64)).add_(z)
         rz = rz_new
-    info.iterations = it
+        if len(pending) == check_every or it >= maxiter:
+            for v in torch.stack(pending).cpu().tolist():
+                info.residuals.append(math.sqrt(max(v, 0.0)) / b_norm)
+                best.append(min(best[-1], info.residuals[-1]))
+                done = done or info.residuals[-1] < rtol
+            pending = []
+            if not done and precond is not None and it > stall_window and best[-1] > 0.9 * best[-1 - stall_window]:
+                info.fallback = "stall"
+                precond = None
+                z = r
+                p = r.clone()
+                rz = vdot_f64(r, r).real
+    info.iterations_run = it
+    info.iterations = next((i for i, v in enumerate(info.residuals) if v < rtol), it)
     res = b - A_mv(x)
     info.final_residual = math.sqrt(vdot_f64(res, res).real.item()) / b_norm
-    logger.debug(f"CG converged in {it} iterations")
+    logger.debug(f"CG converged in {info.iterations} iterations")
     return (x, info) if return_info else x
 
 

@@ -374,10 +374,10 @@ def cg_comparison(reg_lhs: torch.Tensor, rhs: torch.Tensor, L: int, model: Optio
             if best is None or dt < best[0]:
                 best = (dt, info, vol, pre is not None)
         dt, info, vol, used = best
-        ms_it = 1e3 * dt / max(info.iterations, 1)
+        ms_it = 1e3 * dt / max(info.iterations_run, 1)
         reached = {f"{t:.0e}": next((i for i, rr in enumerate(info.residuals) if rr < t), None)
                    for t in (1e-3, 1e-4, 1e-5, 1e-6)}
-        return {"iterations": info.iterations, "converged": info.residuals[-1] < rtol, "seconds": round(dt, 4),
+        return {"iterations": info.iterations, "iterations_run": info.iterations_run, "converged": min(info.residuals) < rtol, "seconds": round(dt, 4),
                 "ms_per_iteration": round(ms_it, 4), "final_residual": info.final_residual,
                 "iterations_to": reached, "preconditioned": used, "fallback": info.fallback}, vol
 

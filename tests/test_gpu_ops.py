@@ -240,3 +240,43 @@ def test_mlp_forward_backward(dev):
     ref_y = pc.mlp_torch_reference(p64, x.double().cpu())
     (ref_y * gy.double().cpu()).sum().backward()
     assert rel_l2(grad.cpu().numpy(), p64.grad.numpy()) < 1e-4
+
+
+@pytest.mark.parametrize("L,nf", [(7, 64), (32, 64), (48, 128), (64, 128), (128, 256), (40, 96)])
+def test_toeplitz_operator_matches_oracle(dev, L, nf):
+    """cov3d_toeplitz_apply against the FP64 oracle: the pruned passes on 64/128/256-cell grids
+    (odd and even L, L up to nf/2) and the rocFFT path on other grids (nf = 96)."""
+    o = oracle()
+    mean = pkg("mean")
+    rng = np.random.default_rng(L + nf)
+    lhs = (rng.standard_normal((nf,) * 3) + 1j * rng.standard_normal((nf,) * 3)).astype(np.complex64)
+    x = (rng.standard_normal(L**3) + 1j * rng.standard_normal(L**3)).astype(np.complex64)
+    out = mean.toeplitz_apply(torch.tensor(x, device=dev), torch.tensor(lhs, device=dev), L)
+    ref = o.toeplitz_apply(x.astype(np.complex128), lhs.astype(np.complex128), L).ravel()
+    err = rel_l2(out.cpu().numpy(), ref)
+    print(f"Toeplitz L={L} nf={nf}: {err:.2e}")
+    assert err < 1e-5
+
+
+@pytest.mark.parametrize("L,nf", [(16, 512), (200, 512), (128, 256)])
+def test_toeplitz_operator_closed_forms(dev, L, nf):
+    """Sizes where the FP64 oracle is too slow (512-cell grids): a constant lhs is a multiple of the
+    identity and the centred phase ramp exp(-2 pi i k.s / nf) translates x by s with zero fill."""
+    mean = pkg("mean")
+    g = torch.Generator(device=dev).manual_seed(L)
+    x = torch.complex(torch.randn(L**3, device=dev, generator=g), torch.randn(L**3, device=dev, generator=g))
+    y = mean.toeplitz_apply(x, torch.full((nf,) * 3, 1.5 + 0j, dtype=torch.complex64, device=dev), L)
+    assert rel_l2(y.cpu().numpy(), 1.5 * x.cpu().numpy()) < 1e-6
+    k = torch.arange(nf, device=dev, dtype=torch.float64) - nf // 2
+    s = (2, -1, 3)
+    ph = -2 * np.pi * (k[:, None, None] * s[0] + k[None, :, None] * s[1] + k[None, None, :] * s[2]) / nf
+    ramp = torch.polar(torch.ones_like(ph), ph).to(torch.complex64)
+    y = mean.toeplitz_apply(x, ramp, L).reshape(L, L, L).cpu().numpy()
+    xs = x.reshape(L, L, L).cpu().numpy()
+    ref = np.zeros_like(xs)
+    src = tuple(slice(max(0, -d), L - max(0, d)) for d in s)
+    dst = tuple(slice(max(0, d), L - max(0, -d)) for d in s)
+    ref[dst] = xs[src]
+    err = rel_l2(y, ref)
+    print(f"Toeplitz shift L={L} nf={nf}: {err:.2e}")
+    assert err < 1e-5

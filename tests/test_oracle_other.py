@@ -162,6 +162,37 @@ def test_cg_and_toeplitz_small():
     assert rel_l2(x, np.linalg.solve(A, b)) < 1e-8
 
 
+@pytest.mark.parametrize("L,nf", [(4, 8), (5, 10), (6, 16)])
+def test_toeplitz_closed_forms(L, nf):
+    """crop(ifft3c(lhs * fft3c(pad(x)))) (mean.py:118-123) pinned by what fixes it independently of
+    any FFT convention: a constant lhs = c is c times the identity; the centred phase ramp
+    lhs[k] = exp(-2 pi i k.s / nf) translates x by s with zero fill (the crop window never sees the
+    circular wrap when nf >= 2L); and a random lhs equals the direct convolution of x with the
+    kernel K[m] = nf^-3 sum_k lhs[k] exp(+2 pi i k.m / nf), the operator's Toeplitz form."""
+    o = oracle()
+    rng = np.random.default_rng(L * nf)
+    x = rng.standard_normal((L,) * 3) + 1j * rng.standard_normal((L,) * 3)
+    assert rel_l2(o.toeplitz_apply(x, np.full((nf,) * 3, 2.5 + 0j), L), 2.5 * x) < 1e-13
+    k = np.arange(nf) - nf // 2
+    s = (1, -2, 1)
+    ramp = np.exp(-2j * np.pi * (k[:, None, None] * s[0] + k[None, :, None] * s[1] + k[None, None, :] * s[2]) / nf)
+    shifted = np.zeros_like(x)
+    src = [slice(max(0, -d), L - max(0, d)) for d in s]
+    dst = [slice(max(0, d), L - max(0, -d)) for d in s]
+    shifted[tuple(dst)] = x[tuple(src)]
+    assert rel_l2(o.toeplitz_apply(x, ramp, L), shifted) < 1e-12
+    lhs = rng.standard_normal((nf,) * 3) + 1j * rng.standard_normal((nf,) * 3)
+    m = np.arange(-(L - 1), L)
+    ph = np.exp(2j * np.pi * np.outer(k, m) / nf)  # (nf, 2L-1)
+    K = np.einsum("abc,ai,bj,ck->ijk", lhs, ph, ph, ph) / nf**3  # K[m0, m1, m2], m = -(L-1)..L-1
+    y = np.zeros_like(x)
+    for i0 in range(L):
+        for i1 in range(L):
+            for i2 in range(L):
+                y[i0, i1, i2] = np.sum(K[i0 - np.arange(L) + L - 1][:, i1 - np.arange(L) + L - 1][:, :, i2 - np.arange(L) + L - 1] * x)
+    assert rel_l2(o.toeplitz_apply(x, lhs, L), y) < 1e-12
+
+
 def test_mlp_forward_matches_torch():
     o = oracle()
     n_in, hid = 3, 64
