@@ -46,6 +46,7 @@ _SIGNATURES = {
     "cov3d_ctf_eval": [c_vp, c_vp, c_i64, c_i64, c_vp],
     "cov3d_vdot_c64_f64": [c_vp, c_vp, c_i64, c_vp, c_vp],
     "cov3d_toeplitz_apply": [c_vp, c_vp, c_vp, c_i64, c_i64, c_vp],
+    "cov3d_toeplitz_apply_noalloc": [c_vp, c_vp, c_vp, c_i64, c_i64, c_vp],
     "cov3d_mlp_forward": [c_vp, c_vp, c_vp, c_i64, c_i64, c_vp],
     "cov3d_mlp_backward": [c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_vp],
     "cov3d_timing_enable": [c_int],

@@ -27,6 +27,13 @@
 
 #include "fft_lds.h"
 
+// Workspace of cov3d_toeplitz_apply_noalloc: the L * nf^2 complex buffer of the pruned passes for
+// L * nf^2 <= 2^23 (64 MiB; L = 128 with nf = 256 fits).  Each device the library runs on holds
+// its own copy from the first use until the process ends.  Host code never passes this name to a
+// kernel: it uses the device address returned by hipGetSymbolAddress (cov3d::scratch_address).
+constexpr int64_t kTplScratchElems = (int64_t)1 << 23;
+static __device__ __align__(256) float2 scratch[kTplScratchElems];
+
 namespace cov3d {
 
 // ---- Z+ and Z-: contiguous z lines, 16 per workgroup.  g = i0 * L + i1 < L^2.
@@ -275,7 +282,49 @@ static int tpl_get(int64_t L, int64_t nf, TplCtx** out) {
   return 0;
 }
 
+// Device address of `scratch` on the current device, looked up once per device with that device
+// current and kept.
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_addr[kMaxDevices] = {};
+
+static int scratch_address(void** out) {
+  int dev = 0;
+  COV3D_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("device index %d out of range for the no-allocation workspace", dev);
+    return COV3D_ERR_HIP;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_addr[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+      set_error("hipGetSymbolAddress of the Toeplitz workspace failed: %s", hipGetErrorString(e));
+      return COV3D_ERR_HIP;
+    }
+    g_scratch_addr[dev] = p;
+  }
+  *out = g_scratch_addr[dev];
+  return 0;
+}
+
 }  // namespace cov3d
+
+extern "C" int cov3d_toeplitz_apply_noalloc(const void* x, const void* lhs, void* y, int64_t L, int64_t nf,
+                                            void* stream) {
+  COV3D_CHECK_ARG(L >= 1 && nf >= L, "need nf >= L >= 1");
+  COV3D_CHECK_ARG(x && lhs && y, "NULL pointer");
+  COV3D_CHECK_ARG(cov3d::tpl_pruned_supported(L, nf),
+                  "no-allocation Toeplitz operator: nf must be 64, 128, 256 or 512 (got %lld)", (long long)nf);
+  COV3D_CHECK_ARG(L * nf * nf <= kTplScratchElems,
+                  "no-allocation Toeplitz operator: L * nf^2 = %lld exceeds the workspace of %lld elements",
+                  (long long)(L * nf * nf), (long long)kTplScratchElems);
+  void* buf = nullptr;
+  int rc = cov3d::scratch_address(&buf);
+  if (rc) return rc;
+  return cov3d::tpl_pruned(x, lhs, y, buf, L, nf, cov3d::as_stream(stream));
+}
 
 extern "C" int cov3d_toeplitz_apply(const void* x, const void* lhs, void* y, int64_t L, int64_t nf,
                                     void* stream) {

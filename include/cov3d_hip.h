@@ -213,10 +213,26 @@ int cov3d_vdot_c64_f64(const void* x, const void* y, int64_t n, void* out,
 
 /* Toeplitz normal operator of the mean CG (mean.py:118-123) in one call:
  *   y = crop_L(ifft3c(lhs * fft3c(pad_{nf}(x))))  for x (L^3) complex64,
- *   lhs (nf^3) complex64 with nf = us * L.  Uses rocFFT; workspace owned by
- *   a cached per-(L, nf) context. */
+ *   lhs (nf^3) complex64, nf >= L (nf = us * L in the mean solve).  For
+ *   nf = 64, 128, 256 or 512 it runs pruned line FFTs on the non-zero lines
+ *   (workspace L * nf^2 complex64); other grids use rocFFT on the full grid.
+ *   The workspace is allocated on the first call per (device, L, nf) and
+ *   kept until the process ends. */
 int cov3d_toeplitz_apply(const void* x, const void* lhs, void* y, int64_t L,
                          int64_t nf, void* stream);
+
+/* cov3d_toeplitz_apply without any device allocation: the pruned passes run
+ * on a static device workspace of 2^23 complex64 values (64 MiB, held by each
+ * device the library runs on until the process ends).  No counterpart in the
+ * reference; it is for callers that cannot allocate device memory (code under
+ * stream capture, tight solver loops, allocation-free harnesses).  Requires
+ * nf in {64, 128, 256, 512}, nf >= L and L * nf^2 <= 2^23 (L = 128 with
+ * nf = 256 fits); other sizes return COV3D_ERR_ARG.  Every call on one device
+ * shares that workspace: calls that overlap on one device (other streams or
+ * threads) race and must be serialised by the caller.  The result is
+ * bit-identical to cov3d_toeplitz_apply's. */
+int cov3d_toeplitz_apply_noalloc(const void* x, const void* lhs, void* y,
+                                 int64_t L, int64_t nf, void* stream);
 
 /* Learned diagonal preconditioner MLP (in -> 64 -> 64 -> 64 -> 1, GELU):
  * forward over n rows; weights packed [W1 b1 W2 b2 W3 b3 W4 b4] f32. */
