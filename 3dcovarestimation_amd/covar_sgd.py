@@ -192,6 +192,7 @@ class CovarTrainer:
         self.fourier_reg = None
         self.form = _DEFAULT_FORM
         self.precision = ml_ops.PRECISION_BF16X3
+        self.half_plane = True
 
     @property
     def dataset(self):
@@ -216,7 +217,7 @@ class CovarTrainer:
         self.optimizer.zero_grad()
         kw = {}
         if self.cost_func is cost_maximum_liklihood:
-            kw = {"form": self.form, "precision": self.precision}
+            kw = {"form": self.form, "precision": self.precision, "half_plane": self.half_plane}
         cost_val = self.cost_func(self._covar(None), images, self.nufft_plans, filters, self.noise_var,
                                   self.reg_scale, self.fourier_reg, **kw)
         cost_val.backward()

@@ -24,12 +24,14 @@ from .covar_sgd import CovarTrainer
 from .dataset import CovarDataset, create_dataloader
 from .nufft_plan import NufftPlan
 
-# (tag, form, precision, dtype)
+# tag -> (form, precision, dtype, half plane)
 FORMS = {
-    "fp64": ("autograd", ml_ops.PRECISION_FP32, torch.float64),
-    "chosen": ("fused", ml_ops.PRECISION_BF16X3, torch.float32),
-    "fused_fp32": ("fused", ml_ops.PRECISION_FP32, torch.float32),
-    "plain_fp32": ("autograd", ml_ops.PRECISION_FP32, torch.float32),
+    "fp64": ("autograd", ml_ops.PRECISION_FP32, torch.float64, False),
+    "chosen": ("fused", ml_ops.PRECISION_BF16X3, torch.float32, True),
+    "fused_fp32": ("fused", ml_ops.PRECISION_FP32, torch.float32, True),
+    "fused_fp32_full": ("fused", ml_ops.PRECISION_FP32, torch.float32, False),
+    "chosen_full": ("fused", ml_ops.PRECISION_BF16X3, torch.float32, False),
+    "plain_fp32": ("autograd", ml_ops.PRECISION_FP32, torch.float32, False),
 }
 
 
@@ -62,7 +64,7 @@ def sgd_subspaces(dataset: CovarDataset, rank: int, v0: torch.Tensor, batch_size
     d64 = None
     out, secs = {}, {}
     for tag in forms:
-        form, prec, dt = FORMS[tag]
+        form, prec, dt, half = FORMS[tag]
         ds = dataset
         if dt == torch.float64:
             if d64 is None:
@@ -71,6 +73,7 @@ def sgd_subspaces(dataset: CovarDataset, rank: int, v0: torch.Tensor, batch_size
         covar = Covar(L, rank, dtype=dt, pixel_var_estimate=1.0, vectors=v0.to(dt)).to(dev)
         trainer = CovarTrainer(covar, create_dataloader(ds, batch_size, idx), dev)
         trainer.setup_training(nufft_disc="nufft", objective_func="ml", lr=lr, form=form, precision=prec)
+        trainer.half_plane = half
         if dt == torch.float64:
             trainer.nufft_plans = NufftPlan((L,) * 3, batch_size=rank, eps=fp64_eps, dtype=torch.float64, device=dev)
         torch.cuda.synchronize()

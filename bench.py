@@ -52,10 +52,14 @@ def parse():
     ap.add_argument("--plain-steps", type=int, default=None)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-images", type=int, default=256)
+    ap.add_argument("--cpu-images", type=int, default=1024)
+    ap.add_argument("--cpu-steps", type=int, default=3, help="timed CPU steps (the median is reported)")
+    ap.add_argument("--no-nudft", action="store_true", help="skip the FP64 NUDFT check of the NUFFT outputs")
     ap.add_argument("--no-cg", action="store_true", help="skip the mean-CG leg")
-    ap.add_argument("--subspace-batches", type=int, default=4,
-                    help="SGD steps of the FP64-trajectory subspace comparison (0 skips it)")
+    ap.add_argument("--subspace-batches", type=int, default=98,
+                    help="SGD steps of the FP64-trajectory subspace comparison (98 = one epoch of the "
+                         "100k-image workload at batch 1024; 0 skips it)")
+    ap.add_argument("--fp64-eps", type=float, default=1e-9, help="NUFFT tolerance of the FP64 trajectory")
     ap.add_argument("--cg-images", type=int, default=100000)
     ap.add_argument("--cg-cap", type=int, default=20000, help="iteration cap of each CG solve")
     ap.add_argument("--copy-probe", action="store_true",
@@ -63,7 +67,7 @@ def parse():
     return ap.parse_args()
 
 
-def sgd_run(pkg, data, L, r, B, steps, warmup, form, timing=False):
+def sgd_run(pkg, data, L, r, B, steps, warmup, form, timing=False, precision=None):
     import torch
 
     covar_mod = importlib.import_module(pkg + ".covar")
@@ -75,7 +79,8 @@ def sgd_run(pkg, data, L, r, B, steps, warmup, form, timing=False):
     covar = covar_mod.Covar(L, r, pixel_var_estimate=1.0 / L).to(dev)
     loader = dsm.create_dataloader(data.dataset, batch_size=B)
     trainer = sgd.CovarTrainer(covar, loader, dev)
-    trainer.setup_training(nufft_disc="nufft", objective_func="ml", form=form)
+    kw = {} if precision is None else {"precision": precision}
+    trainer.setup_training(nufft_disc="nufft", objective_func="ml", form=form, **kw)
     batches = iter(loader)
     for _ in range(warmup):
         images, pts, filt = trainer.prepare_batch(next(batches))
@@ -135,29 +140,37 @@ def contraction_run(pkg, data, L, r, B, reps=20):
     return out
 
 
-def subspace_run(pkg, data, L, r, B, n_batches):
-    """Subspace of the chosen and plain forms vs an FP64 trajectory (same start, same batches)."""
+def subspace_run(pkg, data, L, r, B, n_batches, fp64_eps):
+    """Subspaces of the SGD forms vs an FP64 trajectory (same start, same batches, lr 1e-6 per image).
+
+    chosen = split-BF16 Gram on the half plane; fused_fp32 = FP32 Gram on the half plane;
+    chosen_full = split-BF16 Gram on the full plane; plain_fp32 = the reference's FP32 autograd
+    objective.  Comparing them says which of the two changes (BF16 contractions, half plane)
+    moves the learned subspace."""
     import torch
 
     val = importlib.import_module(pkg + ".validation")
     ds = data.dataset
     signal_var = max(float(ds.images[: n_batches * B].var()) - ds.noise_var, 1e-6)
     v0 = torch.randn((r,) + (L,) * 3, generator=torch.Generator().manual_seed(0)) * (signal_var / L) ** 0.5
+    forms = ("fp64", "chosen", "fused_fp32", "chosen_full", "plain_fp32")
     # lr 1e-6 per image: the reference workflow's default for the eigenvector fit (workflow.py:377)
-    bases, secs = val.sgd_subspaces(ds, r, v0, batch_size=B, epochs=1, n_images=n_batches * B, lr=1e-6,
-                                    forms=("fp64", "chosen", "plain_fp32"))
+    bases, secs = val.sgd_subspaces(ds, r, v0, batch_size=B, epochs=1, n_images=min(n_batches * B, len(ds)),
+                                    lr=1e-6, forms=forms, fp64_eps=fp64_eps)
     err = val.subspace_errors(bases)
     q0, _ = torch.linalg.qr(v0.reshape(r, -1).double().T.to(bases["fp64"].device))
-    return {
-        "chosen_vs_fp64": err["chosen"],
-        "plain_fp32_vs_fp64": err["plain_fp32"],
+    out = {f"{k}_vs_fp64": err[k] for k in forms[1:]}
+    out.update({
+        "chosen_vs_plain_fp32": val.subspace_distance(bases["plain_fp32"], bases["chosen"]),
         "fp64_moved_from_start": val.subspace_distance(q0, bases["fp64"]),
         "steps": n_batches,
+        "images": min(n_batches * B, len(ds)),
         "seconds": {k: round(v, 2) for k, v in secs.items()},
         "definition": (f"RMS sine of principal angles between r={r} eigen-subspaces after {n_batches} Adam steps "
-                       f"of {B} images (lr 1e-6 per image) from the same start; reference trajectory in FP64 (parameters, images, "
-                       f"NUFFT eps 1e-12, plain autograd objective)"),
-    }
+                       f"of {B} images (lr 1e-6 per image) from the same start; reference trajectory in FP64 "
+                       f"(parameters, images, NUFFT eps {fp64_eps:g}, plain autograd objective)"),
+    })
+    return out
 
 
 def mean_cg_run(pkg, L, n_images, dev, cap=20000):
@@ -198,31 +211,122 @@ def mean_cg_run(pkg, L, n_images, dev, cap=20000):
     return res
 
 
-def cpu_baseline(L, r, B_cpu, noise_var=1.0):
+def cpu_baseline(L, r, B_cpu, steps=3, noise_var=1.0):
+    """The FP32 CPU port (oracle/cpu_nufft.py + cpu_nufft.cpp, OpenMP) timed on this host: the
+    median of ``steps`` SGD steps (setpts, type-2 NUFFT, ML cost and gradient on the half plane,
+    type-1 NUFFT, Adam) of B_cpu images at L, r, with the step's own point set each time."""
     import numpy as np
 
     cpu = importlib.import_module("oracle.cpu_nufft")
     o = importlib.import_module("oracle.oracle")
     o.build(force=False)
+    cpu.build(force=False)
     rng = np.random.default_rng(0)
-    vols = rng.standard_normal((r, L, L, L)) / L
-    R = np.linalg.qr(rng.standard_normal((B_cpu, 3, 3)))[0]
-    pts = o.pts_rot(R, L)
-    ctf = o.ctf(np.tile(np.array([[15000, 15000, 0, 300, 2.7, 0.1, 1.5]]), (B_cpu, 1)), L)
-    images = rng.standard_normal((B_cpu, L, L))
-    cores = os.cpu_count() or 1
-    nu = cpu.CpuNufft(L, r, 1e-6, workers=cores)
-    t0 = time.perf_counter()
-    cpu.sgd_step_cpu(vols, images, pts, ctf, noise_var, nu)
-    dt = time.perf_counter() - t0
+    vols = (rng.standard_normal((r, L, L, L)) / L).astype(np.float32)
+    m, v = np.zeros_like(vols), np.zeros_like(vols)
+    nu = cpu.CpuNufft(L, r, 1e-6)
+    times = []
+    for k in range(steps):
+        R = np.linalg.qr(rng.standard_normal((B_cpu, 3, 3)))[0]
+        pts = o.pts_rot(R, L).astype(np.float32)
+        ctf = o.ctf(np.tile(np.array([[15000, 15000, 0, 300, 2.7, 0.1, 1.5]]), (B_cpu, 1)), L).astype(np.float32)
+        images = rng.standard_normal((B_cpu, L, L)).astype(np.float32)
+        t0 = time.perf_counter()
+        _, g = cpu.sgd_step_cpu(vols, images, pts, ctf, noise_var, nu)
+        cpu.adam_step(vols, g, m, v, k + 1, 1e-6 * B_cpu)
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
     return {
-        "value": B_cpu / dt,
+        "value": B_cpu / med,
         "unit": "images/s",
-        "cores": cores,
+        "cores": nu.workers,
         "kind": "port",
-        "sample": f"one ML cost+gradient step (type-2 + type-1 ES NUFFT in FP64, numpy ML terms) on "
-        f"{B_cpu} images at L={L}, r={r}; {dt:.1f} s",
+        "dtype": "f32",
+        "sample": (f"median of {steps} SGD steps of {B_cpu} images at L={L}, r={r} (FP32 C++/OpenMP ES-NUFFT "
+                   f"spreading/interpolation, pruned scipy.fft fine-grid FFTs, numpy ML terms on the half plane, "
+                   f"Adam); step seconds {[round(t, 2) for t in times]}"),
+        "pinned_by": "tests/test_cpu_port.py (NUFFT vs FP64 NUDFT <= 1e-5; ML step vs FP64 oracle <= 1e-4)",
     }
+
+
+def nudft_check(pkg, data, L, r, B, sub=512, per_image=4, n_modes=64):
+    """Error of the benchmark's NUFFT outputs against the FP64 direct NUDFT on a 512-image subset of
+    the first batch (the chosen form's half-plane points): type 2 of the ground-truth eigenvolumes at
+    ``per_image`` random pixels of each subset image; type 1 of the resulting values on the subset's
+    points at ``n_modes`` random output modes."""
+    import numpy as np
+    import torch
+
+    o = importlib.import_module("oracle.oracle")
+    o.build(force=False)
+    npl = importlib.import_module(pkg + ".nufft_plan")
+    ml = importlib.import_module(pkg + ".ml_ops")
+    dev = data.dataset.images.device
+    _, pts, _, _ = data.dataset[0:B]
+    pix = ml.half_plane_pixels(L, dev).long()
+    npx = pix.numel()
+    P = pts[:, :, pix].transpose(0, 1).reshape(3, -1)
+    P = (torch.remainder(P + torch.pi, 2 * torch.pi) - torch.pi).contiguous()
+    vols = (data.eigenvecs * data.eigenvals.sqrt().reshape(-1, 1, 1, 1)).float()[:r].contiguous()
+    t0 = time.perf_counter()
+    plan = npl.NufftPlan((L,) * 3, batch_size=r, device=dev)
+    plan.set_points_flat(P, B)
+    c = plan.execute_forward_flat(vols)
+    rng = np.random.default_rng(5)
+    cols = (np.arange(sub)[:, None] * npx + rng.integers(0, npx, (sub, per_image))).ravel()
+    ci = torch.tensor(cols, device=dev)
+    ref2 = o.nudft_type2(vols.cpu().numpy().astype(np.complex128), P[:, ci].cpu().numpy().astype(np.float64))
+    e2 = float(np.linalg.norm(c[:, ci].cpu().numpy() - ref2) / np.linalg.norm(ref2))
+    del plan
+    Ps = P[:, : sub * npx].contiguous()
+    cs = c[:, : sub * npx].contiguous()
+    plan1 = npl.NufftPlan((L,) * 3, batch_size=r, device=dev)
+    plan1.set_points_flat(Ps, sub)
+    f1 = plan1.execute_adjoint(cs).reshape(r, -1)
+    modes = rng.choice(L**3, n_modes, replace=False)
+    ref1 = o.nudft_type1_modes(cs.cpu().numpy().astype(np.complex128), Ps.cpu().numpy().astype(np.float64),
+                               (L, L, L), modes)
+    e1 = float(np.linalg.norm(f1[:, torch.tensor(modes, device=dev)].cpu().numpy() - ref1) / np.linalg.norm(ref1))
+    del plan1
+    torch.cuda.empty_cache()
+    return {"type2": e2, "type1": e1, "subset_images": sub,
+            "type2_points": int(cols.size), "type1_points": int(Ps.shape[1]), "type1_modes": int(n_modes),
+            "transforms": r, "reference": "FP64 direct NUDFT (oracle/nudft.c)",
+            "seconds": round(time.perf_counter() - t0, 1)}
+
+
+def ml_vs_fp64(pkg, data, L, r, n_images=48):
+    """Cost and gradient of the three forms of the ML objective against the FP64 autograd objective
+    (FP64 NUFFT plan, eps 1e-12) on the first ``n_images`` images of the benchmark data."""
+    import torch
+
+    npl = importlib.import_module(pkg + ".nufft_plan")
+    sgd = importlib.import_module(pkg + ".covar_sgd")
+    dev = data.dataset.images.device
+    images, pts, ctf, _ = data.dataset[0:n_images]
+    nv = data.dataset.noise_var
+    vols = (data.eigenvecs * data.eigenvals.sqrt().reshape(-1, 1, 1, 1)).float()[:r]
+    vols = vols + 0.3 * vols.std() * torch.randn(vols.shape, generator=torch.Generator().manual_seed(4)).to(dev)
+    p64 = npl.NufftPlan((L,) * 3, batch_size=r, eps=1e-12, dtype=torch.float64, device=dev)
+    p64.setpts(pts.double())
+    v = vols.double().clone().requires_grad_(True)
+    ref = sgd.cost_maximum_liklihood(v, images.double(), p64, ctf.double(), nv, form="autograd")
+    ref.backward()
+    ref_c, ref_g = ref.item(), v.grad
+    del p64
+    plan = npl.NufftPlan((L,) * 3, batch_size=r, device=dev)
+    plan.setpts(pts)
+    out = {}
+    for tag, form, prec in (("chosen", "fused", 1), ("fused_fp32", "fused", 0), ("plain_fp32", "autograd", 0)):
+        w = vols.clone().requires_grad_(True)
+        cst = sgd.cost_maximum_liklihood(w, images, plan, ctf, nv, form=form, precision=prec)
+        cst.backward()
+        out[tag] = {"cost": abs(cst.item() - ref_c) / abs(ref_c),
+                    "gradient": float((w.grad.double() - ref_g).norm() / ref_g.norm())}
+    del plan
+    torch.cuda.empty_cache()
+    out["images"] = n_images
+    return out
 
 
 def main():
@@ -241,9 +345,10 @@ def main():
     dev = torch.device("cuda:0")
     L, r, B = args.L, args.rank, args.batch
     plain_steps = args.plain_steps or max(2, args.steps // 2)
-    n_img = (args.warmup + max(args.steps, plain_steps)) * B
+    n_img = max((args.warmup + max(args.steps, plain_steps)) * B, args.subspace_batches * B)
     cfg = dsm.config_by_name("L128") if L == 128 else dsm.SyntheticConfig(L, n_img, r, [1.0] * r, 1.5, 0.02,
                                                                              (0.5, 3.0))
+    n_img = min(n_img, cfg.n_images)
     cfg.rank = r
     cfg.eigenvalues = list(cfg.eigenvalues)[:r] + [0.1] * max(0, r - len(cfg.eigenvalues))
     log(f"generating {n_img} synthetic images L={L} r={r}")
@@ -256,14 +361,20 @@ def main():
     dt, kern = sgd_run(pkg, data, L, r, B, args.steps, args.warmup, "fused", timing=True)
     ips = args.steps * B / dt
     log(f"chosen: {ips:.1f} images/s ({1e3 * dt / args.steps:.2f} ms/step)")
-    plain = None
+    plain = fused32 = None
     if not args.no_plain:
+        log("fused FP32 form (FP32 Gram + hand adjoint)")
+        dtf, _ = sgd_run(pkg, data, L, r, B, plain_steps, 1, "fused", precision=0)
+        fused32 = {"value": plain_steps * B / dtf, "ms_per_step": 1e3 * dtf / plain_steps, "steps": plain_steps}
+        log(f"fused FP32: {fused32['value']:.1f} images/s")
         log("plain FP32 autograd form")
         dtp, _ = sgd_run(pkg, data, L, r, B, plain_steps, 1, "autograd")
         plain = {"value": plain_steps * B / dtp, "ms_per_step": 1e3 * dtp / plain_steps, "steps": plain_steps}
         log(f"plain: {plain['value']:.1f} images/s")
 
-    # roofline of the dominant NUFFT kernel (algorithmic bytes per launch)
+    # roofline of the dominant NUFFT kernel.  Both NUFFT kernels are arithmetic-bound (about 80 flop per
+    # algorithmic byte against an FP32 ridge of about 20), so the binding figure is FP32 vector
+    # throughput; the algorithmic HBM rate is reported beside it.
     nf = 2 * L
     # points per image on the chosen path: one pixel of every +-k pair (Hermitian half plane)
     ml_ops = importlib.import_module(pkg + ".ml_ops")
@@ -273,29 +384,36 @@ def main():
     # per point: 3 coordinates (12 B) + r complex values (8r B); plus the r fine grids once
     bytes_per_launch = B * (12 * d + 8 * r * d) + 8 * r * nf**3
     w = 7
+    # per point and transform: w^3 taps, each a real weight times a complex value (2 FMAs = 4 flops)
     flops_per_launch = B * d * (w**3 * r * 4)
-    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    # measured HBM bytes per launch of the same kernel (rocprofv3 FETCH_SIZE / WRITE_SIZE,
-    # scaled by a 1 GiB copy) from the committed profile, when it covers this workload
-    traffic = None
+    achieved_gbs = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    achieved_tf = flops_per_launch / (avg_ms * 1e-3) / 1e12
+    # measured HBM bytes per launch of the same kernel (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a
+    # 1 GiB copy) from the committed profile of this workload: not measured in this run
+    traffic, traffic_src = None, None
     tj = ROOT / "profiles" / f"hbm_traffic_sgd_L{L}_r{r}_B{B}.json"
     if tj.exists():
         try:
             traffic = round(json.loads(tj.read_text())[f"k_{dom}"]["traffic_GB"] * 1e9)
+            traffic_src = f"profiles/{tj.name} (committed rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE run, not this run)"
         except (KeyError, ValueError):
             traffic = None
     roof = {
         "kernel": f"k_{dom} (ES w=7, nf={nf}, r={r}, {B} images x {d} half-plane points)",
-        "bound": "hbm",
-        "achieved": round(achieved, 1),
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "bound": "valu",
+        "achieved": round(achieved_tf, 2),
+        "peak": FP32_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(achieved_tf / FP32_PEAK_TFLOPS, 4),
+        "frac_of_nonpacked_peak": round(achieved_tf / (FP32_PEAK_TFLOPS / 2), 4),
         "traffic": traffic,
+        "traffic_source": traffic_src,
+        "algorithmic_flops": flops_per_launch,
         "algorithmic_bytes": bytes_per_launch,
-        "valu_tflops": round(flops_per_launch / (avg_ms * 1e-3) / 1e12, 2),
-        "valu_frac_fp32_peak": round(flops_per_launch / (avg_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
+        "hbm_achieved_GBs": round(achieved_gbs, 1),
+        "hbm_frac": round(achieved_gbs / HBM_PEAK_GBS, 4),
         "avg_launch_ms": round(avg_ms, 3),
+        "timing": "HIP events on the launch stream around every launch in the timed steps (cov3d_timing_*)",
     }
     if args.copy_probe:
         src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
@@ -310,15 +428,24 @@ def main():
         log(f"contractions: {contr}")
     except Exception as e:  # reported, never required
         contr = {"error": repr(e)}
-    # achieved algorithmic GB/s of both NUFFT kernels
+    # achieved algorithmic rates of both NUFFT kernels
     per_kernel = {k: {"avg_ms": round(kern[k]["avg_ms"], 3),
-                      "achieved_GBs": round(bytes_per_launch / (kern[k]["avg_ms"] * 1e-3) / 1e9, 1)}
+                      "achieved_GBs": round(bytes_per_launch / (kern[k]["avg_ms"] * 1e-3) / 1e9, 1),
+                      "achieved_TFLOPs": round(flops_per_launch / (kern[k]["avg_ms"] * 1e-3) / 1e12, 2)}
                   for k in ("interp", "spread") if kern[k]["avg_ms"]}
+    nud = mlf = None
+    if not args.no_nudft:
+        log("NUFFT outputs vs the FP64 NUDFT on a 512-image subset")
+        nud = nudft_check(pkg, data, L, r, B)
+        log(f"NUDFT check: {nud}")
+        log("ML cost and gradient of the three forms vs FP64")
+        mlf = ml_vs_fp64(pkg, data, L, r)
+        log(f"ML vs FP64: {mlf}")
     sub = None
     if args.subspace_batches > 0:
         log(f"subspace agreement vs FP64 over {args.subspace_batches} steps")
-        sub = subspace_run(pkg, data, L, r, B, args.subspace_batches)
-        log(f"subspace: chosen {sub['chosen_vs_fp64']:.2e}, plain {sub['plain_fp32_vs_fp64']:.2e} ({sub['seconds']})")
+        sub = subspace_run(pkg, data, L, r, B, args.subspace_batches, args.fp64_eps)
+        log(f"subspace: {sub}")
     cg = None
     if not args.no_cg:
         del data
@@ -329,9 +456,10 @@ def main():
             f"{cg.get('learned', {}).get('iterations')} it / {cg.get('learned', {}).get('seconds')} s")
     cpu = None
     if not args.no_cpu:
-        log(f"CPU baseline on {args.cpu_images} images")
+        log(f"CPU baseline: {args.cpu_steps} steps of {args.cpu_images} images")
         try:
-            cpu = cpu_baseline(L, r, args.cpu_images)
+            cpu = cpu_baseline(L, r, args.cpu_images, args.cpu_steps)
+            log(f"CPU baseline: {cpu['value']:.2f} images/s on {cpu['cores']} threads")
         except Exception as e:  # the baseline is reported, never required
             cpu = {"error": repr(e)}
     out = {
@@ -350,11 +478,15 @@ def main():
         "config": {"workload": f"covariance SGD step, NUFFT path, ML objective, L={L}, r={r}, batch={B}",
                    "L": L, "rank": r, "batch": B, "form": "bf16x3 Gram + hand-written adjoint"},
         "plain_fp32_autograd": plain,
+        "fused_fp32": fused32,
         "speedup_vs_plain": round(ips / plain["value"], 3) if plain else None,
+        "speedup_vs_fused_fp32": round(ips / fused32["value"], 3) if fused32 else None,
         "kernels_ms": {k: round(v["avg_ms"], 3) if v["avg_ms"] else None for k, v in kern.items()},
         "roofline": roof,
         "nufft_kernels": per_kernel,
         "contractions": contr,
+        "nudft_rel_err": nud,
+        "ml_vs_fp64": mlf,
         "subspace_error": sub,
         "mean_cg": cg,
         "cpu_baseline": cpu,
