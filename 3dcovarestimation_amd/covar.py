@@ -116,12 +116,34 @@ class Covar(VolumeBase):
         self.rank = rank
         self.pixel_var_estimate = pixel_var_estimate
         if vectors is None:
-            vectors = torch.randn((rank,) + (resolution,) * 3, dtype=dtype) * (float(pixel_var_estimate) ** 0.5)
+            # a scalar pixel variance gives white vectors; a (radial or full) PSD shapes them (covar.py:316-322)
+            psd_given = isinstance(pixel_var_estimate, torch.Tensor) and pixel_var_estimate.ndim > 0
+            vectors = (self.init_random_vectors_from_psd(rank, pixel_var_estimate) if psd_given
+                       else self.init_random_vectors(rank))
         else:
             vectors = vectors.clone()
         v, l = self._split(vectors)
         self.vectors = torch.nn.Parameter(v)
         self.log_sqrt_eigenvals = torch.nn.Parameter(l)
+
+    def init_random_vectors(self, num_vectors: int) -> torch.Tensor:
+        """White Gaussian vectors with the estimated pixel variance (covar.py:377-387)."""
+        return torch.randn((num_vectors,) + (self.resolution,) * 3, dtype=self.dtype) * (
+            float(self.pixel_var_estimate) ** 0.5)
+
+    def init_random_vectors_from_psd(self, num_vectors: int, psd: torch.Tensor) -> torch.Tensor:
+        """Gaussian vectors whose spectrum follows ``psd`` (covar.py:389-405): white noise, centred
+        FFT scaled by L^-1.5 (so unit white noise has unit spectral density), times sqrt(psd), back
+        to space.  A 1-D ``psd`` is radial and expanded over the Fourier shells."""
+        from .fsc_utils import expand_fourier_shell
+
+        L = self.resolution
+        psd = torch.as_tensor(psd)
+        if psd.ndim == 1:
+            psd = expand_fourier_shell(psd, L, 3)
+        vectors = torch.randn((num_vectors,) + (L,) * 3, dtype=self.dtype, device=psd.device)
+        vf = centered_fft3(vectors) / (L**1.5) * torch.sqrt(psd.to(vectors.dtype))
+        return centered_ifft3(vf).real.to(self.dtype)
 
     @staticmethod
     def _split(vectors: torch.Tensor) -> Tuple[torch.Tensor, torch.Tensor]:
