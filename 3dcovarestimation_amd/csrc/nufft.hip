@@ -1,16 +1,24 @@
 // 3-D type-1 / type-2 NUFFT for gfx950 with the exponential-of-semicircle
-// (ES) kernel, oversampling factor 2 and rocFFT for the dense fine-grid FFT.
+// (ES) kernel and oversampling factor 2.
 //
 // Replaces the finufft / cufinufft plans driven by cov3d/nufft_plan.py:263-392.
 // Method (published FINUFFT / cuFINUFFT design, re-derived here):
 //   type 2: deconvolve + zero-pad modes onto the (nf0, nf1, nf2) fine grid,
 //           FFT, then interpolate every point with a W^3 ES stencil;
 //   type 1: spread every point with the W^3 ES stencil, FFT, deconvolve+crop.
-// Points are bin-sorted once per setpts.  Interpolation and spreading work on
-// one bin at a time: the bin's fine-grid tile plus a W-wide halo is staged in
-// LDS with the transform index innermost ([z][y][x][t]), so each stencil tap
-// is a contiguous LDS read (type 2) or LDS atomic (type 1); the type-1 tile
-// is flushed to HBM with one float atomic per tile cell.
+// Points are bin-sorted once per setpts (stable radix sort by 4^3 bin and axis-0
+// stencil position), so every per-bin sum runs over the same points in the same order.
+//   interpolation (k_interp_pipe, FP32; k_interp otherwise): a workgroup stages one
+//     super-bin's fine-grid tile plus its W-wide halo in LDS, two transforms per
+//     16-byte cell, and one thread per point reads its W^3 taps from LDS and writes
+//     the caller's (ntrans, M) layout; the pipelined kernel is persistent, takes its
+//     items from per-XCD counters and loads the next tile while the current taps run;
+//   spreading (k_spread_mfma, FP32; k_spread_gather for FP64): output-driven gather,
+//     one workgroup per 4^3 bin walks the points of the 27 neighbouring bins and forms
+//     each cell as a matrix product (weights x values) on the FP32 matrix cores, so each
+//     cell is written once, with no atomics, and the result is deterministic;
+//   FFTs: pruned real 3-D FFTs on 64/128/256/512-cell grids (fft_pruned.hip), rocFFT
+//     for FP64 and other grids.
 #include <math.h>
 #include <rocfft/rocfft.h>
 
@@ -588,30 +596,6 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-template <typename T>
-__global__ void k_from_sorted(const Cplx<T>* __restrict__ cs, Cplx<T>* __restrict__ c,
-                              const int* __restrict__ spos, int64_t M, int ntrans, int tc) {
-  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= M) return;
-  const int64_t sp = spos[j];
-  if constexpr (sizeof(T) == 4) {
-    if (tc == 2) {
-      // chunk h holds transforms 2h, 2h + 1 of a point in one 16-byte word
-      const float4* s4 = reinterpret_cast<const float4*>(cs);
-      for (int h = 0; 2 * h < ntrans; ++h) {
-        const float4 v = s4[(int64_t)h * M + sp];
-        c[(int64_t)(2 * h) * M + j] = Cplx<T>{v.x, v.y};
-        if (2 * h + 1 < ntrans) c[(int64_t)(2 * h + 1) * M + j] = Cplx<T>{v.z, v.w};
-      }
-      return;
-    }
-  }
-  for (int t = 0; t < ntrans; ++t) {
-    const int h = t / tc, tt = t - h * tc;
-    c[(int64_t)t * M + j] = cs[((int64_t)h * M + sp) * tc + tt];
-  }
-}
-
 struct SubDesc {
   const int* sub_sb;
   const int* sub_pt;
@@ -1183,235 +1167,6 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-// Type-2 interpolation with eight lanes per point (FP32, W <= 8, TC even).
-// Same subproblem and LDS tile as k_interp.  Lane a0 = lane % 8 of a point's
-// group sums the W^2 taps of stencil plane a0; the group then adds its eight
-// partial sums with xor shuffles.  Consecutive a0 planes are E^2 cells apart,
-// which is 4 (mod 32) LDS banks for E = 15 and 16-byte cells, so the eight
-// 16-byte reads of a group fall on disjoint bank quads instead of the random
-// addresses of one-thread-per-point.  Each lane evaluates the three axis
-// weights of index a0; the group exchanges them with shuffles.
-template <int W, int TC>
-__global__ void __launch_bounds__(kBlock)
-    k_interp_rows(const Cplx<float>* __restrict__ fine, Cplx<float>* __restrict__ c, SubDesc sd,
-                  BinGeom g, const float* __restrict__ sv0, const float* __restrict__ sv1,
-                  const float* __restrict__ sv2, int64_t M, int ntrans, float kc, float beta, int half_h2,
-                  int conj_all) {
-  static_assert(W <= 8 && TC % 2 == 0, "eight lanes per point hold at most 8 stencil planes");
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  Cplx<float>* tile = reinterpret_cast<Cplx<float>*>(smem_raw);
-  const int sub = xcd_remap(blockIdx.x, gridDim.x);
-  if (sub >= *sd.nsub) return;
-  const int t0 = blockIdx.y * TC;
-  const int sb = sd.sub_sb[sub];
-  const int s2 = sb % g.nsb[2];
-  const int s1 = (sb / g.nsb[2]) % g.nsb[1];
-  const int s0 = sb / (g.nsb[2] * g.nsb[1]);
-  constexpr int E = kSuperEdge + W;
-  constexpr int ncell = E * E * E;
-  const int o0 = s0 * kSuperEdge - W / 2, o1 = s1 * kSuperEdge - W / 2, o2 = s2 * kSuperEdge - W / 2;
-  stage_tile<float, W, TC>(fine, tile, g, o0, o1, o2, t0, ntrans, half_h2, conj_all);
-  __syncthreads();
-  const int p0 = sd.sub_pt[sub];
-  const int pend = p0 + sd.sub_n[sub];
-  const int lane = threadIdx.x & 63;
-  const int gbase = lane & ~7;
-  const int a0 = lane & 7;
-  const float halfw = 0.5f * (float)W;
-  const float beta_l2 = beta * 1.4426950408889634f;
-  const float4* tile4 = reinterpret_cast<const float4*>(tile);
-  for (int pb = p0; pb < pend; pb += kBlock / 8) {
-    const int p = pb + (threadIdx.x >> 3);
-    const bool valid = p < pend;
-    float u0 = 0.f, u1 = 0.f, u2 = 0.f;
-    if (valid) {
-      u0 = sv0[p];
-      u1 = sv1[p];
-      u2 = sv2[p];
-    }
-    const int j00 = (int)ceilf(u0 - halfw), j01 = (int)ceilf(u1 - halfw), j02 = (int)ceilf(u2 - halfw);
-    const bool tap = valid && a0 < W;
-    const float wo0 = tap ? es_eval_native((float)(j00 + a0) - u0, kc, beta_l2) : 0.f;
-    const float wo1 = tap ? es_eval_native((float)(j01 + a0) - u1, kc, beta_l2) : 0.f;
-    const float wo2 = tap ? es_eval_native((float)(j02 + a0) - u2, kc, beta_l2) : 0.f;
-    float w1[W], w2[W];
-#pragma unroll
-    for (int a = 0; a < W; ++a) {
-      w1[a] = __shfl(wo1, gbase + a);
-      w2[a] = __shfl(wo2, gbase + a);
-    }
-    const int l0 = min(max(j00 + W / 2, 0), kSuperEdge);
-    const int l1 = min(max(j01 + W / 2, 0), kSuperEdge);
-    const int l2 = min(max(j02 + W / 2, 0), kSuperEdge);
-    float ax[TC], ay[TC];
-#pragma unroll
-    for (int tt = 0; tt < TC; ++tt) ax[tt] = ay[tt] = 0.f;
-    if (tap) {
-#pragma unroll
-      for (int a1 = 0; a1 < W; ++a1) {
-        const float w01 = wo0 * w1[a1];
-        const int row = ((l0 + a0) * E + (l1 + a1)) * E + l2;
-#pragma unroll
-        for (int a2 = 0; a2 < W; ++a2) {
-          const float ww = w01 * w2[a2];
-#pragma unroll
-          for (int h = 0; h < TC / 2; ++h) {
-            const float4 v = tile4[(row + a2) * (TC / 2) + h];
-            ax[2 * h] += ww * v.x;
-            ay[2 * h] += ww * v.y;
-            ax[2 * h + 1] += ww * v.z;
-            ay[2 * h + 1] += ww * v.w;
-          }
-        }
-      }
-    }
-#pragma unroll
-    for (int tt = 0; tt < TC; ++tt) {
-#pragma unroll
-      for (int m = 1; m < 8; m <<= 1) {
-        ax[tt] += __shfl_xor(ax[tt], m);
-        ay[tt] += __shfl_xor(ay[tt], m);
-      }
-    }
-    if (valid && a0 < TC) {
-      // sorted, chunk-major output: cs[chunk][p][TC]; lane a0 writes transform a0 of its group
-      float rx = ax[0], ry = ay[0];
-#pragma unroll
-      for (int tt = 1; tt < TC; ++tt)
-        if (a0 == tt) {
-          rx = ax[tt];
-          ry = ay[tt];
-        }
-      c[((int64_t)blockIdx.y * M + p) * TC + a0] = Cplx<float>{rx, ry};
-    }
-  }
-}
-
-// Type-2 interpolation, tap-parallel: same LDS tile as k_interp, but each
-// wavefront takes one point at a time and lane q < W^2 owns the stencil row
-// (a0, a1) = (q / W, q % W), walking a2 = 0..W-1.  The lanes of one LDS read
-// touch W^2 distinct, regularly strided cells (few bank conflicts); the
-// per-row partial sums are then reduced across the wavefront with xor
-// shuffles.  The 3W kernel weights are evaluated once per point (one per
-// lane) and broadcast.
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
-  return v;
-}
-
-template <typename T, int W, int TC>
-__global__ void __launch_bounds__(kBlock)
-    k_interp_tp(const Cplx<T>* __restrict__ fine, Cplx<T>* __restrict__ c, SubDesc sd, BinGeom g,
-                const T* __restrict__ sv0, const T* __restrict__ sv1, const T* __restrict__ sv2,
-                int64_t M, int ntrans, T kc, T beta) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  Cplx<T>* tile = reinterpret_cast<Cplx<T>*>(smem_raw);
-  const int sub = xcd_remap(blockIdx.x, gridDim.x);
-  if (sub >= *sd.nsub) return;
-  const int t0 = blockIdx.y * TC;
-  const int sb = sd.sub_sb[sub];
-  const int s2 = sb % g.nsb[2];
-  const int s1 = (sb / g.nsb[2]) % g.nsb[1];
-  const int s0 = sb / (g.nsb[2] * g.nsb[1]);
-  constexpr int E = kSuperEdge + W;
-  constexpr int ncell = E * E * E;
-  const int o0 = s0 * kSuperEdge - W / 2, o1 = s1 * kSuperEdge - W / 2, o2 = s2 * kSuperEdge - W / 2;
-  const int64_t nfine = (int64_t)g.nf[0] * g.nf[1] * g.nf[2];
-  for (int cidx = threadIdx.x; cidx < ncell; cidx += blockDim.x) {
-    int e2 = cidx % E;
-    int e1 = (cidx / E) % E;
-    int e0 = cidx / (E * E);
-    int m0 = ((o0 + e0) % g.nf[0] + g.nf[0]) % g.nf[0];
-    int m1 = ((o1 + e1) % g.nf[1] + g.nf[1]) % g.nf[1];
-    int m2 = ((o2 + e2) % g.nf[2] + g.nf[2]) % g.nf[2];
-    int64_t off = ((int64_t)m0 * g.nf[1] + m1) * g.nf[2] + m2;
-#pragma unroll
-    for (int tt = 0; tt < TC; ++tt) {
-      int t = t0 + tt;
-      Cplx<T> v = {(T)0, (T)0};
-      if (t < ntrans) v = fine[t * nfine + off];
-      tile[cidx * TC + tt] = v;
-    }
-  }
-  __syncthreads();
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int nwave = blockDim.x >> 6;
-  const int p0 = sd.sub_pt[sub];
-  const int np = sd.sub_n[sub];
-  const T halfw = (T)(0.5 * W);
-  constexpr int NQ = (W * W + 63) / 64;
-  for (int p = p0 + wave; p < p0 + np; p += nwave) {
-    const T v0 = sv0[p], v1 = sv1[p], v2 = sv2[p];
-    const int j00 = (int)ceil(v0 - halfw), j01 = (int)ceil(v1 - halfw), j02 = (int)ceil(v2 - halfw);
-    T wv = (T)0;
-    if (lane < 3 * W) {
-      const int axis = lane / W, k = lane - axis * W;
-      const T v = axis == 0 ? v0 : (axis == 1 ? v1 : v2);
-      const int j0 = axis == 0 ? j00 : (axis == 1 ? j01 : j02);
-      wv = es_eval<T>((T)(j0 + k) - v, kc, beta);
-    }
-    const int l0 = min(max(j00 + W / 2, 0), kSuperEdge);
-    const int l1 = min(max(j01 + W / 2, 0), kSuperEdge);
-    const int l2 = min(max(j02 + W / 2, 0), kSuperEdge);
-    T wrow[NQ];
-#pragma unroll
-    for (int k = 0; k < NQ; ++k) {
-      const int q = lane + 64 * k;
-      const int a0 = min(q / W, W - 1), a1 = q % W;
-      wrow[k] = __shfl(wv, a0) * __shfl(wv, W + a1);
-    }
-    T w2[W];
-#pragma unroll
-    for (int a2 = 0; a2 < W; ++a2) w2[a2] = __shfl(wv, 2 * W + a2);
-    T ax[TC], ay[TC];
-#pragma unroll
-    for (int tt = 0; tt < TC; ++tt) ax[tt] = ay[tt] = (T)0;
-#pragma unroll
-    for (int k = 0; k < NQ; ++k) {
-      const int q = lane + 64 * k;
-      if (q < W * W) {
-        const int a0 = q / W, a1 = q % W;
-        const int base = ((l0 + a0) * E + (l1 + a1)) * E + l2;
-#pragma unroll
-        for (int a2 = 0; a2 < W; ++a2) {
-          const T w = wrow[k] * w2[a2];
-          if constexpr (TC % 2 == 0 && sizeof(T) == 4) {
-            const float4* cell = reinterpret_cast<const float4*>(tile + (base + a2) * TC);
-#pragma unroll
-            for (int h = 0; h < TC / 2; ++h) {
-              const float4 v = cell[h];
-              ax[2 * h] += w * v.x;
-              ay[2 * h] += w * v.y;
-              ax[2 * h + 1] += w * v.z;
-              ay[2 * h + 1] += w * v.w;
-            }
-          } else {
-#pragma unroll
-            for (int tt = 0; tt < TC; ++tt) {
-              const Cplx<T> v = tile[(base + a2) * TC + tt];
-              ax[tt] += w * v.x;
-              ay[tt] += w * v.y;
-            }
-          }
-        }
-      }
-    }
-#pragma unroll
-    for (int tt = 0; tt < TC; ++tt) {
-      ax[tt] = wave_sum(ax[tt]);
-      ay[tt] = wave_sum(ay[tt]);
-    }
-    if (lane == 0) {
-      Cplx<T>* dst = c + ((int64_t)blockIdx.y * M + p) * TC;
-#pragma unroll
-      for (int tt = 0; tt < TC; ++tt) dst[tt] = Cplx<T>{ax[tt], ay[tt]};
-    }
-  }
-}
-
 // Type-1 spreading as an output-driven gather: one 64-lane workgroup owns one
 // 4^3 bin of fine-grid cells (one cell per lane, TC transforms in registers)
 // and walks the points of the neighbouring bins in batches of up to 64.  A
@@ -1559,125 +1314,6 @@ __global__ void __launch_bounds__(kGatherBlock)
 
 __device__ __forceinline__ float lane_bcast(float v, int src) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
-}
-
-// Type-1 spreading as a gather without LDS staging (FP32).  Ownership is as
-// in k_spread_gather: one 64-lane workgroup per 4^3 bin, one cell per lane,
-// TC transforms in registers, each cell written once (deterministic, no
-// atomics).  The points of each neighbouring bin are taken 64 at a time:
-// every lane tests its own point against the block and a ballot lists the
-// hits.  For each hit, the point's coordinates are broadcast with readlane;
-// lanes 0..11 evaluate the 12 distinct kernel weights of the block (4 cell
-// offsets x 3 axes) and every lane picks its three with bpermute; the
-// point's TC values are read at a wave-uniform address (scalar loads) and
-// enter the FMAs as scalar operands.  The sorted values have row stride ldc =
-// gridDim.y * TC with zeros past ntrans, so every load is unconditional.
-// With TOUCH, each hit lane first reads its own value row with vector loads,
-// which leaves the rows in L2 for the scalar loads (measured: 28.4 -> 24.8 ms
-// at L=128, r=20).  Nothing occupies LDS, so occupancy is bounded by
-// registers only.
-template <int TC, bool TOUCH>
-__global__ void __launch_bounds__(kGatherBlock)
-    k_spread_sc(const Cplx<float>* __restrict__ cs, Cplx<float>* __restrict__ fine, BinGeom g,
-                const int* __restrict__ bin_start, const int* __restrict__ bin_count,
-                const float* __restrict__ sv0, const float* __restrict__ sv1,
-                const float* __restrict__ sv2, int ntrans, int ldc, int W, float kc, float beta_l2) {
-  const int t0 = blockIdx.y * TC;
-  const int lane = threadIdx.x;
-  int b0, b1, b2;
-  gather_bin(blockIdx.x, gridDim.x, g, b0, b1, b2);
-  const int c0 = lane >> 4, c1 = (lane >> 2) & 3, c2 = lane & 3;
-  const float halfw = 0.5f * (float)W;
-  const int R = (W + 2 * kBinEdge - 1) / (2 * kBinEdge);
-  // weight lanes: axis wa = lane / 4 (lanes >= 12 duplicate axis 2), cell offset wc = lane % 4
-  const int wa = min(lane >> 2, 2), wc = lane & 3;
-  float ax[TC], ay[TC];
-#pragma unroll
-  for (int tt = 0; tt < TC; ++tt) ax[tt] = ay[tt] = 0.f;
-
-  const int nside = 2 * R + 1;
-  const int nnb = nside * nside * nside;
-  // neighbour bin nbi of this block: its bin index and the block's first cell in the neighbour's
-  // super-bin frame (unwrapped across the periodic boundary)
-  auto neighbour = [&](int nbi, int& nbin, int& q0, int& q1, int& q2) {
-    const int d0 = nbi / (nside * nside) - R, d1 = (nbi / nside) % nside - R, d2 = nbi % nside - R;
-    int n0 = b0 + d0, n1 = b1 + d1, n2 = b2 + d2, sh0 = 0, sh1 = 0, sh2 = 0;
-    if (n0 < 0) { n0 += g.nb[0]; sh0 = -g.nf[0]; }
-    if (n0 >= g.nb[0]) { n0 -= g.nb[0]; sh0 = g.nf[0]; }
-    if (n1 < 0) { n1 += g.nb[1]; sh1 = -g.nf[1]; }
-    if (n1 >= g.nb[1]) { n1 -= g.nb[1]; sh1 = g.nf[1]; }
-    if (n2 < 0) { n2 += g.nb[2]; sh2 = -g.nf[2]; }
-    if (n2 >= g.nb[2]) { n2 -= g.nb[2]; sh2 = g.nf[2]; }
-    nbin = bin_index(g, n0, n1, n2);
-    q0 = kBinEdge * b0 - (n0 >> 1) * kSuperEdge - sh0;
-    q1 = kBinEdge * b1 - (n1 >> 1) * kSuperEdge - sh1;
-    q2 = kBinEdge * b2 - (n2 >> 1) * kSuperEdge - sh2;
-  };
-  // TOUCH: before the hits of a batch read their value rows with scalar loads, each hit lane
-  // reads its own row once with vector loads (one per 64-byte line), which leaves the rows in
-  // L2 for the scalar loads (these alone are not served from L2 across workgroups as well)
-  auto touch_row = [&](int pt) {
-    const float* row = reinterpret_cast<const float*>(cs + (int64_t)pt * ldc + t0);
-    float t = 0.f;
-#pragma unroll
-    for (int o = 0; o < 2 * TC; o += 16) t += row[o];
-    return t;
-  };
-  for (int nbi = 0; nbi < nnb; ++nbi) {
-        int nbin, q0, q1, q2;
-        neighbour(nbi, nbin, q0, q1, q2);
-        const int start = bin_start[nbin];
-        const int end = start + bin_count[nbin];
-        const int qa = wa == 0 ? q0 : (wa == 1 ? q1 : q2);
-        for (int base = start; base < end; base += kGatherBlock) {
-          float v0 = 0.f, v1 = 0.f, v2 = 0.f;
-          bool hit = false;
-          if (base + lane < end) {
-            v0 = sv0[base + lane];
-            v1 = sv1[base + lane];
-            v2 = sv2[base + lane];
-            const int j0 = (int)ceilf(v0 - halfw), j1 = (int)ceilf(v1 - halfw), j2 = (int)ceilf(v2 - halfw);
-            hit = !(q0 > j0 + W - 1 || q0 + 3 < j0 || q1 > j1 + W - 1 || q1 + 3 < j1 || q2 > j2 + W - 1 ||
-                    q2 + 3 < j2);
-          }
-          unsigned long long mask = __ballot(hit);
-          if constexpr (TOUCH) {
-            if (hit) {
-              const float t = touch_row(base + lane);
-              asm volatile("" ::"v"(t));
-            }
-          }
-          while (mask) {
-            const int q = __builtin_ctzll(mask);
-            mask &= mask - 1;
-            // all TC values at once (rows are zero-padded to ldc): wide scalar loads issued
-            // before the weights are formed, one wait
-            const Cplx<float>* src = cs + (int64_t)(base + q) * ldc + t0;
-            Cplx<float> v[TC];
-#pragma unroll
-            for (int tt = 0; tt < TC; ++tt) v[tt] = src[tt];
-            const float p0 = lane_bcast(v0, q), p1 = lane_bcast(v1, q), p2 = lane_bcast(v2, q);
-            const float pa = wa == 0 ? p0 : (wa == 1 ? p1 : p2);
-            const int j0a = (int)ceilf(pa - halfw);
-            const int xa = qa + wc - j0a;
-            const float wl = (xa >= 0 && xa < W) ? es_eval_native((float)(j0a + xa) - pa, kc, beta_l2) : 0.f;
-            const float w = __shfl(wl, c0) * __shfl(wl, 4 + c1) * __shfl(wl, 8 + c2);
-#pragma unroll
-            for (int tt = 0; tt < TC; ++tt) {
-              ax[tt] += w * v[tt].x;
-              ay[tt] += w * v[tt].y;
-            }
-          }
-        }
-      }
-  const int g0 = kBinEdge * b0 + c0, g1 = kBinEdge * b1 + c1, g2 = kBinEdge * b2 + c2;
-  const int64_t nfine = (int64_t)g.nf[0] * g.nf[1] * g.nf[2];
-  const int64_t off = ((int64_t)g0 * g.nf[1] + g1) * g.nf[2] + g2;
-#pragma unroll
-  for (int tt = 0; tt < TC; ++tt) {
-    const int t = t0 + tt;
-    if (t < ntrans) fine[t * nfine + off] = Cplx<float>{ax[tt], ay[tt]};
-  }
 }
 
 // Type-1 spreading of one 4^3 bin as an FP32 matrix product on the matrix
@@ -1979,113 +1615,6 @@ __global__ void __launch_bounds__(kGatherBlock) __attribute__((amdgpu_waves_per_
   }
 }
 
-// Type-1 spreading, point-driven ("tap-parallel"): one workgroup per
-// (super-bin subproblem, transform chunk) accumulates into an LDS tile
-// [TC][re|im][cell] of the 8^3 super-bin plus halo.  Each wavefront takes one
-// point at a time; lane q < W^2 owns the stencil row (a0, a1) = (q / W, q % W)
-// and walks a2 = 0..W-1, so the W^2 lanes of one ds_add_f32 hit W^2 distinct
-// cells (no same-address serialisation).  The 3W kernel weights are computed
-// once per point by 3W lanes and broadcast with readlane / bpermute.  The
-// tile is then added to the periodic fine grid with float atomics (tiles of
-// neighbouring super-bins overlap in their halos).
-template <typename T, int W, int TC>
-__global__ void __launch_bounds__(kBlock)
-    k_spread_tiles(const Cplx<T>* __restrict__ cs, Cplx<T>* __restrict__ fine, SubDesc sd, BinGeom g,
-                   const T* __restrict__ sv0, const T* __restrict__ sv1, const T* __restrict__ sv2,
-                   int64_t M, int ntrans, T kc, T beta) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* tile = reinterpret_cast<T*>(smem_raw);
-  const int sub = xcd_remap(blockIdx.x, gridDim.x);
-  if (sub >= *sd.nsub) return;
-  const int t0 = blockIdx.y * TC;
-  const int sb = sd.sub_sb[sub];
-  const int s2 = sb % g.nsb[2];
-  const int s1 = (sb / g.nsb[2]) % g.nsb[1];
-  const int s0 = sb / (g.nsb[2] * g.nsb[1]);
-  constexpr int E = kSuperEdge + W;
-  constexpr int ncell = E * E * E;
-  for (int i = threadIdx.x; i < ncell * TC * 2; i += blockDim.x) tile[i] = (T)0;
-  __syncthreads();
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int nwave = blockDim.x >> 6;
-  const int p0 = sd.sub_pt[sub];
-  const int np = sd.sub_n[sub];
-  const T halfw = (T)(0.5 * W);
-  for (int p = p0 + wave; p < p0 + np; p += nwave) {
-    // weights: lane j < 3W evaluates axis j / W, tap j % W
-    const T v0 = sv0[p], v1 = sv1[p], v2 = sv2[p];
-    const int j00 = (int)ceil(v0 - halfw), j01 = (int)ceil(v1 - halfw), j02 = (int)ceil(v2 - halfw);
-    T wv = (T)0;
-    if (lane < 3 * W) {
-      const int axis = lane / W, k = lane - axis * W;
-      const T v = axis == 0 ? v0 : (axis == 1 ? v1 : v2);
-      const int j0 = axis == 0 ? j00 : (axis == 1 ? j01 : j02);
-      wv = es_eval<T>((T)(j0 + k) - v, kc, beta);
-    }
-    const int l0 = min(max(j00 + W / 2, 0), kSuperEdge);
-    const int l1 = min(max(j01 + W / 2, 0), kSuperEdge);
-    const int l2 = min(max(j02 + W / 2, 0), kSuperEdge);
-    Cplx<T> cv[TC];
-#pragma unroll
-    for (int tt = 0; tt < TC; ++tt) {
-      const int t = t0 + tt;
-      cv[tt] = t < ntrans ? cs[(int64_t)p * ntrans + t] : Cplx<T>{(T)0, (T)0};
-    }
-    // broadcasts with the whole wave active: the (a0, a1) row weight of each of
-    // this lane's stencil rows and the W weights along axis 2
-    constexpr int NQ = (W * W + 63) / 64;
-    T wrow[NQ];
-#pragma unroll
-    for (int k = 0; k < NQ; ++k) {
-      const int q = lane + 64 * k;
-      const int a0 = min(q / W, W - 1), a1 = q % W;
-      wrow[k] = __shfl(wv, a0) * __shfl(wv, W + a1);
-    }
-    T w2[W];
-#pragma unroll
-    for (int a2 = 0; a2 < W; ++a2) w2[a2] = __shfl(wv, 2 * W + a2);
-#pragma unroll
-    for (int k = 0; k < NQ; ++k) {
-      const int q = lane + 64 * k;
-      if (q < W * W) {
-        const int a0 = q / W, a1 = q % W;
-        const int base = ((l0 + a0) * E + (l1 + a1)) * E + l2;
-#pragma unroll
-        for (int a2 = 0; a2 < W; ++a2) {
-          const T w = wrow[k] * w2[a2];
-#pragma unroll
-          for (int tt = 0; tt < TC; ++tt) {
-            atomicAdd(tile + (2 * tt) * ncell + base + a2, w * cv[tt].x);
-            atomicAdd(tile + (2 * tt + 1) * ncell + base + a2, w * cv[tt].y);
-          }
-        }
-      }
-    }
-  }
-  __syncthreads();
-  const int64_t nfine = (int64_t)g.nf[0] * g.nf[1] * g.nf[2];
-  const int o0 = s0 * kSuperEdge - W / 2, o1 = s1 * kSuperEdge - W / 2, o2 = s2 * kSuperEdge - W / 2;
-  T* finef = reinterpret_cast<T*>(fine);
-  for (int cidx = threadIdx.x; cidx < ncell; cidx += blockDim.x) {
-    const int e2 = cidx % E;
-    const int e1 = (cidx / E) % E;
-    const int e0 = cidx / (E * E);
-    const int m0 = ((o0 + e0) % g.nf[0] + g.nf[0]) % g.nf[0];
-    const int m1 = ((o1 + e1) % g.nf[1] + g.nf[1]) % g.nf[1];
-    const int m2 = ((o2 + e2) % g.nf[2] + g.nf[2]) % g.nf[2];
-    const int64_t off = ((int64_t)m0 * g.nf[1] + m1) * g.nf[2] + m2;
-#pragma unroll
-    for (int tt = 0; tt < TC; ++tt) {
-      const int t = t0 + tt;
-      if (t >= ntrans) break;
-      const T vx = tile[(2 * tt) * ncell + cidx], vy = tile[(2 * tt + 1) * ncell + cidx];
-      if (vx != (T)0) atomicAdd(finef + 2 * (t * nfine + off), vx);
-      if (vy != (T)0) atomicAdd(finef + 2 * (t * nfine + off) + 1, vy);
-    }
-  }
-}
-
 // ----------------------------------------------------------------------------
 // Plan
 // ----------------------------------------------------------------------------
@@ -2099,16 +1628,12 @@ struct Plan {
   int64_t ntrans = 1;
   int w = 7;
   int tc = 2;  // transforms per interpolation workgroup
-  int spread_mode = 0;  // 0 = LDS-staged gather, 1 = LDS atomic tiles, 2/3 = LDS-free gather (FP32; 3 with L2 touch), 4 = FP32 MFMA gather
-  int interp_mode = 0;  // 0 = thread per point, 1 = wavefront per point, 2 = eight lanes per point
-  int interp_direct = 1;  // mode 0 writes the (ntrans, M) output directly instead of sorted + permute
-  int interp_pipe = 1;    // mode 0, FP32 with point records: k_interp_pipe instead of k_interp
+  int interp_pipe = 1;    // FP32 with point records: k_interp_pipe instead of k_interp
   int pipe_grid = 0;      // k_interp_pipe's grid: resident workgroups on the device, a multiple of 8
   int* pipe_ctr = nullptr;  // k_interp_pipe's per-XCD item counters (8 ints)
   EsPoly poly{};            // polynomial stencil weights of the FP32 interpolation kernels
   bool presorted = false;  // the current type-1 call's values are already in csort (no permute)
   int max_tc_gather = 20;  // transforms per gather workgroup
-  int tc_spread = 2;    // transforms per spreading workgroup (tile mode)
   double beta = 0, eps = 1e-6;
   BinGeom geom;
   int64_t nbins = 0, nsuper = 0;
@@ -2197,80 +1722,49 @@ static int free_points(Plan* p) {
 
 template <typename T, int W, int TC>
 static int launch_interp(Plan* p, void* c_out, int ntrans_run, hipStream_t st) {
-  const size_t smem = (size_t)(kSuperEdge + W) * (kSuperEdge + W) * (kSuperEdge + W) * TC * sizeof(Cplx<T>);
   // k_interp's tile may pad its planes (interp_plane)
   const size_t smem_k = (size_t)(kSuperEdge + W) * interp_plane<T, W, TC>() * TC * sizeof(Cplx<T>);
   COV3D_CHECK_ARG(smem_k <= 160 * 1024, "NUFFT tile of %zu bytes exceeds LDS", smem_k);
   static bool attr_done = false;
-  if (std::max(smem, smem_k) > 65536 && !attr_done) {
+  if (smem_k > 65536 && !attr_done) {
     COV3D_HIP(hipFuncSetAttribute((const void*)k_interp<T, W, TC>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    COV3D_HIP(hipFuncSetAttribute((const void*)k_interp_tp<T, W, TC>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    if constexpr (sizeof(T) == 4 && W <= 8 && TC % 2 == 0)
-      COV3D_HIP(hipFuncSetAttribute((const void*)k_interp_rows<W, TC>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     attr_done = true;
   }
   SubDesc sd{p->sub_sb, p->sub_pt, p->sub_n, p->nsub};
   dim3 grid((unsigned)p->sub_cap, (unsigned)ceil_div(ntrans_run, TC));
   const T kc = (T)(4.0 / ((double)W * W));
-  // thread-per-point interpolation can write the caller's layout itself (no permute pass)
-  const bool direct = p->interp_direct && p->interp_mode == 0;
-  {
-    ScopedTiming tm(kTInterp, st);
-    bool done = false;
-    if constexpr (sizeof(T) == 4 && W <= 8 && TC % 2 == 0) {
-      if (p->interp_mode == 2) {
-        hipLaunchKernelGGL((k_interp_rows<W, TC>), grid, dim3(kBlock), smem, st, (const Cplx<float>*)p->fine,
-                           (Cplx<float>*)p->csort, sd, p->geom, (const float*)p->sv[0], (const float*)p->sv[1],
-                           (const float*)p->sv[2], p->M, ntrans_run, (float)kc, (float)p->beta, p->stage_half,
-                           p->stage_conj);
-        done = true;
+  ScopedTiming tm(kTInterp, st);
+  // both kernels write the caller's (ntrans, M) layout directly (no permute pass)
+  if constexpr (sizeof(T) == 4 && W <= 8 && TC == 2) {
+    if (p->interp_pipe && p->irec_valid) {
+      if (p->pipe_grid == 0) {
+        COV3D_HIP(hipFuncSetAttribute((const void*)k_interp_pipe<W>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)smem_k));
+        int dev = 0, ncu = 0, per_cu = 0;
+        COV3D_HIP(hipGetDevice(&dev));
+        COV3D_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        COV3D_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_interp_pipe<W>, kBlock,
+                                                               smem_k));
+        p->pipe_grid = std::max(8, (ncu * std::max(per_cu, 1)) / 8 * 8);
+        COV3D_HIP(hipMalloc(&p->pipe_ctr, 8 * sizeof(int)));
       }
+      // the XCD counters start at zero (a kernel, not a memset node, so that the launch
+      // sequence stays capturable)
+      hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, p->pipe_ctr, (int64_t)8, 0);
+      hipLaunchKernelGGL((k_interp_pipe<W>), dim3((unsigned)p->pipe_grid), dim3(kBlock), smem_k, st,
+                         (const Cplx<float>*)p->fine, (Cplx<float>*)p->csort, sd, p->geom,
+                         (const float4*)p->irec, (const int*)p->iidx, p->M, ntrans_run, (float)kc,
+                         (float)p->beta, p->stage_half, p->stage_conj, (Cplx<float>*)c_out, p->pipe_ctr, p->poly);
+      COV3D_LAUNCH_CHECK();
+      return 0;
     }
-    if constexpr (sizeof(T) == 4 && W <= 8 && TC == 2) {
-      if (!done && p->interp_mode == 0 && p->interp_pipe && p->irec_valid) {
-        if (p->pipe_grid == 0) {
-          COV3D_HIP(hipFuncSetAttribute((const void*)k_interp_pipe<W>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                        (int)smem_k));
-          int dev = 0, ncu = 0, per_cu = 0;
-          COV3D_HIP(hipGetDevice(&dev));
-          COV3D_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-          COV3D_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_interp_pipe<W>, kBlock,
-                                                                 smem_k));
-          p->pipe_grid = std::max(8, (ncu * std::max(per_cu, 1)) / 8 * 8);
-          COV3D_HIP(hipMalloc(&p->pipe_ctr, 8 * sizeof(int)));
-        }
-        // the XCD counters start at zero (a kernel, not a memset node, so that the launch
-        // sequence stays capturable)
-        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, p->pipe_ctr, (int64_t)8, 0);
-        hipLaunchKernelGGL((k_interp_pipe<W>), dim3((unsigned)p->pipe_grid), dim3(kBlock), smem_k, st,
-                           (const Cplx<float>*)p->fine, (Cplx<float>*)p->csort, sd, p->geom,
-                           (const float4*)p->irec, (const int*)p->iidx, p->M, ntrans_run, (float)kc,
-                           (float)p->beta, p->stage_half, p->stage_conj,
-                           direct ? (Cplx<float>*)c_out : nullptr, p->pipe_ctr, p->poly);
-        done = true;
-      }
-    }
-    if (done) {
-    } else if (p->interp_mode == 1)
-      hipLaunchKernelGGL((k_interp_tp<T, W, TC>), grid, dim3(kBlock), smem, st, (const Cplx<T>*)p->fine,
-                         (Cplx<T>*)p->csort, sd, p->geom, (const T*)p->sv[0], (const T*)p->sv[1],
-                         (const T*)p->sv[2], p->M, ntrans_run, kc, (T)p->beta);
-    else
-      hipLaunchKernelGGL((k_interp<T, W, TC>), grid, dim3(kBlock), smem_k, st, (const Cplx<T>*)p->fine,
-                         (Cplx<T>*)p->csort, sd, p->geom, (const T*)p->sv[0], (const T*)p->sv[1],
-                         (const T*)p->sv[2], p->sidx, p->M, ntrans_run, kc, (T)p->beta, p->stage_half,
-                         p->stage_conj, (TC == 2 && p->irec_valid) ? (const float4*)p->irec : nullptr,
-                         (TC == 2 && p->irec_valid) ? (const int*)p->iidx : nullptr,
-                         direct ? (Cplx<T>*)c_out : nullptr, p->poly);
   }
-  COV3D_LAUNCH_CHECK();
-  if (direct) return 0;
-  ScopedTiming tm2(kTPermute, st);
-  hipLaunchKernelGGL((k_from_sorted<T>), dim3((unsigned)ceil_div(p->M, kBlock)), dim3(kBlock), 0, st,
-                     (const Cplx<T>*)p->csort, (Cplx<T>*)c_out, p->spos, p->M, ntrans_run, TC);
+  hipLaunchKernelGGL((k_interp<T, W, TC>), grid, dim3(kBlock), smem_k, st, (const Cplx<T>*)p->fine,
+                     (Cplx<T>*)p->csort, sd, p->geom, (const T*)p->sv[0], (const T*)p->sv[1],
+                     (const T*)p->sv[2], p->sidx, p->M, ntrans_run, kc, (T)p->beta, p->stage_half,
+                     p->stage_conj, (TC == 2 && p->irec_valid) ? (const float4*)p->irec : nullptr,
+                     (TC == 2 && p->irec_valid) ? (const int*)p->iidx : nullptr, (Cplx<T>*)c_out, p->poly);
   COV3D_LAUNCH_CHECK();
   return 0;
 }
@@ -2293,15 +1787,17 @@ static int dispatch_interp_w(Plan* p, void* c_out, int ntrans_run, hipStream_t s
 static int run_interp(Plan* p, void* c_out, int ntrans_run, hipStream_t st) {
   if (p->M == 0) return 0;
   if (p->dtype == 1) return dispatch_interp_w<double, 1>(p, c_out, ntrans_run, st);
-  if (p->tc == 4) return dispatch_interp_w<float, 4>(p, c_out, ntrans_run, st);
-  if (p->tc == 1) return dispatch_interp_w<float, 1>(p, c_out, ntrans_run, st);
   return dispatch_interp_w<float, 2>(p, c_out, ntrans_run, st);
 }
 
+// FP32 plans spread on the matrix cores (kernel widths up to 8, which covers every FP32 tolerance:
+// eps >= 10 * FLT_EPSILON gives w <= 7); FP64 plans use the LDS-staged gather.
+static bool spread_on_mfma(const Plan* p) { return p->dtype == 0 && p->w <= 8; }
+
 template <typename T, int TC>
 static int launch_spread(Plan* p, const void* c_in, int ntrans_run, hipStream_t st) {
-  const bool sc = sizeof(T) == 4 && (p->spread_mode >= 2 && p->spread_mode <= 4);
-  const int ldc = sc ? (int)ceil_div(ntrans_run, TC) * TC : ntrans_run;
+  const bool mfma = spread_on_mfma(p);
+  const int ldc = mfma ? (int)ceil_div(ntrans_run, TC) * TC : ntrans_run;
   COV3D_CHECK_ARG(ldc <= p->ldc_cap, "sorted row stride %d exceeds %d", ldc, p->ldc_cap);
   if (!p->presorted) {
     ScopedTiming tm(kTPermute, st);
@@ -2317,27 +1813,11 @@ static int launch_spread(Plan* p, const void* c_in, int ntrans_run, hipStream_t 
   const T kc = (T)(4.0 / ((double)p->w * p->w));
   ScopedTiming tm(kTSpread, st);
   if constexpr (sizeof(T) == 4) {
-    if (p->spread_mode == 4 && p->w <= 8) {
+    if (mfma) {
       // 32-bit row offsets when every float index of the sorted input fits in 32 bits
       const bool off32 = ((int64_t)p->M * ldc + kCsortSlack) * 2 < ((int64_t)1 << 32);
       auto k = off32 ? k_spread_mfma<TC, true> : k_spread_mfma<TC, false>;
       hipLaunchKernelGGL(k, grid, dim3(kGatherBlock), 0, st, (const Cplx<float>*)p->csort,
-                         (Cplx<float>*)p->fine, p->geom, p->bin_start, p->bin_count, (const float*)p->sv[0],
-                         (const float*)p->sv[1], (const float*)p->sv[2], ntrans_run, ldc, p->w, (float)kc,
-                         (float)(p->beta * 1.4426950408889634));
-      COV3D_LAUNCH_CHECK();
-      return 0;
-    }
-    if (p->spread_mode == 3 || p->spread_mode == 4) {
-      hipLaunchKernelGGL((k_spread_sc<TC, true>), grid, dim3(kGatherBlock), 0, st, (const Cplx<float>*)p->csort,
-                         (Cplx<float>*)p->fine, p->geom, p->bin_start, p->bin_count, (const float*)p->sv[0],
-                         (const float*)p->sv[1], (const float*)p->sv[2], ntrans_run, ldc, p->w, (float)kc,
-                         (float)(p->beta * 1.4426950408889634));
-      COV3D_LAUNCH_CHECK();
-      return 0;
-    }
-    if (p->spread_mode == 2) {
-      hipLaunchKernelGGL((k_spread_sc<TC, false>), grid, dim3(kGatherBlock), 0, st, (const Cplx<float>*)p->csort,
                          (Cplx<float>*)p->fine, p->geom, p->bin_start, p->bin_count, (const float*)p->sv[0],
                          (const float*)p->sv[1], (const float*)p->sv[2], ntrans_run, ldc, p->w, (float)kc,
                          (float)(p->beta * 1.4426950408889634));
@@ -2351,46 +1831,6 @@ static int launch_spread(Plan* p, const void* c_in, int ntrans_run, hipStream_t 
                      (T)p->beta);
   COV3D_LAUNCH_CHECK();
   return 0;
-}
-
-template <typename T, int W, int TC>
-static int launch_spread_tiles(Plan* p, int ntrans_run, hipStream_t st) {
-  const size_t smem = (size_t)(kSuperEdge + W) * (kSuperEdge + W) * (kSuperEdge + W) * TC * 2 * sizeof(T);
-  COV3D_CHECK_ARG(smem <= 160 * 1024, "spread tile of %zu bytes exceeds LDS", smem);
-  static bool attr_done = false;
-  if (smem > 65536 && !attr_done) {
-    COV3D_HIP(hipFuncSetAttribute((const void*)k_spread_tiles<T, W, TC>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    attr_done = true;
-  }
-  const int64_t total = p->nf[0] * p->nf[1] * p->nf[2] * (int64_t)ntrans_run * 2;
-  hipLaunchKernelGGL((k_fill_zero<T>), dim3((unsigned)std::min<int64_t>(ceil_div(total, kBlock), 65536)),
-                     dim3(kBlock), 0, st, (T*)p->fine, total);
-  COV3D_LAUNCH_CHECK();
-  SubDesc sd{p->sub_sb, p->sub_pt, p->sub_n, p->nsub};
-  dim3 grid((unsigned)p->sub_cap, (unsigned)ceil_div(ntrans_run, TC));
-  const T kc = (T)(4.0 / ((double)W * W));
-  ScopedTiming tm(kTSpread, st);
-  hipLaunchKernelGGL((k_spread_tiles<T, W, TC>), grid, dim3(kBlock), smem, st, (const Cplx<T>*)p->csort,
-                     (Cplx<T>*)p->fine, sd, p->geom, (const T*)p->sv[0], (const T*)p->sv[1],
-                     (const T*)p->sv[2], p->M, ntrans_run, kc, (T)p->beta);
-  COV3D_LAUNCH_CHECK();
-  return 0;
-}
-
-template <typename T, int TC>
-static int dispatch_spread_tiles_w(Plan* p, int ntrans_run, hipStream_t st) {
-  switch (p->w) {
-#define COV3D_W(WW) \
-  case WW:          \
-    return launch_spread_tiles<T, WW, TC>(p, ntrans_run, st);
-    COV3D_W(2) COV3D_W(3) COV3D_W(4) COV3D_W(5) COV3D_W(6) COV3D_W(7) COV3D_W(8) COV3D_W(9)
-    COV3D_W(10) COV3D_W(11) COV3D_W(12) COV3D_W(13) COV3D_W(14) COV3D_W(15) COV3D_W(16)
-#undef COV3D_W
-    default:
-      set_error("unsupported kernel width %d", p->w);
-      return COV3D_ERR_ARG;
-  }
 }
 
 // Transforms per gather workgroup: at most max_tc (<= 20) in registers,
@@ -2411,26 +1851,11 @@ static int gather_ldc(int n, int max_tc) {
 
 // Row stride of the sorted values the spreading of n transforms reads (launch_spread / run_spread).
 static int spread_ldc(const Plan* p, int n) {
-  const bool sc = p->dtype == 0 && p->spread_mode >= 2 && p->spread_mode <= 4;
-  return sc ? gather_ldc(n, p->max_tc_gather) : n;
+  return spread_on_mfma(p) ? gather_ldc(n, p->max_tc_gather) : n;
 }
 
 template <typename T>
 static int run_spread(Plan* p, const void* c_in, int ntrans_run, hipStream_t st) {
-  if (p->spread_mode == 1) {
-    if (!p->presorted) {
-      ScopedTiming tm(kTPermute, st);
-      hipLaunchKernelGGL((k_to_sorted<T>), dim3((unsigned)ceil_div(p->M, kBlock)), dim3(kBlock), 0, st,
-                         (const Cplx<T>*)c_in, (Cplx<T>*)p->csort, p->spos, p->M, ntrans_run, ntrans_run);
-      COV3D_LAUNCH_CHECK();
-    }
-    if (p->dtype == 1 || p->tc_spread == 1) return dispatch_spread_tiles_w<T, 1>(p, ntrans_run, st);
-    if constexpr (sizeof(T) == 4) {
-      if (p->tc_spread == 4) return dispatch_spread_tiles_w<T, 4>(p, ntrans_run, st);
-      return dispatch_spread_tiles_w<T, 2>(p, ntrans_run, st);
-    }
-    return dispatch_spread_tiles_w<T, 1>(p, ntrans_run, st);
-  }
   switch (gather_tc(ntrans_run, p->max_tc_gather)) {
     case 1: return launch_spread<T, 1>(p, c_in, ntrans_run, st);
     case 2: return launch_spread<T, 2>(p, c_in, ntrans_run, st);
@@ -2512,7 +1937,7 @@ static int exec_type2(Plan* p, const void* f, bool f_real, void* c, int64_t nrun
   const int64_t nfine = p->nf[0] * p->nf[1] * p->nf[2];
   const int64_t total = nfine * p->ntrans;
   dim3 g1((unsigned)ceil_div(total, kBlock));
-  if (f_real && p->real_fft && p->interp_mode != 1) {
+  if (f_real && p->real_fft) {
     // real volume: real fine grid -> R2C (Hermitian half) -> interpolation mirrors m2 > nf2/2
     int rc = ensure_real(p);
     if (rc) return rc;
@@ -2795,29 +2220,14 @@ static int make_plan(int type, const int64_t* n, int iflag, int64_t ntrans, doub
   // the ES kernel's exp/sqrt form is the default; COV3D_ES_POLY=1 selects the polynomial form
   // in FP32 interpolation (measured 8.58 -> 8.38 ms at L=128, r=20, 1024 images)
   p->poly.on = env_int("COV3D_ES_POLY", 0) != 0 ? 1 : 0;
-  const int tc_env = env_int("COV3D_NUFFT_TC", 2);
-  p->tc = dtype == 1 ? 1 : (tc_env == 1 || tc_env == 4 ? tc_env : 2);
-  {
-    // 4 = FP32 matrix-core gather (FP32 default; W > 8 uses 3), 3 = LDS-free gather with L2 touch,
-    // 2 = the same without the touch, 0 = LDS-staged gather (FP64 default), 1 = LDS atomic tiles
-    const int sm = env_int("COV3D_SPREAD_MODE", p->dtype == 0 ? 4 : 0);
-    p->spread_mode = (sm == 1 || ((sm >= 2 && sm <= 4) && p->dtype == 0)) ? sm : 0;
-  }
-  {
-    // 0 = thread per point (default), 1 = wavefront per point, 2 = eight lanes per point (FP32, W <= 8)
-    const int im = env_int("COV3D_INTERP_MODE", 0);
-    p->interp_mode = (im == 1 || im == 2) ? im : 0;
-    // measured at L=128, r=20, 1024 images: interp +0.5 ms, the 0.9 ms permute pass saved
-    p->interp_direct = env_int("COV3D_INTERP_DIRECT", 1) != 0 ? 1 : 0;
-    // thread-per-point FP32 plans with point records run the persistent pipelined kernel
-    p->interp_pipe = env_int("COV3D_INTERP_PIPE", 1) != 0 ? 1 : 0;
-  }
-  p->max_tc_gather = std::min(std::max(env_int("COV3D_GATHER_TC", 20), 1), 20);
+  p->tc = dtype == 1 ? 1 : 2;  // interpolation transforms per tile: two FP32 values make one 16-byte cell
+  // the persistent pipelined interpolation (default) or the one-workgroup-per-item kernel with the
+  // same arithmetic (COV3D_INTERP_PIPE=0; the tests compare the two bit for bit)
+  p->interp_pipe = env_int("COV3D_INTERP_PIPE", 1) != 0 ? 1 : 0;
+  p->max_tc_gather = 20;
   p->ldc_cap = (int)ceil_div(ntrans, 4) * 4;
   for (int n_run = 1; n_run <= ntrans; ++n_run)
     p->ldc_cap = std::max(p->ldc_cap, gather_ldc(n_run, p->max_tc_gather));
-  const int tcs_env = env_int("COV3D_SPREAD_TC", 2);
-  p->tc_spread = dtype == 1 ? 1 : (tcs_env == 1 || tcs_env == 4 ? tcs_env : 2);
   const size_t csize = dtype == 1 ? 16 : 8;
   for (int k = 0; k < 3; ++k) {
     p->N[k] = n[k];
@@ -2873,7 +2283,7 @@ static int make_plan(int type, const int64_t* n, int iflag, int64_t ntrans, doub
   size_t w1 = 0, w2 = 0;
   rocfft_plan_get_work_buffer_size(p->fft_fwd, &w1);
   rocfft_plan_get_work_buffer_size(p->fft_bwd, &w2);
-  p->real_fft = env_int("COV3D_REAL_FFT", 1) != 0 ? 1 : 0;
+  p->real_fft = 1;
   {
     const size_t tsz = dtype == 1 ? 8 : 4;
     p->rbuf_bytes = (size_t)ntrans * std::max<size_t>((size_t)nfine * tsz,

@@ -155,21 +155,14 @@ def test_vol_forward_fourier_slice_closed_form(dev):
     assert rel_l2(lhs.cpu().numpy(), rhs.cpu().numpy()) < 1e-5
 
 
-@pytest.mark.parametrize("spread_mode,interp_mode,ntrans,direct,pipe", [
-    ("0", "2", 3, "1", "1"), ("2", "1", 5, "1", "1"), ("0", "0", 20, "1", "1"), ("2", "2", 21, "1", "1"),
-    ("3", "0", 7, "0", "1"), ("4", "0", 20, "1", "1"), ("4", "0", 3, "0", "1"), ("4", "2", 21, "1", "1"),
-    ("4", "0", 20, "1", "0"), ("4", "0", 3, "0", "0")])
-def test_kernel_variants_match_nudft(dev, monkeypatch, spread_mode, interp_mode, ntrans, direct, pipe):
-    """Every selectable spreading / interpolation kernel (read at plan creation from
-    COV3D_SPREAD_MODE / COV3D_INTERP_MODE; COV3D_INTERP_DIRECT=0 keeps the sorted output and
-    its permute pass; COV3D_INTERP_PIPE=0 runs the one-workgroup-per-item interpolation instead
-    of the persistent pipelined one) against the NUDFT, including transform
-    counts that leave partial chunks (3, 5, 21) and the 20-transform SGD shape."""
+@pytest.mark.parametrize("ntrans,pipe", [(3, "1"), (5, "1"), (7, "1"), (20, "1"), (21, "1"), (20, "0"), (3, "0")])
+def test_kernel_variants_match_nudft(dev, monkeypatch, ntrans, pipe):
+    """The FP32 interpolation kernels (the persistent pipelined one, and with COV3D_INTERP_PIPE=0 the
+    one-workgroup-per-item kernel, read at plan creation) and the matrix-core spreading against the
+    NUDFT, including transform counts that leave partial chunks (3, 5, 7, 21) and the 20-transform
+    SGD shape."""
     o = oracle()
     npl = pkg("nufft_plan")
-    monkeypatch.setenv("COV3D_SPREAD_MODE", spread_mode)
-    monkeypatch.setenv("COV3D_INTERP_MODE", interp_mode)
-    monkeypatch.setenv("COV3D_INTERP_DIRECT", direct)
     monkeypatch.setenv("COV3D_INTERP_PIPE", pipe)
     L = 12
     pts = _points(L, 2, 1, 300 + ntrans, dev, spread=1.2)
@@ -237,12 +230,12 @@ def test_polynomial_weights_match_nudft(dev, monkeypatch, eps):
     assert rel_l2(outs[0], outs[1]) < max(1e-6, eps / 10)
 
 
-@pytest.mark.parametrize("ntrans,direct", [(20, "1"), (5, "0")])
-def test_pipelined_interp_is_bitwise_equal(dev, monkeypatch, ntrans, direct):
+@pytest.mark.parametrize("ntrans", [20, 5])
+def test_pipelined_interp_is_bitwise_equal(dev, monkeypatch, ntrans):
     """The persistent pipelined interpolation runs the same arithmetic in the same order as the
     one-workgroup-per-subproblem kernel: central slices of 96 images at L=64 (thousands of
-    subproblems, some of them full) give bit-identical type-2 outputs, for the direct and the
-    sorted-plus-permute output paths and a partial transform pair."""
+    subproblems, some of them full) give bit-identical type-2 outputs, for the 20-transform shape
+    and a partial transform pair."""
     npl = pkg("nufft_plan")
     dsm = pkg("dataset")
     L, B = 64, 96
@@ -250,7 +243,6 @@ def test_pipelined_interp_is_bitwise_equal(dev, monkeypatch, ntrans, direct):
     rots = dsm.haar_rotations(B, g).to(dev)
     pts = dsm.pts_rot_from_rotations(rots, L)
     vol = torch.randn((ntrans, L, L, L), generator=g).to(dev)
-    monkeypatch.setenv("COV3D_INTERP_DIRECT", direct)
     outs = []
     for pipe in ("1", "0"):
         monkeypatch.setenv("COV3D_INTERP_PIPE", pipe)
