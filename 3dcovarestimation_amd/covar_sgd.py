@@ -157,10 +157,26 @@ def raw_cost_maximum_liklihood_fourier_domain(projected_eigenvecs, images, noise
 
 
 def cost_maximum_liklihood_fourier_domain(vols, images, nufft_plans, filters, noise_var, reg_scale=0,
-                                          fourier_reg=None, apply_mean_const_term=False, mean_aggregate=True):
-    """ML objective, Fourier domain (covar_sgd.py:1350-1378)."""
-    P = vol_forward(vols, nufft_plans, filters, fourier_domain=True)
-    val = raw_cost_maximum_liklihood_fourier_domain(P, images, noise_var, apply_mean_const_term, mean_aggregate)
+                                          fourier_reg=None, apply_mean_const_term=False, mean_aggregate=True,
+                                          form=None, precision=ml_ops.PRECISION_BF16X3):
+    """ML objective, Fourier domain (covar_sgd.py:1350-1378).
+
+    ``form="fused"`` (default) on a NufftPlanDiscretized runs ``ml_ops.MLCostDiscretized``: native
+    slice gather, split-BF16 MFMA Gram with FP64 Cholesky, hand-written adjoint and native slice
+    scatter; ``form="autograd"`` is the reference's torch formulation differentiated by autograd."""
+    form = form or _DEFAULT_FORM
+    fused_ok = (
+        form == "fused" and isinstance(nufft_plans, NufftPlanDiscretized) and mean_aggregate
+        and not isinstance(vols, tuple) and vols.dtype == torch.complex64 and vols.ndim == 4
+        and vols.shape[0] <= 31 and images.is_complex()
+    )
+    if fused_ok:
+        val = ml_ops.MLCostDiscretized.apply(vols, images.to(torch.complex64), nufft_plans, filters,
+                                             float(noise_var), bool(apply_mean_const_term), int(precision))
+    else:
+        P = vol_forward(vols, nufft_plans, filters, fourier_domain=True)
+        val = raw_cost_maximum_liklihood_fourier_domain(P, images, noise_var, apply_mean_const_term,
+                                                        mean_aggregate)
     if fourier_reg is not None and reg_scale != 0:
         val = val + _ml_regulariser(vols, fourier_reg, noise_var, reg_scale, in_fourier=True)
     return val
@@ -218,6 +234,8 @@ class CovarTrainer:
         kw = {}
         if self.cost_func is cost_maximum_liklihood:
             kw = {"form": self.form, "precision": self.precision, "half_plane": self.half_plane}
+        elif self.cost_func is cost_maximum_liklihood_fourier_domain:
+            kw = {"form": self.form, "precision": self.precision}
         cost_val = self.cost_func(self._covar(None), images, self.nufft_plans, filters, self.noise_var,
                                   self.reg_scale, self.fourier_reg, **kw)
         cost_val.backward()

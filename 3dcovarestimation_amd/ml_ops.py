@@ -221,7 +221,119 @@ class MLCostNufft(torch.autograd.Function):
                            pix_map=ctx.pix)
             grad_v = plan.execute_adjoint_presorted(r, real_output=True)
         else:
-            g = ml_grad_images(c, yhat, z, minv, L, noise_var, 1.0 / B, ctf=ctf_arg, out=c, pix_map=ctx.pix)
+            g = ml_grad_images(c, yhat, z, minv, L, noise_var, 1.0 / B, ctf=ctf_arg, out=torch.empty_like(c),
+                               pix_map=ctx.pix)
             grad_v = plan.execute_adjoint(g.reshape(r, -1), real_output=True)
         grad_v = grad_v[:r] * grad_out
         return grad_v, None, None, None, None, None, None, None
+
+
+def _boundary_pixels(plan, B: int, L: int) -> torch.Tensor:
+    """(B, L*L) mask of the pixels whose discretised slice is not the mirror image of its -k partner.
+
+    The slice of pixel k reads the centred volume at index x = p vL / (2 pi) + vL / 2 per axis,
+    and its -k partner at vL - x; cell j mirrors to cell vL - j, so everything is mirrored except
+    the Nyquist cell 0, whose mirror vL lies outside the volume.  A pixel is flagged when its
+    stencil can reach cell 0 or cell vL on any axis (x within one cell of 0 or of vL; a superset of
+    both interpolation modes), and the flag is made symmetric by OR-ing it with its partner's, so
+    the set is closed under k -> -k.
+    """
+    vL = plan.sz[0] * plan.upsample_factor
+    x = (plan._grid_flat.to(torch.float32) + 1) * (0.5 * (vL - 1))  # (B*L*L, 3) cell coordinates
+    near = ((x > -1.0) & (x < 1.0)) | ((x > vL - 1.0) & (x < vL + 1.0))
+    f = near.any(dim=1).reshape(B, L, L)
+    if L % 2 == 0:
+        m = torch.roll(torch.flip(f, dims=(1, 2)), shifts=(1, 1), dims=(1, 2))  # (i, j) -> (L - i, L - j) mod L
+    else:
+        m = torch.flip(f, dims=(1, 2))  # (i, j) -> (L - 1 - i, L - 1 - j)
+    return (f | m).reshape(B, L * L)
+
+
+class MLCostDiscretized(torch.autograd.Function):
+    """Batch-mean Fourier-domain ML cost on the discretised projection (cov3d/covar_sgd.py:1350-1399
+    with nufft_disc "bilinear" / "nearest", the reference's default trainer path).
+
+    forward(vhat (r, vL, vL, vL) complex64, yhat (B, L, L) complex64 Fourier-domain images, plan
+    (NufftPlanDiscretized with its points set), ctf (B, L, L) or None, noise_var (the Fourier-domain
+    noise variance, L^2 times the spatial one), mean_const, precision) -> scalar cost.
+
+    With P = mask * CTF * S(vhat) / L (the reference's Fourier-domain projection) the objective is
+    0.5 mean_i(-Re(b_i^H z_i) + logdet M_i), M = I + conj(P) P^T / noise_var, b = conj(P) y / noise_var,
+    z = M^-1 b (complex Hermitian M).  The slices come from the native gather (cov3d_slice_forward);
+    the fused kernel (split-BF16 MFMA Gram, raw values c = S(vhat), images yhat, spatial noise
+    variance noise_var / L^2) gives Re M and Re b over all L^2 pixels.  Their imaginary parts cancel
+    between every pixel and its -k partner except where the slice touches the padded volume's
+    Nyquist cell (``_boundary_pixels``), so they are summed over those pixels only.  The r x r
+    complex solve and log-determinant run in FP64.  Backward: the gradient with respect to the raw
+    values, conj(M^-1) Q + conj(z) (z^T Q - y~) in the kernel's units, splits into two passes of the
+    fused gradient kernel with real coefficients (conj(M^-1) = R - iJ, z = a + ib):
+        real part:      (R + b b^T) Q + a (a^T Q - y~)
+        imaginary part: (-J + a b^T - b a^T - b b^T) Q + (-b) ((-b)^T Q - y~)
+    and the native exact adjoint of the slice (cov3d_slice_adjoint) scatters it, replacing autograd
+    through grid_sample, solve and logdet.
+    """
+
+    @staticmethod
+    def forward(ctx, vhat, yhat, plan, ctf, noise_var, mean_const, precision):
+        r = vhat.shape[0]
+        L = yhat.shape[-1]
+        B = yhat.shape[0]
+        dev = vhat.device
+        c = plan.execute_forward(vhat.detach()).reshape(r, B, L * L)
+        nv_s = float(noise_var) / (L * L)
+        res = ml_forward(c, yhat, L, nv_s, ctf=ctf, mean_const=False, precision=precision, want_m=True,
+                         want_minv=False, want_b=True)
+        M = res["m"].double()
+        bvec = res["b"].double()
+        # imaginary parts from the pixels whose -k partner is not their mirror image
+        bd = _boundary_pixels(plan, B, L)
+        nbd = int(bd.sum(dim=1).max())
+        Mi = torch.zeros_like(M)
+        bi = torch.zeros_like(bvec)
+        if nbd > 0:
+            order = torch.sort(bd.to(torch.int8), dim=1, descending=True, stable=True).indices[:, :nbd]
+            valid = torch.gather(bd, 1, order)  # (B, nbd)
+            from .projection_funcs import nyquist_mask
+
+            scale = nyquist_mask(L, dev).reshape(1, -1).expand(B, -1)
+            if ctf is not None:
+                scale = scale * ctf.reshape(B, -1).to(torch.float32)
+            w = torch.gather(scale, 1, order) * valid / float(L * L)  # Q = mask ctf c / L^2
+            q = torch.gather(c, 2, order.unsqueeze(0).expand(r, -1, -1)) * w.unsqueeze(0)  # (r, B, nbd)
+            q = q.permute(1, 0, 2).to(torch.complex128)  # (B, r, nbd)
+            yt = torch.gather(yhat.reshape(B, -1), 1, order).to(torch.complex128) * (valid / float(L))
+            Mi = (q.conj() @ q.transpose(1, 2)).imag / nv_s
+            bi = (q.conj() @ yt.unsqueeze(-1)).squeeze(-1).imag / nv_s
+        Mc = torch.complex(M, Mi)
+        bc = torch.complex(bvec, bi)
+        Minv = torch.linalg.inv(Mc)
+        z = (Minv @ bc.unsqueeze(-1)).squeeze(-1)
+        logdet = torch.linalg.slogdet(Mc)[1]
+        cost_i = 0.5 * (-(bc.conj() * z).sum(dim=1).real + logdet)
+        cost = cost_i.mean().to(torch.float32)
+        if mean_const:
+            cost = cost + 0.5 * (yhat.abs() ** 2).sum(dim=(1, 2)).mean() / float(noise_var)
+        ctx.plan = plan
+        ctx.meta = (L, B, nv_s, tuple(vhat.shape))
+        ctx.save_for_backward(c, yhat, z, Minv, ctf if ctf is not None else torch.empty(0))
+        ctx.has_ctf = ctf is not None
+        return cost
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        c, yhat, z, Minv, ctf = ctx.saved_tensors
+        L, B, nv_s, shape = ctx.meta
+        r = c.shape[0]
+        ctf_arg = ctf if ctx.has_ctf else None
+        Rm, Jm = Minv.real, Minv.imag
+        a, bz = z.real, z.imag
+        A1 = (Rm + bz.unsqueeze(2) * bz.unsqueeze(1)).to(torch.float32)
+        A2 = (-Jm + a.unsqueeze(2) * bz.unsqueeze(1) - bz.unsqueeze(2) * a.unsqueeze(1)
+              - bz.unsqueeze(2) * bz.unsqueeze(1)).to(torch.float32)
+        g1 = ml_grad_images(c, yhat, a.to(torch.float32), A1, L, nv_s, 1.0 / B, ctf=ctf_arg,
+                            out=torch.empty_like(c))
+        g2 = ml_grad_images(c, yhat, (-bz).to(torch.float32), A2, L, nv_s, 1.0 / B, ctf=ctf_arg,
+                            out=torch.empty_like(c))
+        g = g1 + 1j * g2
+        grad_v = ctx.plan._adjoint(g.reshape(r, -1), aggregated=True).reshape(shape) * grad_out
+        return grad_v, None, None, None, None, None, None
