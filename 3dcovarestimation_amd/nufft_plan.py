@@ -143,7 +143,8 @@ class NufftPlanDiscretized(BaseNufftPlan):
             if sig.numel() % npts != 0:
                 raise ValueError("signal size does not match the plan's points")
             ntrans = sig.numel() // npts
-            out = torch.zeros((ntrans, vL, vL, vL), dtype=sig.dtype, device=sig.device)
+            # the gather adjoint writes every cell
+            out = torch.empty((ntrans, vL, vL, vL), dtype=sig.dtype, device=sig.device)
             ppv = npts
         else:
             nimg = sig.shape[0]

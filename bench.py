@@ -55,6 +55,8 @@ def parse():
     ap.add_argument("--cpu-images", type=int, default=1024)
     ap.add_argument("--cpu-steps", type=int, default=3, help="timed CPU steps (the median is reported)")
     ap.add_argument("--no-nudft", action="store_true", help="skip the FP64 NUDFT check of the NUFFT outputs")
+    ap.add_argument("--bilinear-steps", type=int, default=6,
+                    help="timed steps of each form on the discretised bilinear path (0 skips that leg)")
     ap.add_argument("--no-cg", action="store_true", help="skip the mean-CG leg")
     ap.add_argument("--subspace-batches", type=int, default=98,
                     help="SGD steps of the FP64-trajectory subspace comparison (98 = one epoch of the "
@@ -67,7 +69,7 @@ def parse():
     return ap.parse_args()
 
 
-def sgd_run(pkg, data, L, r, B, steps, warmup, form, timing=False, precision=None):
+def sgd_run(pkg, data, L, r, B, steps, warmup, form, timing=False, precision=None, nufft_disc="nufft"):
     import torch
 
     covar_mod = importlib.import_module(pkg + ".covar")
@@ -80,7 +82,7 @@ def sgd_run(pkg, data, L, r, B, steps, warmup, form, timing=False, precision=Non
     loader = dsm.create_dataloader(data.dataset, batch_size=B)
     trainer = sgd.CovarTrainer(covar, loader, dev)
     kw = {} if precision is None else {"precision": precision}
-    trainer.setup_training(nufft_disc="nufft", objective_func="ml", form=form, **kw)
+    trainer.setup_training(nufft_disc=nufft_disc, objective_func="ml", form=form, **kw)
     batches = iter(loader)
     for _ in range(warmup):
         images, pts, filt = trainer.prepare_batch(next(batches))
@@ -100,6 +102,7 @@ def sgd_run(pkg, data, L, r, B, steps, warmup, form, timing=False, precision=Non
             tot, cnt = native.timing_read(k)
             kern[k] = {"total_ms": tot, "launches": cnt, "avg_ms": tot / cnt if cnt else None}
         native.timing_enable(False)
+    trainer.complete_training()  # the discretised path moved the images to the Fourier domain
     del trainer
     return dt, kern
 
@@ -345,7 +348,7 @@ def main():
     dev = torch.device("cuda:0")
     L, r, B = args.L, args.rank, args.batch
     plain_steps = args.plain_steps or max(2, args.steps // 2)
-    n_img = max((args.warmup + max(args.steps, plain_steps)) * B, args.subspace_batches * B)
+    n_img = max((args.warmup + max(args.steps, plain_steps, args.bilinear_steps)) * B, args.subspace_batches * B)
     cfg = dsm.config_by_name("L128") if L == 128 else dsm.SyntheticConfig(L, n_img, r, [1.0] * r, 1.5, 0.02,
                                                                              (0.5, 3.0))
     n_img = min(n_img, cfg.n_images)
@@ -433,6 +436,18 @@ def main():
                       "achieved_GBs": round(bytes_per_launch / (kern[k]["avg_ms"] * 1e-3) / 1e9, 1),
                       "achieved_TFLOPs": round(flops_per_launch / (kern[k]["avg_ms"] * 1e-3) / 1e12, 2)}
                   for k in ("interp", "spread") if kern[k]["avg_ms"]}
+    bil = None
+    if args.bilinear_steps > 0:
+        log("discretised bilinear path (the reference's default trainer): fused vs autograd Fourier-domain ML")
+        bil = {}
+        for tag, form in (("fused", "fused"), ("autograd", "autograd")):
+            dtb, _ = sgd_run(pkg, data, L, r, B, args.bilinear_steps, 1, form, nufft_disc="bilinear")
+            bil[tag] = {"value": round(args.bilinear_steps * B / dtb, 1),
+                        "ms_per_step": round(1e3 * dtb / args.bilinear_steps, 2)}
+        bil["speedup_fused_vs_autograd"] = round(bil["fused"]["value"] / bil["autograd"]["value"], 3)
+        bil["workload"] = (f"covariance SGD step, bilinear slice path (us=2), Fourier-domain ML, L={L}, r={r}, "
+                           f"batch={B}; fused = native slice gather/scatter + MFMA Gram + hand adjoint")
+        log(f"bilinear path: {bil}")
     nud = mlf = None
     if not args.no_nudft:
         log("NUFFT outputs vs the FP64 NUDFT on a 512-image subset")
@@ -485,6 +500,7 @@ def main():
         "roofline": roof,
         "nufft_kernels": per_kernel,
         "contractions": contr,
+        "bilinear_path": bil,
         "nudft_rel_err": nud,
         "ml_vs_fp64": mlf,
         "subspace_error": sub,

@@ -108,10 +108,14 @@ int cov3d_nufft_info(void* plan, int64_t* host_nf3, int* host_w,
  *   normalised xyz convention (already scaled by NufftPlanDiscretized.setpts,
  *   nufft_plan.py:94-118), out: (ntrans, npts) complex.
  *   mode: 0 = trilinear ("bilinear"), 1 = nearest.
- * Adjoint: out (ntrans, D, H, W) is accumulated into (caller zeroes it);
- *   aggregated = 1 sums all points into one volume per trans (execute_adjoint);
- *   aggregated = 0 writes volume k from the points [k*pts_per_vol,
- *   (k+1)*pts_per_vol) (execute_adjoint_unaggregated), ntrans must be 1.
+ * Adjoint: aggregated = 1 sums all points into one volume per trans
+ *   (execute_adjoint) and OVERWRITES vol_out (ntrans, D, H, W) with the sums:
+ *   the points are sorted by their floor cell and every output cell gathers
+ *   its contributions in sorted order (deterministic, no atomics; the sort
+ *   workspace is allocated on first use per device and kept);
+ *   aggregated = 0 ACCUMULATES into vol_out (caller zeroes it) volume k from
+ *   the points [k*pts_per_vol, (k+1)*pts_per_vol)
+ *   (execute_adjoint_unaggregated), ntrans must be 1.
  * ------------------------------------------------------------------------ */
 int cov3d_slice_forward(const void* vol, const void* grid, void* out,
                         int64_t ntrans, int64_t D, int64_t H, int64_t W,
