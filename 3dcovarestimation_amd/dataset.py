@@ -97,14 +97,19 @@ class CovarDataset:
         self.ctf_params = ctf_params
         self.noise_var = float(noise_var)
         self.resolution = images.shape[-1]
-        self.offsets = offsets if offsets is not None else torch.zeros((len(images), 2), device=images.device)
-        self.contrasts = torch.ones(len(images), device=images.device)
+        self.offsets = offsets if offsets is not None else torch.zeros((len(images), 2), device=rotations.device)
+        self.contrasts = torch.ones(len(images), device=rotations.device)
         self.mask = mask
         self._in_spatial_domain = not images.is_complex()
         self.dtype = images.real.dtype if images.is_complex() else images.dtype
 
     def __len__(self) -> int:
         return self.images.shape[0]
+
+    @property
+    def device(self) -> torch.device:
+        """The device the batches are computed on (where the rotations live)."""
+        return self.rotations.device
 
     def get_filters(self, idx) -> torch.Tensor:
         if self.filters is not None:
@@ -153,7 +158,7 @@ class CovarDataset:
         from .utils import soft_edged_kernel
 
         L = self.resolution
-        dev = self.images.device
+        dev = self.device
         if mean_volume.ndim == 3:
             mean_volume = mean_volume.unsqueeze(0)
         if mask is not None and mask.ndim == 3:
@@ -171,9 +176,12 @@ class CovarDataset:
                 sl = slice(i, min(i + batch_size, len(self)))
                 images, pts, filters, _ = self[sl]
                 phase = offset_to_phase_shift(-self.offsets[sl].to(self.dtype), phase_shift_grid=grid)
-                self.images[sl] = preprocess_image_batch(images, plan, filters, (pts, phase), mean_volume, mask,
-                                                         thr, soft, fourier_domain=not self._in_spatial_domain)
+                self._store(sl, preprocess_image_batch(images, plan, filters, (pts, phase), mean_volume, mask,
+                                                       thr, soft, fourier_domain=not self._in_spatial_domain))
         self.offsets.zero_()
+
+    def _store(self, idx, images: torch.Tensor) -> None:
+        self.images[idx] = images
 
     def get_total_gain(self, batch_size: int = 4096) -> torch.Tensor:
         """diag(sum_i P_i^T P_i) on the (L,)^3 grid (dataset.py:322-356): the nearest-neighbour
@@ -184,7 +192,7 @@ class CovarDataset:
 
         L = self.resolution
         plan = NufftPlanDiscretized((L,) * 3, upsample_factor=1, mode="nearest")
-        gain = torch.zeros((L,) * 3, device=self.images.device, dtype=self.dtype)
+        gain = torch.zeros((L,) * 3, device=self.device, dtype=self.dtype)
         for i in range(0, len(self), batch_size):
             _, pts, filters, _ = self[slice(i, min(i + batch_size, len(self)))]
             plan.setpts(pts)
@@ -208,7 +216,7 @@ class CovarDataset:
         from .nufft_plan import NufftPlanDiscretized
 
         L = self.resolution
-        dev = self.images.device
+        dev = self.device
         shell_of, size, n = shell_index(L, 3, dev)
         plan = NufftPlanDiscretized((L,) * 3, upsample_factor=1, mode="nearest")
         acc = torch.zeros((n, n), device=dev, dtype=torch.float64)
@@ -246,7 +254,7 @@ class CovarDataset:
     def half_split(self) -> Tuple["CovarDataset", "CovarDataset"]:
         """Even/odd split into two half-sets."""
         n = len(self)
-        dev = self.images.device
+        dev = self.device
         return self._subset(torch.arange(0, n, 2, device=dev)), self._subset(torch.arange(1, n, 2, device=dev))
 
     def filter_gain(self, batch_size: int = 4096) -> torch.Tensor:
@@ -257,6 +265,138 @@ class CovarDataset:
             s = (f.float() ** 2).sum(dim=0)
             acc = s if acc is None else acc + s
         return acc / len(self)
+
+
+class StreamingCovarDataset(CovarDataset):
+    """A particle stack whose images live in pinned host memory and are streamed to the device.
+
+    Rotations, CTF parameters (or filters), offsets and contrasts stay on the device, so pts_rot
+    and the CTF are made there per batch as for ``CovarDataset``; only the images (N, L, L) float32
+    travel, 4 L^2 bytes per image (256 KiB at L = 256).  ``create_dataloader`` streams contiguous
+    batches through two device buffers on a copy stream, the next batch's copy overlapping the
+    current batch's compute (covar_sgd.py:240-244 and dataset.py:850-895 move each batch with a
+    pinned DataLoader).  ``to_fourier_domain`` keeps the host images real and applies the centred
+    FFT to each batch on the device instead (dataset.py:426-433), scaling noise_var by L^2 as there.
+    """
+
+    def __init__(self, host_images: torch.Tensor, rotations: torch.Tensor, noise_var: float,
+                 filters: Optional[torch.Tensor] = None, ctf_params: Optional[torch.Tensor] = None,
+                 offsets: Optional[torch.Tensor] = None, mask: Optional[torch.Tensor] = None) -> None:
+        if host_images.is_cuda or host_images.is_complex():
+            raise ValueError("host_images must be real images in host memory")
+        if not host_images.is_pinned():
+            host_images = host_images.pin_memory()  # a new page-locked copy
+        super().__init__(host_images, rotations, noise_var, filters=filters, ctf_params=ctf_params,
+                         offsets=offsets, mask=mask)
+        self._fourier = False
+        self._in_spatial_domain = True
+        self._copy_stream = None
+
+    @property
+    def copy_stream(self):
+        if self._copy_stream is None:
+            self._copy_stream = torch.cuda.Stream(self.device)  # default priority
+        return self._copy_stream
+
+    def _finish(self, images: torch.Tensor) -> torch.Tensor:
+        return centered_fft2(images) if self._fourier else images
+
+    def _host_index(self, idx):
+        return idx.cpu() if isinstance(idx, torch.Tensor) else idx
+
+    def __getitem__(self, idx):
+        rots = self.rotations[idx]
+        single = rots.ndim == 2
+        images = self._finish(self.images[self._host_index(idx)].to(self.device, non_blocking=True))
+        pts = pts_rot_from_rotations(rots.reshape(-1, 3, 3), self.resolution)
+        if single:
+            pts = pts[0]
+        return images, pts, self.get_filters(idx), idx
+
+    def _store(self, idx, images: torch.Tensor) -> None:
+        if images.is_complex():
+            images = centered_ifft2(images).real
+        self.images[self._host_index(idx)] = images.to("cpu")
+
+    def to_fourier_domain(self) -> None:
+        if not self._fourier:
+            self._fourier = True
+            self._in_spatial_domain = False
+            self.noise_var *= self.resolution**2
+
+    def to_spatial_domain(self) -> None:
+        if self._fourier:
+            self._fourier = False
+            self._in_spatial_domain = True
+            self.noise_var /= self.resolution**2
+
+    def _subset(self, idx: torch.Tensor) -> "StreamingCovarDataset":
+        sub = StreamingCovarDataset(
+            self.images[self._host_index(idx)], self.rotations[idx], self.noise_var,
+            filters=None if self.filters is None else self.filters[idx],
+            ctf_params=None if self.ctf_params is None else self.ctf_params[idx],
+            offsets=self.offsets[idx], mask=self.mask,
+        )
+        if self._fourier:
+            sub._fourier, sub._in_spatial_domain = True, False
+        return sub
+
+
+class StreamingBatchIterator:
+    """Contiguous batches of a StreamingCovarDataset through two device image buffers.
+
+    The copy of batch i+1 is queued on the dataset's copy stream as soon as batch i is handed out;
+    it waits for the compute stream to finish with that buffer (the step of batch i-1) and runs
+    while the step of batch i computes.  The compute stream waits for a batch's copy before using
+    it.  A batch's images must not be kept past the next batch (its buffer is then refilled).
+    """
+
+    def __init__(self, dataset: StreamingCovarDataset, batch_size: int) -> None:
+        self.dataset = dataset
+        self.batch_size = int(batch_size)
+
+    def __len__(self) -> int:
+        return math.ceil(len(self.dataset) / self.batch_size)
+
+    def __iter__(self):
+        ds, B = self.dataset, self.batch_size
+        n = len(ds)
+        dev = ds.device
+        L = ds.resolution
+        comp = torch.cuda.current_stream(dev)
+        copy = ds.copy_stream
+        bufs = [torch.empty((B, L, L), dtype=ds.images.dtype, device=dev) for _ in range(2)]
+        for buf in bufs:
+            buf.record_stream(copy)
+        done = [None, None]
+        ready = [None, None]
+
+        def issue(i: int, k: int) -> None:
+            a, b = i * B, min((i + 1) * B, n)
+            if done[k] is not None:
+                copy.wait_event(done[k])
+            with torch.cuda.stream(copy):
+                bufs[k][: b - a].copy_(ds.images[a:b], non_blocking=True)
+                ev = torch.cuda.Event()
+                ev.record(copy)
+            ready[k] = ev
+
+        nb = len(self)
+        if nb:
+            issue(0, 0)
+        for i in range(nb):
+            k = i % 2
+            if i + 1 < nb:
+                issue(i + 1, 1 - k)
+            comp.wait_event(ready[k])
+            a, b = i * B, min((i + 1) * B, n)
+            sl = slice(a, b)
+            images = ds._finish(bufs[k][: b - a])
+            pts = pts_rot_from_rotations(ds.rotations[sl], L)
+            yield images, pts, ds.get_filters(sl), sl
+            ev = torch.cuda.Event()
+            ev.record(comp)  # after everything the consumer queued for this batch
+            done[k] = ev
 
 
 class BatchIterator:
@@ -280,8 +420,11 @@ class BatchIterator:
                 yield self.dataset[self.idx[i : i + self.batch_size]]
 
 
-def create_dataloader(dataset, batch_size: int, idx: Optional[torch.Tensor] = None, **unused) -> BatchIterator:
-    """Batch iterator over a device-resident dataset (dataset.py:850-895)."""
+def create_dataloader(dataset, batch_size: int, idx: Optional[torch.Tensor] = None, **unused):
+    """Batch iterator over a dataset (dataset.py:850-895): contiguous unshuffled batches; a
+    host-streamed dataset without an index subset streams through double device buffers."""
+    if isinstance(dataset, StreamingCovarDataset) and idx is None:
+        return StreamingBatchIterator(dataset, batch_size)
     return BatchIterator(dataset, batch_size, idx)
 
 
@@ -349,7 +492,7 @@ class SyntheticData:
 
 
 def make_synthetic(cfg: SyntheticConfig, device, n_images: Optional[int] = None, subtract_mean: bool = True,
-                   batch: int = 1024, store_filters: bool = True) -> SyntheticData:
+                   batch: int = 1024, store_filters: bool = True, host_images: bool = False) -> SyntheticData:
     """Generates images y_i = T_{o_i}[CTF_i * A_i(mean + sum_k z_ik u_k)] + noise on the device.
 
     Projections use the gfx950 type-2 NUFFT (vol_forward on a NufftPlan).  With
@@ -360,6 +503,8 @@ def make_synthetic(cfg: SyntheticConfig, device, n_images: Optional[int] = None,
     With ``subtract_mean`` the projected mean is left out of the images, as the reference's
     preprocessing removes it before covariance estimation; it is left out before the shift, so
     such a dataset only needs the phase step of ``CovarDataset.preprocess`` (a zero mean volume).
+    With ``host_images`` the images are kept in pinned host memory (a ``StreamingCovarDataset``;
+    the device holds one batch at a time while generating) and the CTFs are made per batch.
     """
     from .nufft_plan import NufftPlan
     from .poses import get_phase_shift_grid, offset_to_phase_shift
@@ -393,8 +538,14 @@ def make_synthetic(cfg: SyntheticConfig, device, n_images: Optional[int] = None,
         shift_grid = get_phase_shift_grid(L, dtype=torch.float32, device=dev)
     vols = torch.cat([mean.unsqueeze(0), eigvecs], dim=0).to(dev)
     plan = NufftPlan((L,) * 3, batch_size=r + 1, device=dev)
-    images = torch.empty((n, L, L), dtype=torch.float32, device=dev)
+    if host_images:
+        images = torch.empty((n, L, L), dtype=torch.float32, pin_memory=True)
+        store_filters = False
+    else:
+        images = torch.empty((n, L, L), dtype=torch.float32, device=dev)
     filters = torch.empty((n, L, L), dtype=torch.float32, device=dev) if store_filters else None
+    s1 = torch.zeros((), dtype=torch.float64, device=dev)
+    s2 = torch.zeros((), dtype=torch.float64, device=dev)
     for i in range(0, n, batch):
         j = min(i + batch, n)
         plan.setpts(pts_rot_from_rotations(rots[i:j], L))
@@ -409,15 +560,31 @@ def make_synthetic(cfg: SyntheticConfig, device, n_images: Optional[int] = None,
             img = img * amp[i:j, None, None]
         if offsets is not None:
             img = centered_ifft2(centered_fft2(img) * offset_to_phase_shift(offsets[i:j], phase_shift_grid=shift_grid)).real
-        images[i:j] = img
+        if host_images:
+            s1 += img.double().sum()
+            s2 += (img.double() ** 2).sum()
+            images[i:j].copy_(img)
+        else:
+            images[i:j] = img
         if filters is not None:
             filters[i:j] = f
-    signal_var = float(images.var())
+    if host_images:
+        cnt = n * L * L
+        signal_var = float((s2 - s1 * s1 / cnt) / max(cnt - 1, 1))
+    else:
+        signal_var = float(images.var())
     noise_var = signal_var / cfg.snr if signal_var > 0 else 1.0
     g = torch.Generator(device=dev).manual_seed(cfg.seed + 7)
     for i in range(0, n, 8 * batch):
         j = min(i + 8 * batch, n)
-        images[i:j] += math.sqrt(noise_var) * torch.randn((j - i, L, L), device=dev, generator=g)
-    ds = CovarDataset(images, rots, noise_var, filters=filters, ctf_params=ctf_params, offsets=offsets)
+        noise = math.sqrt(noise_var) * torch.randn((j - i, L, L), device=dev, generator=g)
+        if host_images:
+            images[i:j].copy_(images[i:j].to(dev) + noise)
+        else:
+            images[i:j] += noise
+    if host_images:
+        ds = StreamingCovarDataset(images, rots, noise_var, ctf_params=ctf_params, offsets=offsets)
+    else:
+        ds = CovarDataset(images, rots, noise_var, filters=filters, ctf_params=ctf_params, offsets=offsets)
     return SyntheticData(ds, mean.to(dev), eigvecs.to(dev), eigvals.to(dev), latents,
                          extra={} if amp is None else {"amplitudes": amp})

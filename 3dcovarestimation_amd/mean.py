@@ -176,7 +176,7 @@ def backproject_lhs_rhs(dataset, idx=None, upsampling_factor: int = 2, batch_siz
     plan = NufftPlanDiscretized((L,) * 3, upsample_factor=upsampling_factor, mode="nearest")
     was_spatial = dataset._in_spatial_domain
     dataset.to_fourier_domain()
-    dev = dataset.images.device
+    dev = dataset.device
     vL = L * upsampling_factor
     rhs = torch.zeros((vL,) * 3, dtype=torch.complex64, device=dev)
     lhs = torch.zeros((vL,) * 3, dtype=torch.complex64, device=dev)
@@ -247,7 +247,7 @@ def reconstruct_mean_from_halfsets(dataset, idx: Optional[torch.Tensor] = None, 
     """Two half reconstructions then FSC regularisation (mean.py:262-291)."""
     kw["return_lhs_rhs"] = True
     n = len(dataset)
-    dev = dataset.images.device
+    dev = dataset.device
     if idx is None:
         idx = torch.cat([torch.arange(0, n, 2, device=dev), torch.arange(1, n, 2, device=dev)])
     m1, rhs1, lhs1 = reconstruct_mean(dataset, idx=idx[: len(idx) // 2], **kw)

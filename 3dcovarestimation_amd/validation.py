@@ -58,7 +58,7 @@ def sgd_subspaces(dataset: CovarDataset, rank: int, v0: torch.Tensor, batch_size
                   n_images: Optional[int] = None, fp64_eps: float = 1e-12
                   ) -> Tuple[Dict[str, torch.Tensor], Dict[str, float]]:
     """Bases of the subspaces learned by each form, and each form's training seconds."""
-    dev = dataset.images.device
+    dev = dataset.device
     L = dataset.resolution
     idx = torch.arange(n_images if n_images is not None else len(dataset), device=dev)
     d64 = None
