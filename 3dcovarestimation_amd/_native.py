@@ -35,6 +35,7 @@ _SIGNATURES = {
     "cov3d_nufft_info": [c_vp, ctypes.POINTER(c_i64), ctypes.POINTER(c_int), ctypes.POINTER(c_dbl)],
     "cov3d_slice_forward": [c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_i64, c_i64, c_int, c_int, c_vp],
     "cov3d_slice_adjoint": [c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_i64, c_i64, c_int, c_int, c_i64, c_int, c_vp],
+    "cov3d_slice_point_grad": [c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_i64, c_i64, c_int, c_int, c_vp],
     "cov3d_ml_forward": [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_i64, c_dbl, c_int, c_int, c_vp, c_vp,
                          c_vp, c_vp, c_vp, c_vp],
     "cov3d_ml_grad_images": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_i64, c_i64, c_i64, c_dbl, c_dbl,
@@ -95,6 +96,13 @@ def require_device(t: torch.Tensor, name: str = "tensor") -> None:
             f"{name} must live on a HIP device: this package runs only on the gfx950 kernels "
             "(no CPU fallback)"
         )
+
+
+def dense(t: torch.Tensor) -> torch.Tensor:
+    """A contiguous tensor whose memory holds its values, for a raw pointer.  Torch's lazy
+    conjugate and negative views (gradients of ``conj()`` arrive as such) keep the unconjugated
+    values in memory; they are materialised here."""
+    return t.resolve_conj().resolve_neg().contiguous()
 
 
 def stream_ptr(device: torch.device | None = None) -> int:

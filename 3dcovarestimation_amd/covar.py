@@ -190,3 +190,56 @@ class Covar(VolumeBase):
             v = self.get_vectors_spatial_domain().reshape(self.rank, -1)
             _, s, vh = torch.linalg.svd(v, full_matrices=False)
             self.set_vectors((s.reshape(-1, 1) * vh).reshape((self.rank,) + (self.resolution,) * 3))
+
+
+class Mean(VolumeBase):
+    """Mean volume as a normalised volume and its log amplitude (covar.py:115-235)."""
+
+    def __init__(self, volume_init: torch.Tensor, resolution: int, dtype: torch.dtype = torch.float32,
+                 fourier_domain: bool = False, volume_mask: Optional[torch.Tensor] = None,
+                 upsampling_factor: int = 2) -> None:
+        super().__init__(resolution, dtype, fourier_domain, upsampling_factor)
+        v, l = self._split(volume_init.reshape(1, resolution, resolution, resolution).to(dtype))
+        self.volume = torch.nn.Parameter(v)
+        self.log_volume_amplitude = torch.nn.Parameter(l)
+        self.volume_mask = volume_mask
+
+    @staticmethod
+    def _split(volume: torch.Tensor) -> Tuple[torch.Tensor, torch.Tensor]:
+        a = volume.reshape(volume.shape[0], -1).norm(dim=1)
+        return volume / a.reshape(-1, 1, 1, 1), torch.log(a)
+
+    def set_mean(self, volume: torch.Tensor) -> None:
+        v, l = self._split(volume.reshape(1, *volume.shape[-3:]).to(self.volume.dtype))
+        self.volume.data.copy_(v)
+        self.log_volume_amplitude.data.copy_(l)
+
+    @property
+    def device(self) -> torch.device:
+        return self.volume.device
+
+    def to(self, *args, **kwargs):
+        super().to(*args, **kwargs)
+        if self.volume_mask is not None:
+            self.volume_mask = self.volume_mask.to(*args, **kwargs)
+        return self
+
+    def get_volume_spatial_domain(self) -> torch.Tensor:
+        return self.volume * torch.exp(self.log_volume_amplitude).reshape(-1, 1, 1, 1)
+
+    def get_volume_fourier_domain(self) -> torch.Tensor:
+        v = self.get_volume_spatial_domain()
+        if self.grid_correction is not None:
+            v = v / self.grid_correction
+        return centered_fft3(v, padding_size=(self.resolution * self.upsampling_factor,) * 3)
+
+    def forward(self, dummy_var: Optional[Any] = None) -> torch.Tensor:
+        return self.get_volume_spatial_domain() if self._in_spatial_domain else self.get_volume_fourier_domain()
+
+    def get_volume_mask(self) -> Optional[torch.Tensor]:
+        if self.volume_mask is None:
+            return None
+        if self._in_spatial_domain:
+            return self.volume_mask
+        m = self.volume_mask / self.grid_correction if self.grid_correction is not None else self.volume_mask
+        return centered_fft3(m, padding_size=(self.resolution * self.upsampling_factor,) * 3)

@@ -25,6 +25,8 @@ from __future__ import annotations
 import copy
 import logging
 import time
+
+import numpy as np
 from typing import Any, Dict, List, Optional, Tuple, Union
 
 import torch
@@ -94,6 +96,14 @@ def _cost_ml_autograd(vols, images, nufft_plans, filters, noise_var, apply_mean_
     return 0.5 * torch.mean(exp_term + noise_term) if mean_aggregate else 0.5 * (exp_term + noise_term)
 
 
+def _only_vols_need_grad(images, filters, plan) -> bool:
+    """The fused objectives return gradients of the volumes only: images, filters (contrasts) or
+    points (rotations) that need gradients take the autograd form."""
+    pts = getattr(plan, "points", None) if isinstance(plan, NufftPlan) else getattr(plan, "_grid_flat", None)
+    return not (images.requires_grad or (filters is not None and filters.requires_grad)
+                or (pts is not None and pts.requires_grad))
+
+
 def cost_maximum_liklihood(vols, images, nufft_plans, filters, noise_var, reg_scale=0, fourier_reg=None,
                            apply_mean_const_term=False, mean_aggregate=True, form=None,
                            precision=ml_ops.PRECISION_BF16X3, half_plane=True):
@@ -102,6 +112,7 @@ def cost_maximum_liklihood(vols, images, nufft_plans, filters, noise_var, reg_sc
     fused_ok = (
         form == "fused" and isinstance(nufft_plans, NufftPlan) and mean_aggregate
         and vols.dtype == torch.float32 and vols.shape[0] <= 31 and vols.ndim == 4
+        and _only_vols_need_grad(images, filters, nufft_plans)
     )
     if fused_ok:
         L = vols.shape[-1]
@@ -168,7 +179,7 @@ def cost_maximum_liklihood_fourier_domain(vols, images, nufft_plans, filters, no
     fused_ok = (
         form == "fused" and isinstance(nufft_plans, NufftPlanDiscretized) and mean_aggregate
         and not isinstance(vols, tuple) and vols.dtype == torch.complex64 and vols.ndim == 4
-        and vols.shape[0] <= 31 and images.is_complex()
+        and vols.shape[0] <= 31 and images.is_complex() and _only_vols_need_grad(images, filters, nufft_plans)
     )
     if fused_ok:
         val = ml_ops.MLCostDiscretized.apply(vols, images.to(torch.complex64), nufft_plans, filters,
@@ -388,6 +399,197 @@ class CovarTrainer:
         torch.save(self.results_dict(), self.save_path)
 
 
+def _project_mean_out(eigenvecs: torch.Tensor, mean: torch.Tensor) -> torch.Tensor:
+    """Removes the mean direction from the eigenvolumes and re-orthogonalises them (utils.py:303-317)."""
+    shape = eigenvecs.shape
+    r = shape[0]
+    v = eigenvecs.reshape(r, -1)
+    m = mean.reshape(1, -1)
+    v = v - (v @ m.conj().T) @ m / torch.norm(m) ** 2
+    _, S, V = torch.linalg.svd(v, full_matrices=False)
+    return (S.reshape(-1, 1) * V).reshape(shape)
+
+
+class CovarPoseTrainer(CovarTrainer):
+    """Covariance SGD with pose refinement (covar_sgd.py:548-1118).
+
+    Each step (``run_batch``, covar_sgd.py:822-891) takes the batch's rotations, offsets (and
+    contrasts) from a ``PoseModule``, preprocesses the images with them (phase shift, projected
+    mean subtraction, soft projected mask) and evaluates the covariance objective with the
+    preprocessed images, so one backward pass gives the gradient of the eigenvolumes and of the
+    rotation vectors (through the NUFFT's point gradient); Adam updates the eigenvolumes and
+    SparseAdam the rotation vectors of the batch (lr ratio 3).  Every ``mean_update_frequency``
+    epochs the offsets are re-estimated per image by Newton's method on the ML objective, the
+    dataset takes the refined poses and the mean is reconstructed from half-sets.
+
+    The objective runs in its autograd form here (``form="autograd"``): its projections return the
+    point gradient the rotations need, through the NUFFT's point derivative (``nufft_disc="nufft"``)
+    or the trilinear slice's grid derivative (``"bilinear"``, the reference's default; ``"nearest"``
+    has none, as with grid_sample).
+    """
+
+    def __init__(self, covar: Covar, train_data, device: torch.device, mean, pose, save_path: Optional[str] = None,
+                 gt_data=None, training_log_freq: int = 50) -> None:
+        super().__init__(covar, train_data, device, save_path, gt_data, training_log_freq)
+        self.mean = mean.to(self.device)
+        self.pose = pose.to(self.device)
+        self.pose_lr_ratio = 3
+        self.num_rep = 1
+        self.set_pose_grad_req(True)
+        self._updated_idx = torch.zeros(len(self.dataset), device=self.device)
+        self.mean_fourier_reg = None
+        self.scheduler_patiece = 3
+        self.downsample_factor = min(3, int(np.log2(self.covar.resolution / 32)))
+        self.mean_est_method = "Reconstruction"
+        self.offset_est_method = "Newton"
+        self.rotation_est_method = "SGD"
+        self.mean_update_frequency = 5
+        self.use_eigen_in_contrast_from_epoch = 0
+        for prm in self.mean.parameters():
+            prm.requires_grad = False
+        for prm in self.pose.offsets.parameters():
+            prm.requires_grad = False
+        if self.pose.use_contrast:
+            for prm in self.pose.contrasts.parameters():
+                prm.requires_grad = False
+
+    @property
+    def downsample_size(self) -> int:
+        return int(2 ** (-self.downsample_factor) * self.pose.resolution)
+
+    def get_mean_module(self):
+        return self.mean
+
+    def get_pose_module(self):
+        return self.pose
+
+    def set_pose_grad_req(self, grad: bool) -> None:
+        for prm in self.pose.parameters():
+            prm.requires_grad = grad
+        for prm in self.mean.parameters():
+            prm.requires_grad = grad
+
+    def get_trainable_parameters(self):
+        return super().get_trainable_parameters() + [{"params": self.mean.parameters(), "lr": 1}]
+
+    def prepare_batch(self, batch):
+        images, _, filters, idx = batch
+        if isinstance(idx, slice):
+            idx = torch.arange(idx.start or 0, idx.stop, device=self.device)
+        return images.to(self.device), idx.to(self.device), filters.to(self.device)
+
+    def run_batch(self, images: torch.Tensor, idx: torch.Tensor, filters: torch.Tensor) -> torch.Tensor:
+        from .projection_funcs import lowpass_volume, preprocess_image_batch
+
+        for _ in range(self.num_rep):
+            self.optimizer.zero_grad()
+            self.pose_optimizer.zero_grad()
+            if self.pose.use_contrast:
+                pts_rot, phase_shift, contrasts = self.pose(idx)
+                filters = filters * contrasts.reshape(-1, 1, 1)
+            else:
+                pts_rot, phase_shift = self.pose(idx)
+            pre = preprocess_image_batch(images, self.nufft_plans, filters, (pts_rot, phase_shift), self.mean(None),
+                                         self.mask, self.mask_threshold, self.softening_kernel_fourier,
+                                         fourier_domain=self.optimize_in_fourier_domain)
+            kw = ({"form": "autograd"} if self.cost_func in (cost_maximum_liklihood,
+                                                              cost_maximum_liklihood_fourier_domain) else {})
+            cost_val = self.cost_func(self._covar(None), pre, self.nufft_plans, filters, self.noise_var,
+                                      self.reg_scale, self.fourier_reg, apply_mean_const_term=True, **kw)
+            cost_val.backward()
+            self.optimizer.step()
+            self.pose_optimizer.step()
+        if self.use_orthogonal_projection:
+            self.covar.orthogonal_projection()
+        with torch.no_grad():
+            self.covar.vectors.data.copy_(lowpass_volume(self.covar.vectors.data, self.downsample_size // 2))
+        self._updated_idx[idx] = 1
+        return cost_val
+
+    def run_epoch(self, epoch: int) -> None:
+        super().run_epoch(epoch)
+        if self.pose.use_contrast:
+            with torch.no_grad():
+                v = _project_mean_out(self.covar.get_vectors_spatial_domain(), self.mean.get_volume_spatial_domain())
+                self.covar.vectors.data.copy_(v / torch.exp(self.covar.log_sqrt_eigenvals).reshape(-1, 1, 1, 1))
+        if epoch % self.mean_update_frequency == self.mean_update_frequency - 1:
+            if self.offset_est_method == "Newton":
+                self.correct_offsets()
+            if self.mean_est_method == "Reconstruction":
+                from .mean import reconstruct_mean_from_halfsets
+
+                self.dataset.update_pose(self.pose)
+                self.mean.set_mean(reconstruct_mean_from_halfsets(self.dataset, mask=self.mean.volume_mask))
+
+    def correct_offsets(self) -> None:
+        """Newton re-estimation of every image's offset on the ML objective with the current mean,
+        eigenvolumes and rotations (covar_sgd.py:703-783)."""
+        from .poses import estimate_image_offsets_newton
+        from .projection_funcs import centered_fft2, centered_ifft2, vol_forward
+
+        for batch in self.train_data:
+            images, idx, filters = self.prepare_batch(batch)
+            with torch.no_grad():
+                pts = self.pose(idx)[0]
+                self.nufft_plans.setpts(pts)
+                fourier = self.optimize_in_fourier_domain
+                mean_fwd = vol_forward(self.mean(None), self.nufft_plans, filters, fourier_domain=fourier).squeeze(1)
+                proj = vol_forward(self._covar(None), self.nufft_plans, filters, fourier_domain=fourier)
+                soft = None
+                if self.mask is not None:
+                    mf = vol_forward(self.mask, self.nufft_plans, filters=None).squeeze(1)
+                    soft = centered_ifft2(centered_fft2((mf > self.mask_threshold).to(mf.dtype)) *
+                                          self.softening_kernel_fourier).real
+
+            def obj(shifted):
+                if not fourier:
+                    shifted = centered_ifft2(shifted).real
+                return self.raw_cost_func(proj, shifted - mean_fwd, self.noise_var, apply_mean_const_term=True,
+                                          mean_aggregate=False)
+
+            off = estimate_image_offsets_newton(images, mean_fwd, mask=soft, init_offsets=self.pose.get_offsets()[idx],
+                                                in_fourier_domain=fourier, obj_func=obj)
+            self.pose.set_offsets(off, idx=idx)
+
+    def setup_training(self, **kwargs) -> None:
+        from .projection_funcs import get_mask_threshold
+        from .utils import soft_edged_kernel
+
+        super().setup_training(**kwargs)
+        self.mean.init_grid_correction(kwargs.get("nufft_disc"))
+        self.mean._in_spatial_domain = not self.optimize_in_fourier_domain
+        self.mask = self.mean.get_volume_mask()
+        # the per-image objectives of the offset search (generic in the domain, as the reference uses them)
+        self.raw_cost_func = (raw_cost_fourier_domain if kwargs.get("objective_func") == "ls"
+                              else raw_cost_maximum_liklihood_fourier_domain)
+        self.softening_kernel_fourier = soft_edged_kernel(radius=5, L=self.mean.resolution, dim=2,
+                                                          in_fourier=True).to(self.device)
+        self.mask_threshold = None
+        if self.mask is not None:
+            with torch.no_grad():
+                idx = torch.arange(min(self.batch_size, len(self.dataset)), device=self.device)
+                self.nufft_plans.setpts(self.pose(idx)[0])
+                self.mask_threshold = get_mask_threshold(self.mask, self.nufft_plans)
+
+    def restart_optimizer(self) -> None:
+        super().restart_optimizer()
+        self.pose_optimizer = torch.optim.SparseAdam([{"params": list(self.pose.parameters()),
+                                                       "lr": self.pose_lr_ratio * self.lr}])
+
+    def train_epochs(self, max_epochs: int, **training_kwargs) -> None:
+        while self.downsample_factor >= 0:
+            super().train_epochs(max_epochs, **training_kwargs)
+            self.downsample_factor -= 1
+        self.downsample_factor = 0
+
+    def results_dict(self) -> dict:
+        ckp = super().results_dict()
+        ckp["mean"] = {k: v.detach().cpu() if isinstance(v, torch.Tensor) else v
+                       for k, v in self.mean.state_dict().items()}
+        ckp["pose"] = {k: v.detach().cpu() for k, v in self.pose.state_dict().items()}
+        return ckp
+
+
 def load_training_results(path: str) -> dict:
     """Reads a ``training_results.bin`` written by ``CovarTrainer.save_result`` (tensors, lists,
     numbers, None only), with the loader that executes nothing from the file."""
@@ -408,15 +610,18 @@ def trainCovar(covar_model, dataset, batch_size, optimize_pose=False, mean_model
     """Entry point (covar_sgd.py:1479-1612): optional regulariser rounds, then a final training pass."""
     from .dataset import create_dataloader
 
-    if optimize_pose:
-        raise NotImplementedError("pose refinement is outside this package")
     use_halfsets = kwargs.pop("use_halfsets")
+    if optimize_pose and use_halfsets:
+        raise NotImplementedError("pose refinement with half-sets (PoseModule.split_module) is not built")
     num_reg_update_iters = kwargs.pop("num_reg_update_iters", None) or 0
     num_epochs = kwargs.pop("max_epochs")
     device = covar_model.device
     if not use_halfsets:
-        trainer = CovarTrainer(covar_model, create_dataloader(dataset, batch_size=batch_size), device, savepath,
-                               gt_data=gt_data)
+        loader = create_dataloader(dataset, batch_size=batch_size)
+        if optimize_pose:
+            trainer = CovarPoseTrainer(covar_model, loader, device, mean_model, pose, savepath, gt_data=gt_data)
+        else:
+            trainer = CovarTrainer(covar_model, loader, device, savepath, gt_data=gt_data)
         trainer.setup_training(**kwargs)
         for _ in range(num_reg_update_iters):
             trainer.train_epochs(num_epochs, restart_optimizer=True)

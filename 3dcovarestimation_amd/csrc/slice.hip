@@ -106,6 +106,61 @@ __global__ void __launch_bounds__(kSliceBlock)
     if (t0 + u < ntrans) out[(int64_t)(t0 + u) * npts + j] = acc[u];
 }
 
+// Gradient of the trilinear slice with respect to its normalised grid coordinates, the
+// derivative torch's grid_sample takes for the grid (the pose gradient of the discretised
+// projection).  One thread per point, all transforms in a loop: for each transform t the
+// output gradient g and the 8 corner values a_k give s_k = Re(conj(g) a_k), and
+//   d/d ix = sum_k s_k * (+-1)_x * wy_k * wz_k  (likewise y, z),
+// scaled by (size - 1) / 2 from the align_corners unnormalisation.  Corners outside the volume
+// contribute nothing (zero padding).  Nearest sampling is piecewise constant: its gradient is 0.
+template <typename T, int MODE>
+__global__ void __launch_bounds__(kSliceBlock)
+    k_slice_point_grad(const Cplx<T>* __restrict__ vol, const T* __restrict__ grid,
+                       const Cplx<T>* __restrict__ gout, T* __restrict__ ggrid, int64_t D, int64_t H,
+                       int64_t W, int64_t npts, int ntrans) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= npts) return;
+  T gx = 0, gy = 0, gz = 0;
+  if (MODE == 0) {
+    const int64_t vsize = D * H * W;
+    const T ix = unnorm(grid[3 * j], W), iy = unnorm(grid[3 * j + 1], H), iz = unnorm(grid[3 * j + 2], D);
+    Corner8<T> cn;
+    trilinear_corners(ix, iy, iz, D, H, W, cn);
+    const T fx = floor(ix), fy = floor(iy), fz = floor(iz);
+    const T wx[2] = {(fx + (T)1) - ix, ix - fx};
+    const T wy[2] = {(fy + (T)1) - iy, iy - fy};
+    const T wz[2] = {(fz + (T)1) - iz, iz - fz};
+    T s[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s[k] = 0;
+    for (int t = 0; t < ntrans; ++t) {
+      const Cplx<T> g = gout[(int64_t)t * npts + j];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        if (cn.ok[k]) {
+          const Cplx<T> a = vol[(int64_t)t * vsize + cn.off[k]];
+          s[k] += g.x * a.x + g.y * a.y;
+        }
+      }
+    }
+    // corner k: x index bit 0, y bit 1, z bit 2 (tnw, tne, tsw, tse, bnw, bne, bsw, bse)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int bx = k & 1, by = (k >> 1) & 1, bz = (k >> 2) & 1;
+      const T sx = bx ? (T)1 : (T)-1, sy = by ? (T)1 : (T)-1, sz = bz ? (T)1 : (T)-1;
+      gx += s[k] * sx * wy[by] * wz[bz];
+      gy += s[k] * wx[bx] * sy * wz[bz];
+      gz += s[k] * wx[bx] * wy[by] * sz;
+    }
+    gx *= (T)(W - 1) / (T)2;
+    gy *= (T)(H - 1) / (T)2;
+    gz *= (T)(D - 1) / (T)2;
+  }
+  ggrid[3 * j] = gx;
+  ggrid[3 * j + 1] = gy;
+  ggrid[3 * j + 2] = gz;
+}
+
 template <typename T, int MODE>
 __global__ void __launch_bounds__(kSliceBlock)
     k_slice_adjoint(const Cplx<T>* __restrict__ sig, const T* __restrict__ grid,
@@ -359,6 +414,20 @@ static int slice_fwd(const void* vol, const void* grid, void* out, int64_t ntran
 }
 
 template <typename T>
+static int slice_pgrad(const void* vol, const void* grid, const void* gout, void* ggrid, int64_t ntrans,
+                       int64_t D, int64_t H, int64_t W, int64_t npts, int mode, hipStream_t st) {
+  dim3 g((unsigned)ceil_div(npts, kSliceBlock));
+  if (mode == 0)
+    hipLaunchKernelGGL((k_slice_point_grad<T, 0>), g, dim3(kSliceBlock), 0, st, (const Cplx<T>*)vol,
+                       (const T*)grid, (const Cplx<T>*)gout, (T*)ggrid, D, H, W, npts, (int)ntrans);
+  else
+    hipLaunchKernelGGL((k_slice_point_grad<T, 1>), g, dim3(kSliceBlock), 0, st, (const Cplx<T>*)vol,
+                       (const T*)grid, (const Cplx<T>*)gout, (T*)ggrid, D, H, W, npts, (int)ntrans);
+  COV3D_LAUNCH_CHECK();
+  return 0;
+}
+
+template <typename T>
 static int slice_adj(const void* sig, const void* grid, void* vol, int64_t ntrans, int64_t D,
                      int64_t H, int64_t W, int64_t npts, int mode, int aggregated, int64_t ppv,
                      hipStream_t st) {
@@ -407,4 +476,17 @@ extern "C" int cov3d_slice_adjoint(const void* sig, const void* grid, void* vol_
                                     pts_per_vol, st);
   return cov3d::slice_adj<float>(sig, grid, vol_out, ntrans, D, H, W, npts, mode, aggregated,
                                  pts_per_vol, st);
+}
+
+extern "C" int cov3d_slice_point_grad(const void* vol, const void* grid, const void* grad_out,
+                                      void* grid_grad, int64_t ntrans, int64_t D, int64_t H, int64_t W,
+                                      int64_t npts, int mode, int dtype, void* stream) {
+  COV3D_CHECK_ARG(ntrans >= 1 && D >= 1 && H >= 1 && W >= 1 && npts >= 0, "bad slice shape");
+  COV3D_CHECK_ARG(mode == 0 || mode == 1, "mode must be 0 (bilinear) or 1 (nearest)");
+  COV3D_CHECK_ARG(dtype == 0 || dtype == 1, "dtype must be 0 or 1");
+  if (npts == 0) return 0;
+  COV3D_CHECK_ARG(vol && grid && grad_out && grid_grad, "NULL pointer");
+  hipStream_t st = cov3d::as_stream(stream);
+  if (dtype == 1) return cov3d::slice_pgrad<double>(vol, grid, grad_out, grid_grad, ntrans, D, H, W, npts, mode, st);
+  return cov3d::slice_pgrad<float>(vol, grid, grad_out, grid_grad, ntrans, D, H, W, npts, mode, st);
 }

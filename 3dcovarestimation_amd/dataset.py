@@ -183,6 +183,18 @@ class CovarDataset:
     def _store(self, idx, images: torch.Tensor) -> None:
         self.images[idx] = images
 
+    def update_pose(self, pose_module, batch_size: int = 16384) -> None:
+        """Rotations and offsets (and contrasts) from a PoseModule (dataset.py:503-515)."""
+        from .poses import rotvec_to_rotmat
+
+        with torch.no_grad():
+            for i in range(0, len(self), batch_size):
+                sl = slice(i, min(i + batch_size, len(self)))
+                self.rotations[sl] = rotvec_to_rotmat(pose_module.get_rotvecs()[sl]).to(self.rotations.dtype)
+                self.offsets[sl] = pose_module.get_offsets()[sl].to(self.offsets.dtype)
+                if pose_module.use_contrast:
+                    self.contrasts[sl] = pose_module.get_contrasts()[sl].reshape(-1).to(self.contrasts.dtype)
+
     def get_total_gain(self, batch_size: int = 4096) -> torch.Tensor:
         """diag(sum_i P_i^T P_i) on the (L,)^3 grid (dataset.py:322-356): the nearest-neighbour
         (upsampling 1) back-projection of CTF^2 with the Nyquist row and column masked, summed

@@ -42,8 +42,8 @@ def get_grid_correction(L: int, upsampling_factor: int, nufft_disc: str) -> torc
 def vdot_f64(x: torch.Tensor, y: torch.Tensor) -> torch.Tensor:
     """sum(conj(x) * y) of complex64 vectors, accumulated in FP64; returns a complex128 device scalar."""
     _native.require_device(x, "x")
-    x = x.reshape(-1).to(torch.complex64).contiguous()
-    y = y.reshape(-1).to(torch.complex64).contiguous()
+    x = _native.dense(x.reshape(-1).to(torch.complex64))
+    y = _native.dense(y.reshape(-1).to(torch.complex64))
     out = torch.empty(2, dtype=torch.float64, device=x.device)
     _native.check(_native.load().cov3d_vdot_c64_f64(x.data_ptr(), y.data_ptr(), x.numel(), out.data_ptr(),
                                                      _native.stream_ptr(x.device)))
@@ -54,8 +54,8 @@ def toeplitz_apply(x: torch.Tensor, lhs: torch.Tensor, L: int) -> torch.Tensor:
     """crop_L(ifft3c(lhs * fft3c(pad_nf(x)))) for x with L^3 complex entries (mean.py:118-123)."""
     _native.require_device(x, "x")
     nf = lhs.shape[-1]
-    x = x.reshape(-1).to(torch.complex64).contiguous()
-    lhs = lhs.to(torch.complex64).contiguous()
+    x = _native.dense(x.reshape(-1).to(torch.complex64))
+    lhs = _native.dense(lhs.to(torch.complex64))
     y = torch.empty(L**3, dtype=torch.complex64, device=x.device)
     _native.check(_native.load().cov3d_toeplitz_apply(x.data_ptr(), lhs.data_ptr(), y.data_ptr(), L, nf,
                                                        _native.stream_ptr(x.device)))

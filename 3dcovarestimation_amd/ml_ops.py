@@ -75,8 +75,8 @@ def ml_forward(
     r, B = c.shape[0], c.shape[1]
     dev = c.device
     _native.require_device(c, "c")
-    c = c.contiguous()
-    yhat = yhat.to(torch.complex64).contiguous()
+    c = _native.dense(c)
+    yhat = _native.dense(yhat.to(torch.complex64))
     ctf_t = ctf.to(torch.float32).contiguous() if ctf is not None else None
     lam_t = lam_inv.to(device=dev, dtype=torch.float32).contiguous() if lam_inv is not None else None
     out = {
@@ -121,8 +121,8 @@ def ml_grad_images(
     lib = _native.load()
     _native.check(
         lib.cov3d_ml_grad_images(
-            c.data_ptr(), _native.ptr(ctf_t), yhat.contiguous().data_ptr(), z.contiguous().data_ptr(),
-            minv.contiguous().data_ptr(), _native.ptr(pix_map), 0 if pix_map is None else pix_map.numel(),
+            c.data_ptr(), _native.ptr(ctf_t), _native.dense(yhat).data_ptr(), _native.dense(z).data_ptr(),
+            _native.dense(minv).data_ptr(), _native.ptr(pix_map), 0 if pix_map is None else pix_map.numel(),
             g.data_ptr(), B, r, L, float(noise_var), float(grad_scale),
             _native.stream_ptr(c.device),
         )
@@ -150,8 +150,8 @@ def ml_grad_sorted(
     lib = _native.load()
     _native.check(
         lib.cov3d_ml_grad_sorted(
-            c.data_ptr(), _native.ptr(ctf_t), yhat.contiguous().data_ptr(), z.contiguous().data_ptr(),
-            minv.contiguous().data_ptr(), _native.ptr(pix_map), 0 if pix_map is None else pix_map.numel(),
+            c.data_ptr(), _native.ptr(ctf_t), _native.dense(yhat).data_ptr(), _native.dense(z).data_ptr(),
+            _native.dense(minv).data_ptr(), _native.ptr(pix_map), 0 if pix_map is None else pix_map.numel(),
             values, spos, ldc, B, r, L, float(noise_var), float(grad_scale), _native.stream_ptr(c.device),
         )
     )

@@ -115,7 +115,7 @@ class NufftPlanDiscretized(BaseNufftPlan):
         if squeeze:
             vol = vol.unsqueeze(0)
         _native.require_device(vol, "volume")
-        vol = vol.contiguous()
+        vol = _native.dense(vol)
         grid = self._grid_flat.to(device=vol.device, dtype=_real_of(vol.dtype))
         n, D, H, W = vol.shape
         npts = grid.shape[0]
@@ -135,7 +135,7 @@ class NufftPlanDiscretized(BaseNufftPlan):
         vL = L * self.upsample_factor
         sig = signal if signal.is_complex() else signal.to(_complex_of(signal.dtype))
         _native.require_device(sig, "signal")
-        sig = sig.contiguous()
+        sig = _native.dense(sig)
         grid = self._grid_flat.to(device=sig.device, dtype=_real_of(sig.dtype))
         npts = grid.shape[0]
         lib = _native.load()
@@ -248,11 +248,17 @@ class NufftPlan(BaseNufftPlan):
         _native.require_device(points, "points")
         self.points_N = points.shape[0]
         self.points_L = int(round(points.shape[-1] ** 0.5))
-        src = points.to(self.dtype).contiguous()
-        pts = torch.empty((3, src.shape[0] * src.shape[2]), dtype=self.dtype, device=src.device)
-        _native.check(self._lib.cov3d_points_wrap(src.data_ptr(), pts.data_ptr(), src.shape[0], src.shape[2],
-                                                  _dtype_code(self.dtype), _native.stream_ptr(src.device)))
-        self.points = pts
+        if points.requires_grad:
+            # poses being optimised: keep the wrap in the autograd graph (TorchNufftForward returns the
+            # point gradient to self.points, and from there to the pose parameters)
+            pts = points.to(self.dtype).transpose(0, 1).reshape(3, -1)
+            self.points = torch.remainder(pts + torch.pi, 2 * torch.pi) - torch.pi
+        else:
+            src = points.to(self.dtype).contiguous()
+            pts = torch.empty((3, src.shape[0] * src.shape[2]), dtype=self.dtype, device=src.device)
+            _native.check(self._lib.cov3d_points_wrap(src.data_ptr(), pts.data_ptr(), src.shape[0], src.shape[2],
+                                                      _dtype_code(self.dtype), _native.stream_ptr(src.device)))
+            self.points = pts
         self._pts_synced = False
 
     def set_points_flat(self, pts: torch.Tensor, n_images: int) -> None:
@@ -267,7 +273,8 @@ class NufftPlan(BaseNufftPlan):
         if self.points is None:
             raise RuntimeError("setpts must be called before executing the plan")
         if not getattr(self, "_pts_synced", False):
-            pts = self.points
+            pts = self.points.detach().contiguous()
+            self._pts_native = pts  # kept until the next setpts
             with torch.cuda.device(self.device):
                 _native.check(
                     self._lib.cov3d_nufft_setpts(
@@ -290,7 +297,7 @@ class NufftPlan(BaseNufftPlan):
         _native.require_device(signal, "signal")
         signal, n = self._pad_batch(signal)
         real_in = not signal.is_complex()
-        sig = signal.to(self.dtype if real_in else self.complex_dtype).contiguous()
+        sig = _native.dense(signal.to(self.dtype if real_in else self.complex_dtype))
         out = self._output((self.batch_size, self.points.shape[1]), self.complex_dtype, n)
         with torch.cuda.device(self.device):
             _native.check(
@@ -310,12 +317,15 @@ class NufftPlan(BaseNufftPlan):
         return signal, n
 
     def execute_forward(self, signal: torch.Tensor) -> torch.Tensor:
-        """signal (n<=batch_size, *sz) real/complex -> (batch_size, N, Lp, Lp) complex (nufft_plan.py:338-364)."""
+        """signal (n<=batch_size, *sz) real/complex -> (n, N, Lp, Lp) complex (nufft_plan.py:338-364).
+
+        A short batch is zero-padded and its first n transforms returned, as the reference intends
+        with ``forward_signal[:orig_batch]``."""
         self._sync_points()
         _native.require_device(signal, "signal")
         signal, n = self._pad_batch(signal)
         real_in = not signal.is_complex()
-        sig = signal.to(self.dtype if real_in else self.complex_dtype).contiguous()
+        sig = _native.dense(signal.to(self.dtype if real_in else self.complex_dtype))
         M = self.points.shape[1]
         out = self._output((self.batch_size, M), self.complex_dtype, n)
         with torch.cuda.device(self.device):
@@ -325,10 +335,10 @@ class NufftPlan(BaseNufftPlan):
                     _native.stream_ptr(self.device),
                 )
             )
-        return out.reshape(self.batch_size, self.points_N, self.points_L, self.points_L)
+        return out[:n].reshape(n, self.points_N, self.points_L, self.points_L)
 
     def execute_adjoint(self, signal: torch.Tensor, real_output: bool = False) -> torch.Tensor:
-        """signal (n<=batch_size, ...) complex -> (batch_size, *sz) complex (nufft_plan.py:366-392).
+        """signal (n<=batch_size, ...) complex -> (n, *sz) complex (nufft_plan.py:366-392).
 
         ``real_output=True`` returns only the real part (what every real-volume
         caller takes) without materialising the imaginary half.
@@ -337,7 +347,7 @@ class NufftPlan(BaseNufftPlan):
         _native.require_device(signal, "signal")
         signal, n = self._pad_batch(signal)
         M = self.points.shape[1]
-        sig = signal.to(self.complex_dtype).reshape(n, -1).contiguous()
+        sig = _native.dense(signal.to(self.complex_dtype).reshape(n, -1))
         if sig.shape[1] != M:
             raise ValueError(f"signal has {sig.shape[1]} samples per transform, plan has {M} points")
         out_dtype = self.dtype if real_output else self.complex_dtype
@@ -349,7 +359,7 @@ class NufftPlan(BaseNufftPlan):
                     _native.stream_ptr(self.device),
                 )
             )
-        return out
+        return out[:n]
 
     def sorted_input(self, n: int) -> Tuple[int, int, int]:
         """(values, ldc, spos) device pointers of the plan's sorted type-1 input for n transforms:
@@ -413,6 +423,7 @@ class TorchNufftForward(torch.autograd.Function):
         if reset_pts:
             nufft_plan.setpts(points)
         ctx.signal_shape = signal.shape
+        ctx.points_shape = points.shape
         ctx.nufft_plan = nufft_plan
         ctx.complex_input = signal.is_complex()
         ctx.save_for_backward(signal)
@@ -439,7 +450,9 @@ class TorchNufftForward(torch.autograd.Function):
             for k in range(3):
                 fk = plan.execute_forward(signal * (-1j * grids[k]).to(plan.complex_dtype))
                 points_grad[k] = (fk.conj() * grad_output).real.sum(dim=0).reshape(-1)
-            points_grad = points_grad.reshape(3, plan.points_N, -1).transpose(0, 1)
+            if len(ctx.points_shape) == 3:  # (N, 3, L^2) as given to setpts; else the plan's (3, M)
+                points_grad = points_grad.reshape(3, plan.points_N, -1).transpose(0, 1)
+            points_grad = points_grad.reshape(ctx.points_shape)
         return signal_grad, points_grad, None, None
 
 
@@ -465,7 +478,7 @@ def nufft_forward(signal: torch.Tensor, nufft_plan: BaseNufftPlan) -> torch.Tens
     """Forward projection with autograd support (nufft_plan.py:487-501)."""
     if isinstance(nufft_plan, NufftPlan):
         return TorchNufftForward.apply(signal, nufft_plan.points, nufft_plan)
-    return _DiscretizedForward.apply(signal, nufft_plan)
+    return _DiscretizedForward.apply(signal, nufft_plan._grid_flat, nufft_plan)
 
 
 def nufft_adjoint(signal: torch.Tensor, nufft_plan: BaseNufftPlan) -> torch.Tensor:
@@ -476,29 +489,50 @@ def nufft_adjoint(signal: torch.Tensor, nufft_plan: BaseNufftPlan) -> torch.Tens
 
 
 class _DiscretizedForward(torch.autograd.Function):
-    """Slice extraction whose backward is the native scatter adjoint.
+    """Slice extraction whose backward is the native gather adjoint (volumes) and the native
+    trilinear derivative (grid coordinates).
 
-    In the reference, autograd differentiates grid_sample directly; here the
-    gradient of the (n, vL^3) volume is the adjoint slice of the output
-    gradient, one volume per trans.
+    In the reference, autograd differentiates grid_sample directly (nufft_plan.py:148-150): the
+    gradient of the (n, vL^3) volume is the adjoint slice of the output gradient, one volume per
+    trans, and the gradient of the grid (when the points carry the rotations' gradient, as in
+    pose refinement) is ``cov3d_slice_point_grad``, from which autograd continues through the
+    points-to-grid scaling of ``setpts`` to the points.
     """
 
     @staticmethod
-    def forward(ctx, volume, plan):
+    def forward(ctx, volume, grid_flat, plan):
         ctx.plan = plan
         ctx.complex_input = volume.is_complex()
         ctx.shape = volume.shape
+        if ctx.needs_input_grad[1]:
+            vol = plan._as_complex_volume(volume.detach())
+            ctx.save_for_backward(_native.dense(vol.reshape(-1, *vol.shape[-3:])), grid_flat.detach())
         return plan.execute_forward(volume)
 
     @staticmethod
     def backward(ctx, g):
         plan = ctx.plan
         n = g.shape[0]
-        vols = plan._adjoint(g.reshape(n, -1), aggregated=True)
-        vols = vols.reshape(ctx.shape)
-        if not ctx.complex_input:
-            vols = vols.real
-        return vols, None
+        vols = grid_grad = None
+        if ctx.needs_input_grad[0]:
+            vols = plan._adjoint(g.reshape(n, -1), aggregated=True)
+            vols = vols.reshape(ctx.shape)
+            if not ctx.complex_input:
+                vols = vols.real
+        if ctx.needs_input_grad[1]:
+            vol, grid = ctx.saved_tensors
+            gc = _native.dense(g.to(vol.dtype).reshape(vol.shape[0], -1))
+            grid = grid.to(vol.real.dtype).contiguous()
+            grid_grad = torch.empty_like(grid)
+            _, D, H, W = vol.shape
+            _native.check(
+                _native.load().cov3d_slice_point_grad(
+                    vol.data_ptr(), grid.data_ptr(), gc.data_ptr(), grid_grad.data_ptr(), vol.shape[0], D, H, W,
+                    grid.shape[0], 0 if plan.mode == "bilinear" else 1, _dtype_code(vol.dtype),
+                    _native.stream_ptr(vol.device),
+                )
+            )
+        return vols, grid_grad, None
 
 
 class _DiscretizedAdjoint(torch.autograd.Function):

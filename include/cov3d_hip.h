@@ -124,6 +124,17 @@ int cov3d_slice_adjoint(const void* sig, const void* grid, void* vol_out,
                         int64_t ntrans, int64_t D, int64_t H, int64_t W,
                         int64_t npts, int mode, int aggregated,
                         int64_t pts_per_vol, int dtype, void* stream);
+/* Gradient of cov3d_slice_forward with respect to its grid: for each point j,
+ *   grid_grad[j, a] = sum_t Re(conj(grad_out[t, j]) * d out[t, j] / d grid[j, a])
+ * (a = x, y, z), the grid derivative torch's grid_sample returns when the
+ * reference differentiates the discretised projection in the rotations
+ * (NufftPlanDiscretized.execute_forward, cov3d/nufft_plan.py:148-150, under
+ * PoseModule.forward, cov3d/poses.py:237-269).  vol (ntrans, D, H, W) and
+ * grad_out (ntrans, npts) complex; grid_grad (npts, 3) real, overwritten.
+ * mode 1 (nearest) writes zeros. */
+int cov3d_slice_point_grad(const void* vol, const void* grid, const void* grad_out,
+                           void* grid_grad, int64_t ntrans, int64_t D, int64_t H,
+                           int64_t W, int64_t npts, int mode, int dtype, void* stream);
 
 /* ------------------------------------------------------------------------
  * Maximum-likelihood contractions for rank-r projected eigenvectors.
