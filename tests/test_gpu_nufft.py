@@ -263,6 +263,38 @@ def test_pipelined_interp_is_bitwise_equal(dev, monkeypatch, ntrans):
     assert outs[0].abs().max() > 0
 
 
+@pytest.mark.parametrize("ntrans,pipe", [(20, "1"), (5, "1"), (20, "0")])
+def test_dense_slices_interp_matches_fp64(dev, monkeypatch, ntrans, pipe):
+    """Central slices of 96 images at L=64 (thousands of subproblems, the central ones full and
+    split): every output of the FP32 interpolation (bank-ordered point records) against the FP64
+    plan (points in sorted order), and the points of the densest region against the NUDFT.  A
+    point written to another point's place would show as an O(1) relative error."""
+    o = oracle()
+    npl = pkg("nufft_plan")
+    monkeypatch.setenv("COV3D_INTERP_PIPE", pipe)
+    dsm = pkg("dataset")
+    L, B = 64, 96
+    g = torch.Generator().manual_seed(5)
+    rots = dsm.haar_rotations(B, g).to(dev)
+    pts = dsm.pts_rot_from_rotations(rots, L)
+    vol = torch.randn((ntrans, L, L, L), generator=g).to(dev)
+    plan = npl.NufftPlan((L,) * 3, batch_size=ntrans, eps=1e-6, device=dev)
+    plan.setpts(pts)
+    out = plan.execute_forward(vol).reshape(ntrans, -1).cpu().numpy()
+    plan64 = npl.NufftPlan((L,) * 3, batch_size=ntrans, eps=1e-6, dtype=torch.float64, device=dev)
+    plan64.setpts(pts.double())
+    out64 = plan64.execute_forward(vol.double()).reshape(ntrans, -1).cpu().numpy()
+    assert rel_l2(out, out64) < TOL_F32
+    pointwise = np.abs(out - out64).max(axis=0) / np.maximum(np.abs(out64).max(axis=0), 1e-3 * np.abs(out64).max())
+    assert pointwise.max() < 1e-2
+    # the 8 x 8 central pixels of every image: the densest subproblems
+    pix = (np.arange(L // 2 - 4, L // 2 + 4)[:, None] * L + np.arange(L // 2 - 4, L // 2 + 4)[None, :]).ravel()
+    sel = (np.arange(B)[:, None] * L * L + pix[None, :]).ravel()
+    P = plan.points.cpu().numpy().astype(np.float64)[:, sel]
+    ref = o.nudft_type2(vol.cpu().numpy().astype(np.complex128), P)
+    assert rel_l2(out[:, sel], ref) < TOL_F32
+
+
 @pytest.mark.parametrize("L,nrun", [(32, 3), (64, 3), (32, 2), (256, 1)])
 def test_pruned_fft_matches_rocfft_and_nudft(dev, monkeypatch, L, nrun):
     """The pruned real FFT passes (cubic power-of-two fine grids, the FP32 default there) against
