@@ -16,6 +16,11 @@ r=0, 5% amplitude jitter): time and iterations to relative residual 1e-6 for
 plain CG and for PCG with the learned diagonal (and the probed exact
 diagonal as a reference point), reported under ``mean_cg``.
 
+``pipeline_L256`` is the full chosen-form pipeline at L=256 with the images
+streamed from pinned host memory (a bounded sample of the 300k-image workload):
+SGD images/s with the NUFFT kernels' rates, the mean system's plain CG and
+learned PCG per iteration, and the Wiener coordinates' images/s.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cg]
 """
 
@@ -64,6 +69,10 @@ def parse():
     ap.add_argument("--fp64-eps", type=float, default=1e-9, help="NUFFT tolerance of the FP64 trajectory")
     ap.add_argument("--cg-images", type=int, default=100000)
     ap.add_argument("--cg-cap", type=int, default=20000, help="iteration cap of each CG solve")
+    ap.add_argument("--pipeline-images", type=int, default=8192,
+                    help="images of the L=256 streamed pipeline leg (0 skips it)")
+    ap.add_argument("--pipeline-cg-iters", type=int, default=300,
+                    help="iterations of each mean-CG solve in the L=256 pipeline leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 1 GiB device copy (calibrates HBM byte counters)")
     return ap.parse_args()
@@ -76,7 +85,7 @@ def sgd_run(pkg, data, L, r, B, steps, warmup, form, timing=False, precision=Non
     sgd = importlib.import_module(pkg + ".covar_sgd")
     dsm = importlib.import_module(pkg + ".dataset")
     native = importlib.import_module(pkg + "._native")
-    dev = data.dataset.images.device
+    dev = data.dataset.device
     torch.manual_seed(0)
     covar = covar_mod.Covar(L, r, pixel_var_estimate=1.0 / L).to(dev)
     loader = dsm.create_dataloader(data.dataset, batch_size=B)
@@ -212,6 +221,81 @@ def mean_cg_run(pkg, L, n_images, dev, cap=20000):
     del data, lhs, rhs, reg_lhs
     torch.cuda.empty_cache()
     return res
+
+
+def nufft_rates(kern, L, r, B, d, w=7):
+    """Algorithmic HBM GB/s and FP32 TFLOP/s of the interpolation and spreading launches."""
+    nf = 2 * L
+    bytes_per_launch = B * (12 * d + 8 * r * d) + 8 * r * nf**3
+    flops_per_launch = B * d * (w**3 * r * 4)
+    out = {}
+    for k in ("interp", "spread"):
+        ms = kern[k]["avg_ms"]
+        if ms:
+            out[k] = {"avg_ms": round(ms, 3), "achieved_GBs": round(bytes_per_launch / (ms * 1e-3) / 1e9, 1),
+                      "achieved_TFLOPs": round(flops_per_launch / (ms * 1e-3) / 1e12, 2),
+                      "frac_fp32_peak": round(flops_per_launch / (ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4)}
+    return out
+
+
+def pipeline_run(pkg, dev, n_images, steps, cg_iters, L=256, r=20, B=1024):
+    """The L=256 pipeline on one GPU with the images streamed from pinned host memory: covariance
+    SGD steps of the chosen form (batches of B copied on a second stream behind the step), the mean
+    system (back-projection, plain CG and learned-diagonal PCG for a fixed number of iterations)
+    and the Wiener coordinates of every image (FP64 r x r Cholesky per image)."""
+    import numpy as np
+    import torch
+
+    dsm = importlib.import_module(pkg + ".dataset")
+    mean = importlib.import_module(pkg + ".mean")
+    pc = importlib.import_module(pkg + ".precond")
+    wc = importlib.import_module(pkg + ".wiener_coords")
+    ml_ops = importlib.import_module(pkg + ".ml_ops")
+    cfg = dsm.SyntheticConfig(L, n_images, r, list(np.geomspace(10, 0.1, r)), 1.0, 0.01, (0.5, 3.5), 0.0, 1005)
+    t0 = time.perf_counter()
+    data = dsm.make_synthetic(cfg, dev, n_images=n_images, subtract_mean=False, host_images=True)
+    torch.cuda.synchronize()
+    log(f"pipeline L={L}: {n_images} images generated into pinned host memory in {time.perf_counter() - t0:.1f} s")
+    out = {"workload": (f"L={L}, r={r}, batch {B}, {n_images} images of {L}x{L} in pinned host memory "
+                        f"(pixel 1.0 A, SNR 0.01, defocus U[0.5,3.5] um), streamed to the device per batch"),
+           "images": n_images}
+    steps = min(steps, n_images // B - 1)
+    dt, kern = sgd_run(pkg, data, L, r, B, steps, 1, "fused", timing=True)
+    d = int(ml_ops.half_plane_pixels(L, "cpu").numel())
+    out["sgd"] = {"value": round(steps * B / dt, 1), "unit": "images/s", "ms_per_step": round(1e3 * dt / steps, 2),
+                  "steps": steps, "nufft_kernels": nufft_rates(kern, L, r, B, d)}
+    log(f"pipeline L={L}: SGD {out['sgd']['value']} images/s ({out['sgd']['ms_per_step']} ms/step)")
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    lhs, rhs = mean.backproject_lhs_rhs(data.dataset, batch_size=B)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    reg_lhs = mean.regularize_lhs(lhs)
+    try:
+        model = pc.load_default(dev)
+    except FileNotFoundError:
+        model = None
+    pc.cg_comparison(reg_lhs, rhs, L, model, data.dataset.noise_var, maxiter=3, with_probe=False)
+    res = pc.cg_comparison(reg_lhs, rhs, L, model, data.dataset.noise_var, rtol=1e-6, maxiter=cg_iters,
+                           with_probe=False)
+    res["backprojection_images_per_s"] = round(n_images / (t2 - t1), 1)
+    res["note"] = f"each solve stops after {cg_iters} iterations; the residual it reached is final_residual"
+    out["mean_cg"] = res
+    log(f"pipeline L={L}: mean CG {res['plain'].get('ms_per_iteration')} ms/it plain, "
+        f"{(res.get('learned') or {}).get('ms_per_iteration')} ms/it learned")
+    del lhs, rhs, reg_lhs
+    ev = (data.eigenvecs).to(dev)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    coords = wc.latentMAP(data.dataset, ev, data.eigenvals.to(dev), batch_size=B)
+    torch.cuda.synchronize()
+    t4 = time.perf_counter()
+    out["wiener_coords"] = {"value": round(n_images / (t4 - t3), 1), "unit": "images/s",
+                            "finite": bool(torch.isfinite(coords).all())}
+    log(f"pipeline L={L}: Wiener coordinates {out['wiener_coords']['value']} images/s")
+    del data, coords, ev
+    torch.cuda.empty_cache()
+    return out
 
 
 def cpu_baseline(L, r, B_cpu, steps=3, noise_var=1.0):
@@ -469,6 +553,10 @@ def main():
         cg = mean_cg_run(pkg, L, args.cg_images, dev, args.cg_cap)
         log(f"mean CG: plain {cg['plain']['iterations']} it / {cg['plain']['seconds']} s; learned "
             f"{cg.get('learned', {}).get('iterations')} it / {cg.get('learned', {}).get('seconds')} s")
+    pipe = None
+    if args.pipeline_images > 0:
+        log(f"L=256 pipeline leg: {args.pipeline_images} images streamed from pinned host memory")
+        pipe = pipeline_run(pkg, dev, args.pipeline_images, max(2, args.steps // 2), args.pipeline_cg_iters)
     cpu = None
     if not args.no_cpu:
         log(f"CPU baseline: {args.cpu_steps} steps of {args.cpu_images} images")
@@ -505,6 +593,7 @@ def main():
         "ml_vs_fp64": mlf,
         "subspace_error": sub,
         "mean_cg": cg,
+        "pipeline_L256": pipe,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
