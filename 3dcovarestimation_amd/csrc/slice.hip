@@ -27,6 +27,13 @@ struct Corner8 {
   bool ok[8];
 };
 
+// A point of a slice plan in sorted order: its unnormalised grid coordinates and caller index.
+template <typename T>
+struct SortedPoint {
+  T x, y, z;  // unnormalised grid coordinates
+  int j;      // caller index
+};
+
 // grid_sample unnormalisation with align_corners=True: ((g + 1) / 2) * (size - 1).
 template <typename T>
 __device__ __forceinline__ T unnorm(T g, int64_t size) {
@@ -256,6 +263,29 @@ __global__ void __launch_bounds__(kSliceBlock)
   for (int t = 0; t < ntrans; ++t) row[2 + t] = sig[(int64_t)t * npts + j];
 }
 
+// Rows as k_slice_rows for a slice plan: the coordinates come from the point's sorted record
+// (the plan keeps no grid pointer); the values are read in the caller's order (coalesced) and
+// the row is written at the point's sorted position.
+template <typename T>
+__global__ void __launch_bounds__(kSliceBlock)
+    k_slice_rows_plan(const Cplx<T>* __restrict__ sig, const SortedPoint<T>* __restrict__ sp,
+                      const int* __restrict__ spos, Cplx<T>* __restrict__ rows, int64_t npts, int ntrans, int rw) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= npts) return;
+  const int64_t s = spos[j];
+  const SortedPoint<T> p = sp[s];
+  Cplx<T>* row = rows + s * rw;
+  row[0] = {p.x, p.y};
+  row[1] = {p.z, (T)0};
+  for (int t = 0; t < ntrans; ++t) row[2 + t] = sig[(int64_t)t * npts + j];
+}
+
+template <typename T>
+__global__ void k_zero_vals(T* p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = (T)0;
+}
+
 // Output cell q gathers every transform from the rows of the points whose floor cell is q - d,
 // d in {0,1}^3 (nearest: the points rounded to q), in sorted order, and writes each sum once.
 template <typename T, int MODE, int RT>
@@ -443,6 +473,174 @@ static int slice_adj(const void* sig, const void* grid, void* vol, int64_t ntran
   return 0;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Slice plan: the points sorted once per setpts and shared by the forward and the aggregated
+// adjoint.  The forward in caller order reads each point's 8 corners from a volume that does not
+// fit in the caches (134 MB per transform at vL = 256), one cache line per corner, and every
+// volume cell is read by about 8 points of different images at different times.  In floor-cell
+// order the points that share corners are neighbours, so a transform's volume is streamed about
+// once and the corner reads of a wavefront hit a few cache lines; each value is then written to
+// the point's caller position.
+// ---------------------------------------------------------------------------------------------
+
+template <typename T>
+__global__ void __launch_bounds__(kSliceBlock)
+    k_slice_sorted_points(const T* __restrict__ grid, const int* __restrict__ sidx, SortedPoint<T>* __restrict__ sp,
+                          int64_t D, int64_t H, int64_t W, int64_t npts) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= npts) return;
+  const int j = sidx[s];
+  sp[s] = SortedPoint<T>{unnorm(grid[3 * (int64_t)j], W), unnorm(grid[3 * (int64_t)j + 1], H),
+                         unnorm(grid[3 * (int64_t)j + 2], D), j};
+}
+
+// The forward of k_slice_forward (same arithmetic, bit for bit) over the points in sorted order,
+// one thread per (sorted point, transform), the transform the slow grid index.
+template <typename T, int MODE>
+__global__ void __launch_bounds__(kSliceBlock)
+    k_slice_forward_sorted(const Cplx<T>* __restrict__ vol, const SortedPoint<T>* __restrict__ sp,
+                           Cplx<T>* __restrict__ out, int64_t D, int64_t H, int64_t W, int64_t npts) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= npts) return;
+  const int64_t t = blockIdx.y;
+  const int64_t vsize = D * H * W;
+  const Cplx<T>* v = vol + t * vsize;
+  const SortedPoint<T> p = sp[s];
+  Cplx<T> acc = {(T)0, (T)0};
+  if (MODE == 0) {
+    Corner8<T> cn;
+    trilinear_corners(p.x, p.y, p.z, D, H, W, cn);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if (cn.ok[k]) {
+        const Cplx<T> a = v[cn.off[k]];
+        acc.x += a.x * cn.w[k];
+        acc.y += a.y * cn.w[k];
+      }
+    }
+  } else {
+    const int64_t x = (int64_t)rint(p.x), y = (int64_t)rint(p.y), z = (int64_t)rint(p.z);
+    if (x >= 0 && x < W && y >= 0 && y < H && z >= 0 && z < D) acc = v[(z * H + y) * W + x];
+  }
+  out[t * npts + p.j] = acc;
+}
+
+struct SlicePlan {
+  int dev = -1;
+  int64_t npts = 0, D = 0, H = 0, W = 0, nkeys = 0;
+  int mode = 0, dtype = 0;
+  bool ready = false;
+  AdjWork w;             // keys, sort, runs, inverse, rows (grown on demand)
+  void* sp = nullptr;    // SortedPoint<T>[npts]
+  size_t cap_sp = 0;
+};
+
+static int plan_grow(void** p, size_t& cap, size_t need) {
+  if (need <= cap) return 0;
+  hipFree(*p);
+  *p = nullptr;
+  cap = 0;
+  COV3D_HIP(hipMalloc(p, need));
+  cap = need;
+  return 0;
+}
+
+template <typename T>
+static int slice_plan_setpts(SlicePlan* pl, const void* grid, int64_t npts, int64_t D, int64_t H, int64_t W,
+                             int mode, hipStream_t st) {
+  COV3D_CHECK_ARG(npts < ((int64_t)1 << 31), "too many points for a slice plan");
+  const int64_t nkeys = (W + 1) * (H + 1) * (D + 1);
+  COV3D_CHECK_ARG(nkeys < ((int64_t)1 << 31), "volume too large for a slice plan");
+  const unsigned sentinel = (unsigned)nkeys;
+  int nbits = 1;
+  while (((int64_t)1 << nbits) <= nkeys) ++nbits;
+  AdjWork& w = pl->w;
+  size_t tmp_bytes = 0;
+  COV3D_HIP(sort_points_by_bin(nullptr, tmp_bytes, nullptr, nullptr, nullptr, (int)npts, nbits, st));
+  if ((size_t)npts > w.cap_pts) {
+    hipFree(w.keys), hipFree(w.skeys), hipFree(w.sidx), hipFree(w.spos);
+    w.keys = w.skeys = nullptr, w.sidx = w.spos = nullptr, w.cap_pts = 0;
+    COV3D_HIP(hipMalloc(&w.keys, npts * sizeof(unsigned)));
+    COV3D_HIP(hipMalloc(&w.skeys, npts * sizeof(unsigned)));
+    COV3D_HIP(hipMalloc(&w.sidx, npts * sizeof(int)));
+    COV3D_HIP(hipMalloc(&w.spos, npts * sizeof(int)));
+    w.cap_pts = npts;
+  }
+  if ((size_t)nkeys > w.cap_keys) {
+    hipFree(w.start), hipFree(w.end);
+    w.start = w.end = nullptr, w.cap_keys = 0;
+    COV3D_HIP(hipMalloc(&w.start, nkeys * sizeof(int)));
+    COV3D_HIP(hipMalloc(&w.end, nkeys * sizeof(int)));
+    w.cap_keys = nkeys;
+  }
+  if (int rc = plan_grow(&w.tmp, w.cap_tmp, tmp_bytes)) return rc;
+  if (int rc = plan_grow(&pl->sp, pl->cap_sp, (size_t)npts * sizeof(SortedPoint<T>))) return rc;
+  const unsigned nbp = (unsigned)ceil_div(npts, kSliceBlock);
+  if (mode == 0)
+    hipLaunchKernelGGL((k_slice_keys<T, 0>), dim3(nbp), dim3(kSliceBlock), 0, st, (const T*)grid, w.keys, D, H, W,
+                       npts, sentinel);
+  else
+    hipLaunchKernelGGL((k_slice_keys<T, 1>), dim3(nbp), dim3(kSliceBlock), 0, st, (const T*)grid, w.keys, D, H, W,
+                       npts, sentinel);
+  COV3D_LAUNCH_CHECK();
+  size_t tb = w.cap_tmp;
+  COV3D_HIP(sort_points_by_bin(w.tmp, tb, w.keys, w.skeys, w.sidx, (int)npts, nbits, st));
+  const unsigned nbk = (unsigned)ceil_div(nkeys, kSliceBlock);
+  hipLaunchKernelGGL((k_zero_ints<T>), dim3(nbk), dim3(kSliceBlock), 0, st, w.start, nkeys);
+  hipLaunchKernelGGL((k_zero_ints<T>), dim3(nbk), dim3(kSliceBlock), 0, st, w.end, nkeys);
+  COV3D_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_slice_runs, dim3(nbp), dim3(kSliceBlock), 0, st, w.skeys, w.start, w.end, npts, sentinel);
+  hipLaunchKernelGGL(k_slice_inverse, dim3(nbp), dim3(kSliceBlock), 0, st, w.sidx, w.spos, npts);
+  hipLaunchKernelGGL((k_slice_sorted_points<T>), dim3(nbp), dim3(kSliceBlock), 0, st, (const T*)grid, w.sidx,
+                     (SortedPoint<T>*)pl->sp, D, H, W, npts);
+  COV3D_LAUNCH_CHECK();
+  pl->npts = npts, pl->D = D, pl->H = H, pl->W = W, pl->nkeys = nkeys, pl->mode = mode;
+  pl->ready = true;
+  return 0;
+}
+
+template <typename T>
+static int slice_plan_forward(SlicePlan* pl, const void* vol, void* out, int64_t ntrans, hipStream_t st) {
+  if (pl->npts == 0) return 0;
+  dim3 g((unsigned)ceil_div(pl->npts, kSliceBlock), (unsigned)ntrans);
+  if (pl->mode == 0)
+    hipLaunchKernelGGL((k_slice_forward_sorted<T, 0>), g, dim3(kSliceBlock), 0, st, (const Cplx<T>*)vol,
+                       (const SortedPoint<T>*)pl->sp, (Cplx<T>*)out, pl->D, pl->H, pl->W, pl->npts);
+  else
+    hipLaunchKernelGGL((k_slice_forward_sorted<T, 1>), g, dim3(kSliceBlock), 0, st, (const Cplx<T>*)vol,
+                       (const SortedPoint<T>*)pl->sp, (Cplx<T>*)out, pl->D, pl->H, pl->W, pl->npts);
+  COV3D_LAUNCH_CHECK();
+  return 0;
+}
+
+template <typename T>
+static int slice_plan_adjoint(SlicePlan* pl, const void* sig, void* vol, int64_t ntrans, hipStream_t st) {
+  AdjWork& w = pl->w;
+  const int64_t D = pl->D, H = pl->H, W = pl->W, npts = pl->npts;
+  if (npts == 0) {
+    const int64_t n = D * H * W * ntrans * 2;
+    hipLaunchKernelGGL((k_zero_vals<T>), dim3((unsigned)ceil_div(n, kSliceBlock)), dim3(kSliceBlock), 0, st, (T*)vol, n);
+    COV3D_LAUNCH_CHECK();
+    return 0;
+  }
+  const int rw = 2 + (int)ntrans;
+  if (int rc = plan_grow(&w.rows, w.cap_rows, (size_t)npts * rw * sizeof(Cplx<T>))) return rc;
+  const unsigned nbp = (unsigned)ceil_div(npts, kSliceBlock);
+  hipLaunchKernelGGL((k_slice_rows_plan<T>), dim3(nbp), dim3(kSliceBlock), 0, st, (const Cplx<T>*)sig,
+                     (const SortedPoint<T>*)pl->sp, w.spos, (Cplx<T>*)w.rows, npts, (int)ntrans, rw);
+  COV3D_LAUNCH_CHECK();
+  constexpr int RT = 20;
+  dim3 g((unsigned)ceil_div(D * H * W, kSliceBlock), (unsigned)ceil_div(ntrans, RT));
+  if (pl->mode == 0)
+    hipLaunchKernelGGL((k_slice_adjoint_gather<T, 0, RT>), g, dim3(kSliceBlock), 0, st, (const Cplx<T>*)w.rows,
+                       w.start, w.end, (Cplx<T>*)vol, D, H, W, (int)ntrans, rw);
+  else
+    hipLaunchKernelGGL((k_slice_adjoint_gather<T, 1, RT>), g, dim3(kSliceBlock), 0, st, (const Cplx<T>*)w.rows,
+                       w.start, w.end, (Cplx<T>*)vol, D, H, W, (int)ntrans, rw);
+  COV3D_LAUNCH_CHECK();
+  return 0;
+}
+
 }  // namespace cov3d
 
 extern "C" int cov3d_slice_forward(const void* vol, const void* grid, void* out, int64_t ntrans,
@@ -489,4 +687,66 @@ extern "C" int cov3d_slice_point_grad(const void* vol, const void* grid, const v
   hipStream_t st = cov3d::as_stream(stream);
   if (dtype == 1) return cov3d::slice_pgrad<double>(vol, grid, grad_out, grid_grad, ntrans, D, H, W, npts, mode, st);
   return cov3d::slice_pgrad<float>(vol, grid, grad_out, grid_grad, ntrans, D, H, W, npts, mode, st);
+}
+
+extern "C" int cov3d_slice_plan_create(void** plan) {
+  COV3D_CHECK_ARG(plan, "NULL plan pointer");
+  *plan = new cov3d::SlicePlan();
+  return 0;
+}
+
+extern "C" int cov3d_slice_plan_setpts(void* plan, const void* grid, int64_t npts, int64_t D, int64_t H, int64_t W,
+                                       int mode, int dtype, void* stream) {
+  COV3D_CHECK_ARG(plan, "NULL plan");
+  COV3D_CHECK_ARG(D >= 1 && H >= 1 && W >= 1 && npts >= 0, "bad slice shape");
+  COV3D_CHECK_ARG(mode == 0 || mode == 1, "mode must be 0 (bilinear) or 1 (nearest)");
+  COV3D_CHECK_ARG(dtype == 0 || dtype == 1, "dtype must be 0 or 1");
+  COV3D_CHECK_ARG(npts == 0 || grid, "NULL grid");
+  auto* pl = static_cast<cov3d::SlicePlan*>(plan);
+  int dev = 0;
+  COV3D_HIP(hipGetDevice(&dev));
+  COV3D_CHECK_ARG(pl->dev < 0 || pl->dev == dev, "a slice plan stays on the device it was first used on");
+  pl->dev = dev;
+  pl->ready = false;
+  pl->dtype = dtype;
+  if (npts == 0) {
+    pl->npts = 0, pl->D = D, pl->H = H, pl->W = W, pl->mode = mode, pl->ready = true;
+    return 0;
+  }
+  hipStream_t st = cov3d::as_stream(stream);
+  if (dtype == 1) return cov3d::slice_plan_setpts<double>(pl, grid, npts, D, H, W, mode, st);
+  return cov3d::slice_plan_setpts<float>(pl, grid, npts, D, H, W, mode, st);
+}
+
+extern "C" int cov3d_slice_plan_forward(void* plan, const void* vol, void* out, int64_t ntrans, void* stream) {
+  COV3D_CHECK_ARG(plan, "NULL plan");
+  auto* pl = static_cast<cov3d::SlicePlan*>(plan);
+  COV3D_CHECK_ARG(pl->ready, "slice plan: setpts first");
+  COV3D_CHECK_ARG(ntrans >= 1 && ntrans < 65536, "bad ntrans");
+  if (pl->npts == 0) return 0;
+  COV3D_CHECK_ARG(vol && out, "NULL pointer");
+  hipStream_t st = cov3d::as_stream(stream);
+  if (pl->dtype == 1) return cov3d::slice_plan_forward<double>(pl, vol, out, ntrans, st);
+  return cov3d::slice_plan_forward<float>(pl, vol, out, ntrans, st);
+}
+
+extern "C" int cov3d_slice_plan_adjoint(void* plan, const void* sig, void* vol_out, int64_t ntrans, void* stream) {
+  COV3D_CHECK_ARG(plan, "NULL plan");
+  auto* pl = static_cast<cov3d::SlicePlan*>(plan);
+  COV3D_CHECK_ARG(pl->ready, "slice plan: setpts first");
+  COV3D_CHECK_ARG(ntrans >= 1 && ntrans < 65536, "bad ntrans");
+  COV3D_CHECK_ARG(vol_out && (sig || pl->npts == 0), "NULL pointer");
+  hipStream_t st = cov3d::as_stream(stream);
+  if (pl->dtype == 1) return cov3d::slice_plan_adjoint<double>(pl, sig, vol_out, ntrans, st);
+  return cov3d::slice_plan_adjoint<float>(pl, sig, vol_out, ntrans, st);
+}
+
+extern "C" int cov3d_slice_plan_destroy(void* plan) {
+  if (!plan) return 0;
+  auto* pl = static_cast<cov3d::SlicePlan*>(plan);
+  cov3d::AdjWork& w = pl->w;
+  hipFree(w.keys), hipFree(w.skeys), hipFree(w.sidx), hipFree(w.spos), hipFree(w.start), hipFree(w.end);
+  hipFree(w.tmp), hipFree(w.rows), hipFree(pl->sp);
+  delete pl;
+  return 0;
 }

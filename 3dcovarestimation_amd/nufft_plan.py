@@ -71,9 +71,11 @@ class NufftPlanDiscretized(BaseNufftPlan):
     """Discretised projection: interpolation in the (us*L)^3 centred-FFT volume.
 
     Same constructor, ``setpts`` scaling and channel semantics as
-    nufft_plan.py:84-242.  The interpolation itself runs in
-    ``cov3d_slice_forward`` (trilinear for ``mode="bilinear"``, rounding for
-    ``mode="nearest"``; align_corners=True, zero padding outside the volume).
+    nufft_plan.py:84-242.  The interpolation itself runs in a native slice plan
+    (``cov3d_slice_plan_*``: trilinear for ``mode="bilinear"``, rounding for
+    ``mode="nearest"``; align_corners=True, zero padding outside the volume),
+    whose points are sorted by grid cell on the first forward or aggregated
+    adjoint after ``setpts`` and reused by both.
     """
 
     def __init__(
@@ -86,6 +88,34 @@ class NufftPlanDiscretized(BaseNufftPlan):
         self.upsample_factor = upsample_factor
         self.mode = mode
         self.use_half_grid = use_half_grid
+        self._handle = None
+        self._synced = None
+
+    def __del__(self):
+        h = getattr(self, "_handle", None)
+        if h is not None and _native is not None:
+            try:
+                _native.load().cov3d_slice_plan_destroy(h)
+            except Exception:
+                pass
+            self._handle = None
+
+    def _slice_plan(self, D: int, H: int, W: int, dtype: torch.dtype, device: torch.device):
+        """The native plan with this plan's points sorted for (D, H, W) volumes of ``dtype``."""
+        key = (D, H, W, _dtype_code(dtype), device)
+        lib = _native.load()
+        if self._handle is None:
+            h = ctypes.c_void_p()
+            _native.check(lib.cov3d_slice_plan_create(ctypes.byref(h)))
+            self._handle = h
+        if self._synced != key:
+            grid = self._grid_flat.detach().to(device=device, dtype=_real_of(dtype)).contiguous()
+            with torch.cuda.device(device):
+                _native.check(lib.cov3d_slice_plan_setpts(
+                    self._handle, grid.data_ptr(), grid.shape[0], D, H, W, 0 if self.mode == "bilinear" else 1,
+                    _dtype_code(dtype), _native.stream_ptr(device)))
+            self._synced = key
+        return self._handle
 
     def setpts(self, points: torch.Tensor) -> None:
         """points: (n, 3, Lp**2) zyx frequencies in radians (nufft_plan.py:94-118)."""
@@ -100,6 +130,7 @@ class NufftPlanDiscretized(BaseNufftPlan):
         self.points = grid
         self.batch_points = grid.shape[1]
         self._grid_flat = grid.reshape(-1, 3).contiguous()
+        self._synced = None
 
     def _as_complex_volume(self, volume) -> torch.Tensor:
         if isinstance(volume, tuple):
@@ -116,17 +147,12 @@ class NufftPlanDiscretized(BaseNufftPlan):
             vol = vol.unsqueeze(0)
         _native.require_device(vol, "volume")
         vol = _native.dense(vol)
-        grid = self._grid_flat.to(device=vol.device, dtype=_real_of(vol.dtype))
         n, D, H, W = vol.shape
-        npts = grid.shape[0]
+        npts = self._grid_flat.shape[0]
         out = torch.empty((n, npts), dtype=vol.dtype, device=vol.device)
-        lib = _native.load()
-        _native.check(
-            lib.cov3d_slice_forward(
-                vol.data_ptr(), grid.data_ptr(), out.data_ptr(), n, D, H, W, npts,
-                0 if self.mode == "bilinear" else 1, _dtype_code(vol.dtype), _native.stream_ptr(vol.device),
-            )
-        )
+        h = self._slice_plan(D, H, W, vol.dtype, vol.device)
+        _native.check(_native.load().cov3d_slice_plan_forward(h, vol.data_ptr(), out.data_ptr(), n,
+                                                              _native.stream_ptr(vol.device)))
         L = self.points_L
         return out.reshape(n, self.batch_points, L, L)
 
@@ -136,8 +162,7 @@ class NufftPlanDiscretized(BaseNufftPlan):
         sig = signal if signal.is_complex() else signal.to(_complex_of(signal.dtype))
         _native.require_device(sig, "signal")
         sig = _native.dense(sig)
-        grid = self._grid_flat.to(device=sig.device, dtype=_real_of(sig.dtype))
-        npts = grid.shape[0]
+        npts = self._grid_flat.shape[0]
         lib = _native.load()
         if aggregated:
             if sig.numel() % npts != 0:
@@ -145,12 +170,16 @@ class NufftPlanDiscretized(BaseNufftPlan):
             ntrans = sig.numel() // npts
             # the gather adjoint writes every cell
             out = torch.empty((ntrans, vL, vL, vL), dtype=sig.dtype, device=sig.device)
-            ppv = npts
+            h = self._slice_plan(vL, vL, vL, sig.dtype, sig.device)
+            _native.check(lib.cov3d_slice_plan_adjoint(h, sig.data_ptr(), out.data_ptr(), ntrans,
+                                                       _native.stream_ptr(sig.device)))
+            return out
         else:
             nimg = sig.shape[0]
             ntrans = 1
             out = torch.zeros((nimg, vL, vL, vL), dtype=sig.dtype, device=sig.device)
             ppv = npts // nimg
+        grid = self._grid_flat.to(device=sig.device, dtype=_real_of(sig.dtype))
         _native.check(
             lib.cov3d_slice_adjoint(
                 sig.data_ptr(), grid.data_ptr(), out.data_ptr(), ntrans, vL, vL, vL, npts,

@@ -124,6 +124,21 @@ int cov3d_slice_adjoint(const void* sig, const void* grid, void* vol_out,
                         int64_t ntrans, int64_t D, int64_t H, int64_t W,
                         int64_t npts, int mode, int aggregated,
                         int64_t pts_per_vol, int dtype, void* stream);
+/* Slice plan: the points of NufftPlanDiscretized.setpts (nufft_plan.py:94-118) sorted once
+ * by the grid cell they reach, shared by the forward (nufft_plan.py:120-169) and the aggregated
+ * adjoint (:171-200).  setpts takes the same (npts, 3) grid as cov3d_slice_forward and keeps no
+ * pointer to it; forward / adjoint then take the volumes and values only and give the results of
+ * cov3d_slice_forward / cov3d_slice_adjoint (aggregated) bit for bit.  A plan stays on the device
+ * of its first setpts; it is not thread-safe. */
+int cov3d_slice_plan_create(void** plan);
+int cov3d_slice_plan_setpts(void* plan, const void* grid, int64_t npts, int64_t D,
+                            int64_t H, int64_t W, int mode, int dtype, void* stream);
+int cov3d_slice_plan_forward(void* plan, const void* vol, void* out, int64_t ntrans,
+                             void* stream);
+int cov3d_slice_plan_adjoint(void* plan, const void* sig, void* vol_out, int64_t ntrans,
+                             void* stream);
+int cov3d_slice_plan_destroy(void* plan);
+
 /* Gradient of cov3d_slice_forward with respect to its grid: for each point j,
  *   grid_grad[j, a] = sum_t Re(conj(grad_out[t, j]) * d out[t, j] / d grid[j, a])
  * (a = x, y, z), the grid derivative torch's grid_sample returns when the

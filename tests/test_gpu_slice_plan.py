@@ -1,0 +1,78 @@
+"""The slice plan (cov3d_slice_plan_*: points sorted once per setpts, sorted-order forward, shared
+sort for the gather adjoint) against the stateless kernels (cov3d_slice_forward / _adjoint), bit
+for bit, in both modes and precisions, with points outside the volume, and after a second setpts
+on the same plan (nufft_plan.py:94-200)."""
+
+import ctypes
+
+import pytest
+import torch
+
+from tests.conftest import pkg
+
+pytestmark = pytest.mark.gpu
+
+
+def _stateless(nat, vol, grid, mode, sig=None):
+    n, D, H, W = vol.shape
+    npts = grid.shape[0]
+    st = nat.stream_ptr(vol.device)
+    code = 1 if vol.dtype == torch.complex128 else 0
+    out = torch.empty((n, npts), dtype=vol.dtype, device=vol.device)
+    nat.check(nat.load().cov3d_slice_forward(vol.data_ptr(), grid.data_ptr(), out.data_ptr(), n, D, H, W, npts,
+                                             mode, code, st))
+    back = torch.empty_like(vol)
+    nat.check(nat.load().cov3d_slice_adjoint(sig.data_ptr(), grid.data_ptr(), back.data_ptr(), n, D, H, W, npts,
+                                             mode, 1, npts, code, st))
+    return out, back
+
+
+@pytest.mark.parametrize("dtype", [torch.complex64, torch.complex128])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_slice_plan_matches_stateless_kernels(dev, dtype, mode):
+    nat = pkg("_native")
+    rdt = torch.float64 if dtype == torch.complex128 else torch.float32
+    g = torch.Generator(device=dev).manual_seed(3 + mode)
+    D, n = 24, 3
+    vol = torch.randn((n, D, D, D), dtype=dtype, device=dev, generator=g)
+    lib = nat.load()
+    h = ctypes.c_void_p()
+    nat.check(lib.cov3d_slice_plan_create(ctypes.byref(h)))
+    try:
+        for npts in (5000, 777):  # a second setpts on the same plan, fewer points
+            grid = ((torch.rand((npts, 3), dtype=rdt, device=dev, generator=g) * 2 - 1) * 1.1).contiguous()
+            sig = torch.randn((n, npts), dtype=dtype, device=dev, generator=g)
+            ref_f, ref_b = _stateless(nat, vol, grid, mode, sig)
+            st = nat.stream_ptr(dev)
+            nat.check(lib.cov3d_slice_plan_setpts(h, grid.data_ptr(), npts, D, D, D, mode,
+                                                  1 if dtype == torch.complex128 else 0, st))
+            out = torch.empty_like(ref_f)
+            nat.check(lib.cov3d_slice_plan_forward(h, vol.data_ptr(), out.data_ptr(), n, st))
+            back = torch.empty_like(ref_b)
+            nat.check(lib.cov3d_slice_plan_adjoint(h, sig.data_ptr(), back.data_ptr(), n, st))
+            torch.cuda.synchronize()
+            assert torch.equal(out, ref_f)
+            assert torch.equal(back, ref_b)
+    finally:
+        lib.cov3d_slice_plan_destroy(h)
+
+
+def test_discretized_plan_resyncs_on_setpts_and_volume_size(dev):
+    """NufftPlanDiscretized re-sorts after setpts and for a volume of another size."""
+    npl = pkg("nufft_plan")
+    ds = pkg("dataset")
+    L = 16
+    plan = npl.NufftPlanDiscretized((L,) * 3, upsample_factor=2, mode="bilinear")
+    g = torch.Generator().manual_seed(0)
+    outs = []
+    for seed in (1, 2):
+        rots = ds.haar_rotations(6, torch.Generator().manual_seed(seed)).to(dev)
+        plan.setpts(ds.pts_rot_from_rotations(rots, L))
+        for vL in (2 * L, 2 * L + 2):
+            vol = torch.randn((2, vL, vL, vL), generator=g, dtype=torch.complex64).to(dev)
+            a = plan.execute_forward(vol)
+            fresh = npl.NufftPlanDiscretized((L,) * 3, upsample_factor=2, mode="bilinear")
+            fresh.setpts(ds.pts_rot_from_rotations(rots, L))
+            assert torch.equal(a, fresh.execute_forward(vol))
+            outs.append(a)
+    assert not torch.equal(outs[0], outs[2])
