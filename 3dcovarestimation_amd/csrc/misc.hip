@@ -122,6 +122,71 @@ __global__ void __launch_bounds__(kMiscBlock)
   }
 }
 
+// One FP64 atomic pair per block lands on the same two addresses, so the reductions use at most
+// one block per CU: 2048 blocks (4096 contended atomics) took 54 us for 2 M elements, the loads
+// alone take a few.
+__host__ __device__ inline int64_t dot_blocks(int64_t n) {
+  const int64_t nb = (n + kMiscBlock - 1) / kMiscBlock;
+  return nb < 256 ? (nb < 1 ? 1 : nb) : 256;
+}
+
+// Block sum of (a, b) added to out[0], out[1].
+__device__ __forceinline__ void block_add2(double a, double b, double* out) {
+  __shared__ double sa[kMiscBlock], sb[kMiscBlock];
+  sa[threadIdx.x] = a;
+  sb[threadIdx.x] = b;
+  __syncthreads();
+  for (int s = kMiscBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      sa[threadIdx.x] += sa[threadIdx.x + s];
+      sb[threadIdx.x] += sb[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    atomicAdd(out, sa[0]);
+    atomicAdd(out + 1, sb[0]);
+  }
+}
+
+// CG step x += alpha p, r -= alpha Ap with alpha = rz / pAp (complex, from the FP64 device
+// scalars: rz real, pAp = vdot(p, Ap)), rounded to complex64 as the reference's complex64
+// alpha; rr += |r|^2 (FP64) of the updated residual.
+__global__ void __launch_bounds__(kMiscBlock)
+    k_cg_update(float2* __restrict__ x, float2* __restrict__ r, const float2* __restrict__ p,
+                const float2* __restrict__ ap, int64_t n, const double* __restrict__ rz,
+                const double* __restrict__ pap, double* __restrict__ rr) {
+  const double den = pap[0] * pap[0] + pap[1] * pap[1];
+  const float ar = (float)(rz[0] * pap[0] / den), ai = (float)(-rz[0] * pap[1] / den);
+  double acc = 0;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+    const float2 pk = p[k], qk = ap[k];
+    float2 xk = x[k], rk = r[k];
+    const float2 tp = make_float2(ar * pk.x - ai * pk.y, ar * pk.y + ai * pk.x);
+    const float2 tq = make_float2(ar * qk.x - ai * qk.y, ar * qk.y + ai * qk.x);
+    xk.x += tp.x;
+    xk.y += tp.y;
+    rk.x -= tq.x;
+    rk.y -= tq.y;
+    x[k] = xk;
+    r[k] = rk;
+    acc += (double)rk.x * rk.x + (double)rk.y * rk.y;
+  }
+  block_add2(acc, 0.0, rr);
+}
+
+// CG direction p = beta p + z with beta = rz_new / rz_old (real parts of FP64 device scalars),
+// rounded to complex64 as the reference's complex64 beta.
+__global__ void __launch_bounds__(kMiscBlock)
+    k_cg_direction(float2* __restrict__ p, const float2* __restrict__ z, int64_t n, const double* __restrict__ rz_new,
+                   const double* __restrict__ rz_old) {
+  const float beta = (float)(rz_new[0] / rz_old[0]);
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const float2 pk = p[k], zk = z[k];
+  p[k] = make_float2(pk.x * beta + zk.x, pk.y * beta + zk.y);
+}
+
 // ----------------------------------------------------------------------------
 // MLP (n_in -> 64 -> 64 -> 64 -> 1, GELU tanh form), one thread per row.
 // ----------------------------------------------------------------------------
@@ -313,9 +378,7 @@ extern "C" int cov3d_vdot_c64_f64(const void* x, const void* y, int64_t n, void*
   COV3D_LAUNCH_CHECK();
   if (n == 0) return 0;
   COV3D_CHECK_ARG(x && y, "NULL input");
-  int64_t nb = cov3d::ceil_div(n, cov3d::kMiscBlock);
-  if (nb > 2048) nb = 2048;
-  hipLaunchKernelGGL(cov3d::k_vdot, dim3((unsigned)nb), dim3(cov3d::kMiscBlock), 0, st,
+  hipLaunchKernelGGL(cov3d::k_vdot, dim3((unsigned)cov3d::dot_blocks(n)), dim3(cov3d::kMiscBlock), 0, st,
                      (const float2*)x, (const float2*)y, n, (double*)out);
   COV3D_LAUNCH_CHECK();
   return 0;
@@ -352,6 +415,30 @@ extern "C" int cov3d_mlp_backward(const void* params, const void* x, const void*
                      dim3(cov3d::kMiscBlock), smem, cov3d::as_stream(stream),
                      (const float*)params, (const float*)x, (const float*)gy, (float*)grad_params,
                      n, (int)n_in);
+  COV3D_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int cov3d_cg_update(void* x, void* r, const void* p, const void* ap, int64_t n, const void* rz,
+                               const void* pap, void* rr, void* stream) {
+  COV3D_CHECK_ARG(n >= 1, "bad length");
+  COV3D_CHECK_ARG(x && r && p && ap && rz && pap && rr, "NULL pointer");
+  hipStream_t st = cov3d::as_stream(stream);
+  hipLaunchKernelGGL(cov3d::k_zero2, dim3(1), dim3(1), 0, st, (double*)rr);
+  hipLaunchKernelGGL(cov3d::k_cg_update, dim3((unsigned)cov3d::dot_blocks(n)), dim3(cov3d::kMiscBlock), 0, st,
+                     (float2*)x, (float2*)r, (const float2*)p, (const float2*)ap, n, (const double*)rz,
+                     (const double*)pap, (double*)rr);
+  COV3D_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int cov3d_cg_direction(void* p, const void* z, int64_t n, const void* rz_new, const void* rz_old,
+                                  void* stream) {
+  COV3D_CHECK_ARG(n >= 1, "bad length");
+  COV3D_CHECK_ARG(p && z && rz_new && rz_old, "NULL pointer");
+  hipLaunchKernelGGL(cov3d::k_cg_direction, dim3((unsigned)cov3d::ceil_div(n, cov3d::kMiscBlock)),
+                     dim3(cov3d::kMiscBlock), 0, cov3d::as_stream(stream), (float2*)p, (const float2*)z, n,
+                     (const double*)rz_new, (const double*)rz_old);
   COV3D_LAUNCH_CHECK();
   return 0;
 }

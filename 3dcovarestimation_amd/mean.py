@@ -81,58 +81,79 @@ def complex_conjugate_gradient(A_mv: Callable, b: torch.Tensor, x0: Optional[tor
     Stops when the recursively updated residual satisfies ||r|| < rtol ||b||, as the reference
     does; the true residual ||b - A x|| / ||b|| is recomputed at the end
     (``CGInfo.final_residual``).  alpha and beta are FP64 device scalars and stay on the device:
-    the residual norms are read back every ``check_every`` iterations, so the host never waits for
-    the device inside a block and the solve may run up to ``check_every - 1`` iterations past the
-    first one that meets the tolerance (``CGInfo.residuals`` records every iteration).  If
-    ``precond`` stagnates (the best residual improves by less than 10% over ``stall_window``
-    iterations) plain CG resumes from the current iterate.
+    each iteration is the operator, one FP64 dot product (pAp), ``cov3d_cg_update`` (x, r and
+    ||r||^2 in one pass) and ``cov3d_cg_direction`` (plus the preconditioner and one more dot with
+    ``precond``).  The residual norms of a block of ``check_every`` iterations are read back at its
+    end, so the host never waits for the device inside a block and the solve may run up to
+    ``check_every - 1`` iterations past the first one that meets the tolerance
+    (``CGInfo.residuals`` records every iteration).  If ``precond`` stagnates (the best residual
+    improves by less than 10% over ``stall_window`` iterations) plain CG resumes from the current
+    iterate.
     """
     if not b.is_complex():
         raise ValueError("Input vector b must be a complex tensor.")
+    _native.require_device(b, "b")
     info = CGInfo(preconditioned=precond is not None)
-    x = torch.zeros_like(b) if x0 is None else x0.clone()
-    maxiter = b.numel() * 10 if maxiter is None else maxiter
+    dev = b.device
+    lib = _native.load()
+    st = _native.stream_ptr(dev)
+    b = _native.dense(b.reshape(-1).to(torch.complex64))
+    n = b.numel()
+    x = torch.zeros_like(b) if x0 is None else _native.dense(x0.reshape(-1).to(torch.complex64)).clone()
+    maxiter = n * 10 if maxiter is None else maxiter
+    # FP64 device scalars: [rz0 (2) | pAp (2) | rr of the block's iterations (2 each) | rz of them (2 each)]
+    S = torch.zeros(4 + 4 * check_every, dtype=torch.float64, device=dev)
+    base = S.data_ptr()
+    RZ0, PAP = base, base + 16
+    rr_slot = lambda k: base + 32 + 16 * k
+    rz_slot = lambda k: base + 32 + 16 * check_every + 16 * k
+
+    def dot_into(u, v, ptr):
+        _native.check(lib.cov3d_vdot_c64_f64(u.data_ptr(), v.data_ptr(), n, ptr, st))
+
     b_norm = math.sqrt(vdot_f64(b, b).real.item())
-    r = b - A_mv(x)
-    z = precond(r) if precond is not None else r
+    r = _native.dense(b - A_mv(x).reshape(-1))
+    z = _native.dense(precond(r).reshape(-1)) if precond is not None else r
     p = z.clone()
-    rz = vdot_f64(r, z).real
+    dot_into(r, z, RZ0)
+    rz = RZ0
     info.residuals.append(math.sqrt(vdot_f64(r, r).real.item()) / b_norm)
     best = [info.residuals[0]]  # best residual so far, per iteration
     done = info.residuals[0] < rtol
-    pending = []
+    k = 0  # iterations of the current block
     it = 0
     while not done and it < maxiter:
-        Ap = A_mv(p)
-        alpha = (rz / vdot_f64(p, Ap)).to(torch.complex64)
-        x.add_(alpha * p)
-        r.sub_(alpha * Ap)
-        rr = vdot_f64(r, r).real
-        pending.append(rr)
-        it += 1
+        Ap = _native.dense(A_mv(p).reshape(-1))
+        dot_into(p, Ap, PAP)
+        _native.check(lib.cov3d_cg_update(x.data_ptr(), r.data_ptr(), p.data_ptr(), Ap.data_ptr(), n, rz, PAP,
+                                          rr_slot(k), st))
         if precond is not None:
-            z = precond(r)
-            rz_new = vdot_f64(r, z).real
+            z = _native.dense(precond(r).reshape(-1))
+            dot_into(r, z, rz_slot(k))
+            rz_new = rz_slot(k)
         else:
             z = r
-            rz_new = rr
-        p.mul_((rz_new / rz).to(torch.complex64)).add_(z)
+            rz_new = rr_slot(k)
+        _native.check(lib.cov3d_cg_direction(p.data_ptr(), z.data_ptr(), n, rz_new, rz, st))
         rz = rz_new
-        if len(pending) == check_every or it >= maxiter:
-            for v in torch.stack(pending).cpu().tolist():
+        k += 1
+        it += 1
+        if k == check_every or it >= maxiter:
+            for v in S[4:4 + 2 * k:2].cpu().tolist():
                 info.residuals.append(math.sqrt(max(v, 0.0)) / b_norm)
                 best.append(min(best[-1], info.residuals[-1]))
                 done = done or info.residuals[-1] < rtol
-            pending = []
+            k = 0
             if not done and precond is not None and it > stall_window and best[-1] > 0.9 * best[-1 - stall_window]:
                 info.fallback = "stall"
                 precond = None
                 z = r
                 p = r.clone()
-                rz = vdot_f64(r, r).real
+                dot_into(r, r, RZ0)
+                rz = RZ0
     info.iterations_run = it
     info.iterations = next((i for i, v in enumerate(info.residuals) if v < rtol), it)
-    res = b - A_mv(x)
+    res = b - A_mv(x).reshape(-1)
     info.final_residual = math.sqrt(vdot_f64(res, res).real.item()) / b_norm
     logger.debug(f"CG converged in {info.iterations} iterations")
     return (x, info) if return_info else x

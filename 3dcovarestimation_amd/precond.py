@@ -167,13 +167,18 @@ class DiagonalMLP:
 
 
 def preconditioner_from_log_diag(logd: torch.Tensor, L: int) -> Optional[Callable]:
+    """r -> ifft3c(w * fft3c(r)) with w = exp(-logd) on the centred L^3 frequency grid.
+
+    With S the half-length roll, fft3c = S F S, and S F^-1 S (w o S F S r) = F^-1 ((S w) o F r)
+    (the shift theorem's phases (-1)^k cancel in pairs), so the operator runs as two plain FFTs
+    and one multiply by the pre-rolled real weights."""
     w = torch.exp(-logd).reshape(L, L, L)
     if not bool(torch.isfinite(w).all()) or bool((w <= 0).any()):
         return None
-    w = w.to(torch.complex64)
+    ws = torch.fft.ifftshift(w).to(torch.float32).contiguous()
 
     def apply(r: torch.Tensor) -> torch.Tensor:
-        return centered_ifft3(w * centered_fft3(r.reshape(L, L, L))).reshape(-1)
+        return torch.fft.ifftn(ws * torch.fft.fftn(r.reshape(L, L, L).to(torch.complex64))).reshape(-1)
 
     return apply
 

@@ -241,6 +241,19 @@ int cov3d_ctf_eval(const void* params, void* out, int64_t B, int64_t L,
 int cov3d_vdot_c64_f64(const void* x, const void* y, int64_t n, void* out,
                        void* stream);
 
+/* The vector updates of one CG iteration (complex_conjugate_gradient, mean.py:64-80), with
+ * the scalars in FP64 device memory so that the host never waits inside the loop:
+ *   cov3d_cg_update:    alpha = rz / pAp (rz: a double, its real part; pAp: (re, im) doubles),
+ *                       rounded to complex64; x += alpha p; r -= alpha Ap; rr (2 doubles) is
+ *                       overwritten with (||r||^2 of the updated r, 0), accumulated in FP64.
+ *   cov3d_cg_direction: p = beta p + z with beta = rz_new / rz_old (real parts), rounded to
+ *                       complex64.
+ * x, r, p, Ap, z: complex64 of length n (n >= 1). */
+int cov3d_cg_update(void* x, void* r, const void* p, const void* ap, int64_t n,
+                    const void* rz, const void* pap, void* rr, void* stream);
+int cov3d_cg_direction(void* p, const void* z, int64_t n, const void* rz_new,
+                       const void* rz_old, void* stream);
+
 /* Toeplitz normal operator of the mean CG (mean.py:118-123) in one call:
  *   y = crop_L(ifft3c(lhs * fft3c(pad_{nf}(x))))  for x (L^3) complex64,
  *   lhs (nf^3) complex64, nf >= L (nf = us * L in the mean solve).  For

@@ -275,3 +275,28 @@ def test_config2_shifted_batch_gradient(dev):
             an = float((v.grad.double() * d).sum())
             print(f"directional derivative: analytic {an:.6e}, finite difference {fd:.6e}")
             assert abs(an - fd) < 2e-3 * abs(fd) + 1e-9
+
+
+def test_cg_vector_kernels_match_torch(dev):
+    """cov3d_cg_update / cov3d_cg_direction (the fused CG vector steps) against the reference's
+    complex64 torch updates (mean.py:64-80) with the same FP64 scalars."""
+    nat = pkg("_native")
+    lib = nat.load()
+    n = 100003
+    g = torch.Generator(device=dev).manual_seed(4)
+    cx = lambda: torch.complex(torch.randn(n, device=dev, generator=g), torch.randn(n, device=dev, generator=g))
+    x, r, p, ap, z = cx(), cx(), cx(), cx(), cx()
+    S = torch.tensor([3.7, 0.0, 2.5, 0.01, 0.0, 0.0, 1.9, 0.0], dtype=torch.float64, device=dev)
+    rz, pap, rr, rz_new = (S.data_ptr() + 16 * k for k in range(4))
+    alpha = (torch.complex(S[0], S[1]) / torch.complex(S[2], S[3])).to(torch.complex64)
+    x_ref = x + alpha * p
+    r_ref = r - alpha * ap
+    st = nat.stream_ptr(dev)
+    nat.check(lib.cov3d_cg_update(x.data_ptr(), r.data_ptr(), p.data_ptr(), ap.data_ptr(), n, rz, pap, rr, st))
+    assert (x - x_ref).abs().max() < 1e-5 and (r - r_ref).abs().max() < 1e-5
+    rr_ref = (r.to(torch.complex128).abs() ** 2).sum()
+    assert abs(float(S[4]) - float(rr_ref)) < 1e-9 * float(rr_ref) and float(S[5]) == 0.0
+    beta = (torch.complex(S[6], S[7]) / torch.complex(S[0], S[1])).to(torch.complex64)
+    p_ref = p * beta + z
+    nat.check(lib.cov3d_cg_direction(p.data_ptr(), z.data_ptr(), n, rz_new, rz, st))
+    assert (p - p_ref).abs().max() < 1e-5

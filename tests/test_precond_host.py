@@ -9,6 +9,7 @@
 """
 
 import numpy as np
+import pytest
 import torch
 
 from tests.conftest import oracle, pkg
@@ -60,3 +61,18 @@ def test_invalid_weights_fall_back_and_jacobi_helps():
     x, it_pcg, _ = o.conjugate_gradient(A, b, rtol=1e-8, M=_M())
     assert np.linalg.norm(A(x) - b) / np.linalg.norm(b) < 1e-7
     assert it_pcg < it_plain
+
+
+@pytest.mark.parametrize("L", [7, 8])
+def test_preconditioner_equals_centred_fft_form(L):
+    """The preconditioner's two plain FFTs with rolled weights equal ifft3c(w * fft3c(r))
+    (odd and even L)."""
+    pc = pkg("precond")
+    pf = pkg("projection_funcs")
+    g = torch.Generator().manual_seed(L)
+    logd = torch.randn(L**3, generator=g)
+    apply = pc.preconditioner_from_log_diag(logd, L)
+    r = torch.complex(torch.randn(L**3, generator=g), torch.randn(L**3, generator=g))
+    w = torch.exp(-logd).reshape(L, L, L).to(torch.complex64)
+    ref = pf.centered_ifft3(w * pf.centered_fft3(r.reshape(L, L, L))).reshape(-1)
+    assert (apply(r) - ref).abs().max() < 1e-5 * ref.abs().max()
