@@ -41,6 +41,8 @@ _SIGNATURES = {
     "cov3d_slice_plan_forward": [c_vp, c_vp, c_vp, c_i64, c_vp],
     "cov3d_slice_plan_adjoint": [c_vp, c_vp, c_vp, c_i64, c_vp],
     "cov3d_slice_plan_destroy": [c_vp],
+    "cov3d_ml_grad_images_complex": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_i64, c_i64,
+                                     c_i64, c_dbl, c_dbl, c_vp],
     "cov3d_cg_update": [c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp],
     "cov3d_cg_direction": [c_vp, c_vp, c_i64, c_vp, c_vp, c_vp],
     "cov3d_ml_forward": [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_i64, c_dbl, c_int, c_int, c_vp, c_vp,

@@ -387,6 +387,71 @@ __global__ void __launch_bounds__(kMlBlock)
   }
 }
 
+// Complex-coefficient gradient images of the Fourier-domain ML cost (complex M, z): with
+// conj(M^-1) = R - iJ and z = a + ib the gradient splits into two real-coefficient passes
+// (coefficients (A1, z1), (A2, z2) as the host forms them), g = g1 + i g2; both run here on one
+// load of the pixel's r values, and g is written once.
+template <int RM>
+__global__ void __launch_bounds__(kMlBlock)
+    k_ml_grad_cplx(const float2* __restrict__ c, const float* __restrict__ ctf,
+                   const float2* __restrict__ yhat, const float* __restrict__ z1,
+                   const float* __restrict__ m1, const float* __restrict__ z2, const float* __restrict__ m2,
+                   const int* __restrict__ pix_map, int64_t np, float2* __restrict__ g, int64_t B, int r, int L,
+                   float scale) {
+  __shared__ float s_m1[RM * RM], s_m2[RM * RM];
+  __shared__ float s_z1[RM], s_z2[RM];
+  const int64_t i = blockIdx.y;
+  const int64_t d = (int64_t)L * L;
+  for (int e = threadIdx.x; e < RM * RM; e += blockDim.x) {
+    int a = e / RM, b = e % RM;
+    const bool in = a < r && b < r;
+    s_m1[e] = in ? m1[(i * r + a) * r + b] : 0.f;
+    s_m2[e] = in ? m2[(i * r + a) * r + b] : 0.f;
+  }
+  if (threadIdx.x < RM) {
+    s_z1[threadIdx.x] = (int)threadIdx.x < r ? z1[i * r + threadIdx.x] : 0.f;
+    s_z2[threadIdx.x] = (int)threadIdx.x < r ? z2[i * r + threadIdx.x] : 0.f;
+  }
+  __syncthreads();
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= np) return;
+  float mult;
+  const int64_t px = entry_pixel(pix_map, k, L, mult);
+  const float inv_l = 1.f / (float)L;
+  const float ps = pixel_scale(ctf, i, px, L, inv_l * inv_l);
+  float qx[RM], qy[RM];
+  float e1x = 0.f, e1y = 0.f, e2x = 0.f, e2y = 0.f;
+#pragma unroll
+  for (int s = 0; s < RM; ++s) {
+    float2 v = s < r ? c[((int64_t)s * B + i) * np + k] : make_float2(0.f, 0.f);
+    qx[s] = v.x * ps;
+    qy[s] = v.y * ps;
+    e1x += s_z1[s] * qx[s];
+    e1y += s_z1[s] * qy[s];
+    e2x += s_z2[s] * qx[s];
+    e2y += s_z2[s] * qy[s];
+  }
+  const float2 yv = yhat[i * d + px];
+  e1x -= yv.x * inv_l;
+  e1y -= yv.y * inv_l;
+  e2x -= yv.x * inv_l;
+  e2y -= yv.y * inv_l;
+  const float gs = ps * scale * (float)(L * L) * mult;
+  for (int t = 0; t < r; ++t) {
+    float ax = s_z1[t] * e1x, ay = s_z1[t] * e1y;
+    float bx = s_z2[t] * e2x, by = s_z2[t] * e2y;
+#pragma unroll
+    for (int s = 0; s < RM; ++s) {
+      const float u = s_m1[t * RM + s], w = s_m2[t * RM + s];
+      ax += u * qx[s];
+      ay += u * qy[s];
+      bx += w * qx[s];
+      by += w * qy[s];
+    }
+    g[((int64_t)t * B + i) * np + k] = make_float2((ax - by) * gs, (ay + bx) * gs);
+  }
+}
+
 // Gradient images written straight into the sorted rows a type-1 NUFFT plan spreads from:
 // value t of pixel k of image i goes to gs[spos[i * np + k] * ldc + t] (t < r; zeros up to ldc).
 // Each workgroup stages its 256 pixels' rows in LDS and writes them as consecutive 16-byte pieces.
@@ -574,6 +639,47 @@ extern "C" int cov3d_ml_grad_sorted(const void* c, const void* ctf, const void* 
   else
     COV3D_GRADS(24);
 #undef COV3D_GRADS
+  COV3D_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int cov3d_ml_grad_images_complex(const void* c, const void* ctf, const void* yhat, const void* z1,
+                                            const void* minv1, const void* z2, const void* minv2,
+                                            const int* pix_map, int64_t npix, void* g, int64_t B, int64_t r,
+                                            int64_t L, double noise_var, double grad_scale, void* stream) {
+  COV3D_CHECK_ARG(r >= 1 && r <= 31, "rank r=%lld must be in [1, 31]", (long long)r);
+  COV3D_CHECK_ARG(B >= 0 && L >= 1, "bad batch/size");
+  COV3D_CHECK_ARG(noise_var > 0, "noise_var must be positive");
+  if (B == 0) return 0;
+  COV3D_CHECK_ARG(c && yhat && z1 && minv1 && z2 && minv2 && g, "NULL pointer");
+  COV3D_CHECK_ARG(g != c, "g must not alias c");
+  COV3D_CHECK_ARG(B < 65536, "batch too large for one launch (B=%lld)", (long long)B);
+  COV3D_CHECK_ARG(pix_map ? (npix >= 1 && npix <= L * L) : (npix == 0 || npix == L * L),
+                  "npix=%lld invalid for L=%lld", (long long)npix, (long long)L);
+  hipStream_t st = cov3d::as_stream(stream);
+  const int64_t np = pix_map ? npix : L * L;
+  const float scale = (float)(grad_scale / ((double)(L * L) * noise_var));
+  dim3 grid((unsigned)cov3d::ceil_div(np, cov3d::kMlBlock), (unsigned)B);
+#define COV3D_GRADC(RM)                                                                                 \
+  hipLaunchKernelGGL(cov3d::k_ml_grad_cplx<RM>, grid, dim3(cov3d::kMlBlock), 0, st, (const float2*)c,   \
+                     (const float*)ctf, (const float2*)yhat, (const float*)z1, (const float*)minv1,     \
+                     (const float*)z2, (const float*)minv2, pix_map, np, (float2*)g, B, (int)r, (int)L, \
+                     scale)
+  if (r <= 4)
+    COV3D_GRADC(4);
+  else if (r <= 8)
+    COV3D_GRADC(8);
+  else if (r <= 12)
+    COV3D_GRADC(12);
+  else if (r <= 16)
+    COV3D_GRADC(16);
+  else if (r <= 20)
+    COV3D_GRADC(20);
+  else if (r <= 24)
+    COV3D_GRADC(24);
+  else
+    COV3D_GRADC(32);
+#undef COV3D_GRADC
   COV3D_LAUNCH_CHECK();
   return 0;
 }

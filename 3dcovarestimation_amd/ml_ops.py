@@ -119,10 +119,13 @@ def ml_grad_images(
     g = out if out is not None else torch.empty_like(c)
     ctf_t = ctf.to(torch.float32).contiguous() if ctf is not None else None
     lib = _native.load()
+    # the converted inputs are bound to names: a temporary freed inside the argument list could
+    # hand its memory to the next one before the call
+    yhat, z, minv = _native.dense(yhat), _native.dense(z), _native.dense(minv)
     _native.check(
         lib.cov3d_ml_grad_images(
-            c.data_ptr(), _native.ptr(ctf_t), _native.dense(yhat).data_ptr(), _native.dense(z).data_ptr(),
-            _native.dense(minv).data_ptr(), _native.ptr(pix_map), 0 if pix_map is None else pix_map.numel(),
+            c.data_ptr(), _native.ptr(ctf_t), yhat.data_ptr(), z.data_ptr(),
+            minv.data_ptr(), _native.ptr(pix_map), 0 if pix_map is None else pix_map.numel(),
             g.data_ptr(), B, r, L, float(noise_var), float(grad_scale),
             _native.stream_ptr(c.device),
         )
@@ -148,10 +151,11 @@ def ml_grad_sorted(
     r, B = c.shape[0], c.shape[1]
     ctf_t = ctf.to(torch.float32).contiguous() if ctf is not None else None
     lib = _native.load()
+    yhat, z, minv = _native.dense(yhat), _native.dense(z), _native.dense(minv)
     _native.check(
         lib.cov3d_ml_grad_sorted(
-            c.data_ptr(), _native.ptr(ctf_t), _native.dense(yhat).data_ptr(), _native.dense(z).data_ptr(),
-            _native.dense(minv).data_ptr(), _native.ptr(pix_map), 0 if pix_map is None else pix_map.numel(),
+            c.data_ptr(), _native.ptr(ctf_t), yhat.data_ptr(), z.data_ptr(),
+            minv.data_ptr(), _native.ptr(pix_map), 0 if pix_map is None else pix_map.numel(),
             values, spos, ldc, B, r, L, float(noise_var), float(grad_scale), _native.stream_ptr(c.device),
         )
     )
@@ -269,8 +273,9 @@ class MLCostDiscretized(torch.autograd.Function):
     fused gradient kernel with real coefficients (conj(M^-1) = R - iJ, z = a + ib):
         real part:      (R + b b^T) Q + a (a^T Q - y~)
         imaginary part: (-J + a b^T - b a^T - b b^T) Q + (-b) ((-b)^T Q - y~)
-    and the native exact adjoint of the slice (cov3d_slice_adjoint) scatters it, replacing autograd
-    through grid_sample, solve and logdet.
+    both computed in one pass by ``cov3d_ml_grad_images_complex`` (g = g1 + i g2 written once), and the
+    native exact adjoint of the slice (the slice plan's gather) scatters it, replacing autograd through
+    grid_sample, solve and logdet.
     """
 
     @staticmethod
@@ -330,10 +335,14 @@ class MLCostDiscretized(torch.autograd.Function):
         A1 = (Rm + bz.unsqueeze(2) * bz.unsqueeze(1)).to(torch.float32)
         A2 = (-Jm + a.unsqueeze(2) * bz.unsqueeze(1) - bz.unsqueeze(2) * a.unsqueeze(1)
               - bz.unsqueeze(2) * bz.unsqueeze(1)).to(torch.float32)
-        g1 = ml_grad_images(c, yhat, a.to(torch.float32), A1, L, nv_s, 1.0 / B, ctf=ctf_arg,
-                            out=torch.empty_like(c))
-        g2 = ml_grad_images(c, yhat, (-bz).to(torch.float32), A2, L, nv_s, 1.0 / B, ctf=ctf_arg,
-                            out=torch.empty_like(c))
-        g = g1 + 1j * g2
+        g = torch.empty_like(c)
+        ctf_t = ctf_arg.to(torch.float32).contiguous() if ctf_arg is not None else None
+        # bound to names, so that no converted input is freed before the call
+        yh = _native.dense(yhat)
+        z1, m1 = _native.dense(a.to(torch.float32)), _native.dense(A1)
+        z2, m2 = _native.dense((-bz).to(torch.float32)), _native.dense(A2)
+        _native.check(_native.load().cov3d_ml_grad_images_complex(
+            c.data_ptr(), _native.ptr(ctf_t), yh.data_ptr(), z1.data_ptr(), m1.data_ptr(), z2.data_ptr(),
+            m2.data_ptr(), None, 0, g.data_ptr(), B, r, L, float(nv_s), 1.0 / B, _native.stream_ptr(c.device)))
         grad_v = ctx.plan._adjoint(g.reshape(r, -1), aggregated=True).reshape(shape) * grad_out
         return grad_v, None, None, None, None, None, None

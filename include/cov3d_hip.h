@@ -199,6 +199,17 @@ int cov3d_ml_grad_images(const void* c, const void* ctf, const void* yhat,
                          int64_t npix, void* g, int64_t B, int64_t r, int64_t L,
                          double noise_var, double grad_scale, void* stream);
 
+/* The gradient of the Fourier-domain ML cost (covar_sgd.py:1350-1399 with a complex
+ * solve): with conj(M^-1) = R - iJ and z = a + ib it is g1 + i g2, where g1 and g2 are
+ * cov3d_ml_grad_images with (z1, minv1) = (a, R + b b^T) and (z2, minv2) =
+ * (-b, -J + a b^T - b a^T - b b^T); both run on one load of each pixel's values and g is
+ * written once.  g must not alias c. */
+int cov3d_ml_grad_images_complex(const void* c, const void* ctf, const void* yhat,
+                                 const void* z1, const void* minv1, const void* z2,
+                                 const void* minv2, const int* pix_map, int64_t npix,
+                                 void* g, int64_t B, int64_t r, int64_t L, double noise_var,
+                                 double grad_scale, void* stream);
+
 /* cov3d_ml_grad_images with the output written to a plan's sorted type-1
  * input (cov3d_nufft_sorted_input): value t of pixel k of image i goes to
  * g_sorted[spos[i * np + k] * ldc + t], zeros for r <= t < ldc (ldc even,
