@@ -169,7 +169,7 @@ def raw_cost_maximum_liklihood_fourier_domain(projected_eigenvecs, images, noise
 
 def cost_maximum_liklihood_fourier_domain(vols, images, nufft_plans, filters, noise_var, reg_scale=0,
                                           fourier_reg=None, apply_mean_const_term=False, mean_aggregate=True,
-                                          form=None, precision=ml_ops.PRECISION_BF16X3):
+                                          form=None, precision=ml_ops.PRECISION_BF16X3, half_plane=True):
     """ML objective, Fourier domain (covar_sgd.py:1350-1378).
 
     ``form="fused"`` (default) on a NufftPlanDiscretized runs ``ml_ops.MLCostDiscretized``: native
@@ -183,7 +183,8 @@ def cost_maximum_liklihood_fourier_domain(vols, images, nufft_plans, filters, no
     )
     if fused_ok:
         val = ml_ops.MLCostDiscretized.apply(vols, images.to(torch.complex64), nufft_plans, filters,
-                                             float(noise_var), bool(apply_mean_const_term), int(precision))
+                                             float(noise_var), bool(apply_mean_const_term), int(precision),
+                                             bool(half_plane))
     else:
         P = vol_forward(vols, nufft_plans, filters, fourier_domain=True)
         val = raw_cost_maximum_liklihood_fourier_domain(P, images, noise_var, apply_mean_const_term,
@@ -246,7 +247,7 @@ class CovarTrainer:
         if self.cost_func is cost_maximum_liklihood:
             kw = {"form": self.form, "precision": self.precision, "half_plane": self.half_plane}
         elif self.cost_func is cost_maximum_liklihood_fourier_domain:
-            kw = {"form": self.form, "precision": self.precision}
+            kw = {"form": self.form, "precision": self.precision, "half_plane": self.half_plane}
         cost_val = self.cost_func(self._covar(None), images, self.nufft_plans, filters, self.noise_var,
                                   self.reg_scale, self.fourier_reg, **kw)
         cost_val.backward()

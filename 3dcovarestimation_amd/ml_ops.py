@@ -259,7 +259,7 @@ class MLCostDiscretized(torch.autograd.Function):
 
     forward(vhat (r, vL, vL, vL) complex64, yhat (B, L, L) complex64 Fourier-domain images, plan
     (NufftPlanDiscretized with its points set), ctf (B, L, L) or None, noise_var (the Fourier-domain
-    noise variance, L^2 times the spatial one), mean_const, precision) -> scalar cost.
+    noise variance, L^2 times the spatial one), mean_const, precision, half) -> scalar cost.
 
     With P = mask * CTF * S(vhat) / L (the reference's Fourier-domain projection) the objective is
     0.5 mean_i(-Re(b_i^H z_i) + logdet M_i), M = I + conj(P) P^T / noise_var, b = conj(P) y / noise_var,
@@ -276,41 +276,102 @@ class MLCostDiscretized(torch.autograd.Function):
     both computed in one pass by ``cov3d_ml_grad_images_complex`` (g = g1 + i g2 written once), and the
     native exact adjoint of the slice (the slice plan's gather) scatters it, replacing autograd through
     grid_sample, solve and logdet.
+
+    With ``half`` (the default) the slices are taken on one pixel of every +-k pair only (the
+    ``half_plane_pixels`` the NUFFT path uses, weight 2, the centre 1): for a volume that is the
+    transform of a real one the slice at -k is the conjugate of the slice at k, so the pair sums
+    conj(Q) Q^T + Q conj(Q)^T and conj(Q) y + Q conj(y) are twice the real parts, which the fused
+    kernel forms.  Only the boundary pairs break the mirror (the -k member reads the Nyquist cell):
+    for each of them the other member is sliced as well (a second, small point set) and the cost
+    takes the exact terms conj(Q_o) Q_o^T - Q_h conj(Q_h)^T (and likewise for b) as a correction.
+    Backward: a mirrored pair contributes g(k) + conj(g(-k)) = 2 g1(k) once the volume gradient
+    reaches the real eigenvolumes (the imaginary pass cancels within the pair), so the half plane
+    runs the real-coefficient pass only, with the pair weight; a boundary pair takes both members'
+    exact complex gradients with weight 1 (the h value replaced, the o member scattered through the
+    second point set).  The resulting volume gradient differs from the full-plane one only by a
+    part that is anti-Hermitian in the volume, which vanishes through the Fourier transform of the
+    real eigenvolumes.
     """
 
     @staticmethod
-    def forward(ctx, vhat, yhat, plan, ctf, noise_var, mean_const, precision):
+    def forward(ctx, vhat, yhat, plan, ctf, noise_var, mean_const, precision, half=True):
         r = vhat.shape[0]
         L = yhat.shape[-1]
         B = yhat.shape[0]
         dev = vhat.device
-        c = plan.execute_forward(vhat.detach()).reshape(r, B, L * L)
         nv_s = float(noise_var) / (L * L)
-        res = ml_forward(c, yhat, L, nv_s, ctf=ctf, mean_const=False, precision=precision, want_m=True,
-                         want_minv=False, want_b=True)
-        M = res["m"].double()
-        bvec = res["b"].double()
-        # imaginary parts from the pixels whose -k partner is not their mirror image
-        bd = _boundary_pixels(plan, B, L)
-        nbd = int(bd.sum(dim=1).max())
-        Mi = torch.zeros_like(M)
-        bi = torch.zeros_like(bvec)
-        if nbd > 0:
-            order = torch.sort(bd.to(torch.int8), dim=1, descending=True, stable=True).indices[:, :nbd]
-            valid = torch.gather(bd, 1, order)  # (B, nbd)
-            from .projection_funcs import nyquist_mask
+        from .projection_funcs import nyquist_mask
 
-            scale = nyquist_mask(L, dev).reshape(1, -1).expand(B, -1)
-            if ctf is not None:
-                scale = scale * ctf.reshape(B, -1).to(torch.float32)
-            w = torch.gather(scale, 1, order) * valid / float(L * L)  # Q = mask ctf c / L^2
-            q = torch.gather(c, 2, order.unsqueeze(0).expand(r, -1, -1)) * w.unsqueeze(0)  # (r, B, nbd)
-            q = q.permute(1, 0, 2).to(torch.complex128)  # (B, r, nbd)
-            yt = torch.gather(yhat.reshape(B, -1), 1, order).to(torch.complex128) * (valid / float(L))
-            Mi = (q.conj() @ q.transpose(1, 2)).imag / nv_s
-            bi = (q.conj() @ yt.unsqueeze(-1)).squeeze(-1).imag / nv_s
-        Mc = torch.complex(M, Mi)
-        bc = torch.complex(bvec, bi)
+        scale = nyquist_mask(L, dev).reshape(1, -1).expand(B, -1)
+        if ctf is not None:
+            scale = scale * ctf.reshape(B, -1).to(torch.float32)
+        bd = _boundary_pixels(plan, B, L)
+        ctx.half = bool(half)
+        if half:
+            pix = half_plane_pixels(L, dev)
+            npix = pix.numel()
+            hp = _disc_companion(plan, "_disc_half")
+            gfull = plan._grid_flat.detach().reshape(B, L * L, 3)
+            hp.set_grid_flat(gfull[:, pix.long()])
+            c = hp.execute_forward_flat(vhat.detach()).reshape(r, B, npix)
+            res = ml_forward(c, yhat, L, nv_s, ctf=ctf, mean_const=False, precision=precision, want_m=True,
+                             want_minv=False, want_b=True, pix_map=pix)
+            M = res["m"].double()
+            bvec = res["b"].double()
+            Mc = torch.complex(M, torch.zeros_like(M))
+            bc = torch.complex(bvec, torch.zeros_like(bvec))
+            # boundary pairs: h in the half plane, o = its mirror pixel
+            in_h = torch.zeros(L * L, dtype=torch.bool, device=dev)
+            in_h[pix.long()] = True
+            bh = bd & in_h.unsqueeze(0)
+            nb = int(bh.sum(dim=1).max())
+            bnd = None
+            if nb > 0:
+                order = torch.sort(bh.to(torch.int8), dim=1, descending=True, stable=True).indices[:, :nb]
+                valid = torch.gather(bh, 1, order)  # (B, nb)
+                hpos = torch.full((L * L,), -1, dtype=torch.long, device=dev)
+                hpos[pix.long()] = torch.arange(npix, device=dev)
+                hidx = hpos[order].clamp(min=0)  # positions in the half set
+                oidx = _mirror_index(L, dev)[order]  # mirror pixels (full-plane indices)
+                go = torch.gather(gfull, 1, oidx.unsqueeze(-1).expand(-1, -1, 3)).clone()
+                go[~valid] = 4.0  # outside the volume: padded entries slice to zero
+                bp = _disc_companion(plan, "_disc_bnd")
+                bp.set_grid_flat(go)
+                co = bp.execute_forward_flat(vhat.detach()).reshape(r, B, nb)
+                vf = valid.to(torch.float32)
+                wh = torch.gather(scale, 1, order) * vf / float(L * L)
+                wo = torch.gather(scale, 1, oidx) * vf / float(L * L)
+                qh = (torch.gather(c, 2, hidx.unsqueeze(0).expand(r, -1, -1)) * wh).permute(1, 0, 2).to(torch.complex128)
+                qo = (co * wo).permute(1, 0, 2).to(torch.complex128)  # (B, r, nb)
+                yf = yhat.reshape(B, -1)
+                yh = (torch.gather(yf, 1, order) * (vf / L)).to(torch.complex128)
+                yo = (torch.gather(yf, 1, oidx) * (vf / L)).to(torch.complex128)
+                Mc = Mc + ((qo.conj() @ qo.transpose(1, 2)) - (qh @ qh.conj().transpose(1, 2))) / nv_s
+                bc = bc + ((qo.conj() @ yo.unsqueeze(-1)).squeeze(-1) - (qh @ yh.conj().unsqueeze(-1)).squeeze(-1)) / nv_s
+                bnd = (hidx, valid, co, wh, wo, order, oidx)
+        else:
+            pix = None
+            c = plan.execute_forward(vhat.detach()).reshape(r, B, L * L)
+            res = ml_forward(c, yhat, L, nv_s, ctf=ctf, mean_const=False, precision=precision, want_m=True,
+                             want_minv=False, want_b=True)
+            M = res["m"].double()
+            bvec = res["b"].double()
+            # imaginary parts from the pixels whose -k partner is not their mirror image
+            nbd = int(bd.sum(dim=1).max())
+            Mi = torch.zeros_like(M)
+            bi = torch.zeros_like(bvec)
+            if nbd > 0:
+                order = torch.sort(bd.to(torch.int8), dim=1, descending=True, stable=True).indices[:, :nbd]
+                valid = torch.gather(bd, 1, order)  # (B, nbd)
+                w = torch.gather(scale, 1, order) * valid / float(L * L)  # Q = mask ctf c / L^2
+                q = torch.gather(c, 2, order.unsqueeze(0).expand(r, -1, -1)) * w.unsqueeze(0)  # (r, B, nbd)
+                q = q.permute(1, 0, 2).to(torch.complex128)  # (B, r, nbd)
+                yt = torch.gather(yhat.reshape(B, -1), 1, order).to(torch.complex128) * (valid / float(L))
+                Mi = (q.conj() @ q.transpose(1, 2)).imag / nv_s
+                bi = (q.conj() @ yt.unsqueeze(-1)).squeeze(-1).imag / nv_s
+            Mc = torch.complex(M, Mi)
+            bc = torch.complex(bvec, bi)
+            bnd = None
         Minv = torch.linalg.inv(Mc)
         z = (Minv @ bc.unsqueeze(-1)).squeeze(-1)
         logdet = torch.linalg.slogdet(Mc)[1]
@@ -319,6 +380,8 @@ class MLCostDiscretized(torch.autograd.Function):
         if mean_const:
             cost = cost + 0.5 * (yhat.abs() ** 2).sum(dim=(1, 2)).mean() / float(noise_var)
         ctx.plan = plan
+        ctx.pix = pix
+        ctx.bnd = bnd
         ctx.meta = (L, B, nv_s, tuple(vhat.shape))
         ctx.save_for_backward(c, yhat, z, Minv, ctf if ctf is not None else torch.empty(0))
         ctx.has_ctf = ctf is not None
@@ -337,12 +400,73 @@ class MLCostDiscretized(torch.autograd.Function):
               - bz.unsqueeze(2) * bz.unsqueeze(1)).to(torch.float32)
         g = torch.empty_like(c)
         ctf_t = ctf_arg.to(torch.float32).contiguous() if ctf_arg is not None else None
-        # bound to names, so that no converted input is freed before the call
-        yh = _native.dense(yhat)
-        z1, m1 = _native.dense(a.to(torch.float32)), _native.dense(A1)
-        z2, m2 = _native.dense((-bz).to(torch.float32)), _native.dense(A2)
-        _native.check(_native.load().cov3d_ml_grad_images_complex(
-            c.data_ptr(), _native.ptr(ctf_t), yh.data_ptr(), z1.data_ptr(), m1.data_ptr(), z2.data_ptr(),
-            m2.data_ptr(), None, 0, g.data_ptr(), B, r, L, float(nv_s), 1.0 / B, _native.stream_ptr(c.device)))
-        grad_v = ctx.plan._adjoint(g.reshape(r, -1), aggregated=True).reshape(shape) * grad_out
-        return grad_v, None, None, None, None, None, None
+        pix = ctx.pix
+        if not ctx.half:
+            # bound to names, so that no converted input is freed before the call
+            yh = _native.dense(yhat)
+            z1, m1 = _native.dense(a.to(torch.float32)), _native.dense(A1)
+            z2, m2 = _native.dense((-bz).to(torch.float32)), _native.dense(A2)
+            _native.check(_native.load().cov3d_ml_grad_images_complex(
+                c.data_ptr(), _native.ptr(ctf_t), yh.data_ptr(), z1.data_ptr(), m1.data_ptr(), z2.data_ptr(),
+                m2.data_ptr(), None, 0, g.data_ptr(), B, r, L, float(nv_s), 1.0 / B, _native.stream_ptr(c.device)))
+            grad_v = ctx.plan._adjoint(g.reshape(r, -1), aggregated=True)
+        else:
+            # A +-k pair of mirrored slices contributes g(k) + conj(g(-k)) = 2 g1(k) once the volume
+            # gradient reaches the real eigenvolumes: the imaginary pass cancels within the pair, so
+            # the half plane needs the real-coefficient pass only, with the pair weight.
+            ml_grad_images(c, yhat, a.to(torch.float32), A1, L, nv_s, 1.0 / B, ctf=ctf_arg, out=g, pix_map=pix)
+            bnd = ctx.bnd
+            if bnd is not None:
+                hidx, valid, co, wh, wo, order, oidx = bnd
+                yf = yhat.reshape(B, -1)
+                vf = valid.to(torch.float32)
+
+                def exact(q, y, w):
+                    # conj(Minv) Q + conj(z) (z^T Q - y~) in the kernel's units (weight 1)
+                    e = (z.unsqueeze(1) @ q).squeeze(1) - y
+                    gg = Minv.conj() @ q + z.conj().unsqueeze(2) * e.unsqueeze(1)
+                    return (gg * (w / (B * nv_s)).unsqueeze(1)).permute(1, 0, 2).to(torch.complex64)
+
+                qh = (torch.gather(c, 2, hidx.unsqueeze(0).expand(r, -1, -1)) * wh).permute(1, 0, 2).to(torch.complex128)
+                qo = (co * wo).permute(1, 0, 2).to(torch.complex128)
+                yh = (torch.gather(yf, 1, order) * (vf / L)).to(torch.complex128)
+                yo = (torch.gather(yf, 1, oidx) * (vf / L)).to(torch.complex128)
+                # the boundary pairs are exact: both members with weight 1 and their complex gradients
+                gh = exact(qh, yh, wh)  # (r, B, nb)
+                npix = g.shape[2]
+                flat = (torch.arange(B, device=g.device).unsqueeze(1) * npix + hidx)[valid]
+                gv = g.view(r, B * npix)
+                gv[:, flat] = gh[:, valid]
+                gov = exact(qo, yo, wo).contiguous()
+                grad_o = ctx.plan._disc_bnd._adjoint(gov.reshape(r, -1), aggregated=True)
+            grad_v = ctx.plan._disc_half._adjoint(g.reshape(r, -1), aggregated=True)
+            if bnd is not None:
+                grad_v = grad_v + grad_o
+        grad_v = grad_v.reshape(shape) * grad_out
+        return grad_v, None, None, None, None, None, None, None
+
+
+def _disc_companion(plan, name: str):
+    """A discretised plan kept on ``plan`` for a derived point set (same volume geometry)."""
+    from .nufft_plan import NufftPlanDiscretized
+
+    cp = getattr(plan, name, None)
+    if cp is None:
+        cp = NufftPlanDiscretized(plan.sz, upsample_factor=plan.upsample_factor, mode=plan.mode)
+        setattr(plan, name, cp)
+    return cp
+
+
+_MIRROR_CACHE: dict = {}
+
+
+def _mirror_index(L: int, device) -> torch.Tensor:
+    """Row-major index of the -k partner of every pixel: (L - i, L - j) mod L for even L (the
+    Nyquist lines pair with themselves), (L - 1 - i, L - 1 - j) for odd L."""
+    key = (L, str(device))
+    if key not in _MIRROR_CACHE:
+        i = torch.arange(L)
+        m = (L - i) % L if L % 2 == 0 else L - 1 - i
+        _MIRROR_CACHE[key] = (m[:, None] * L + m[None, :]).reshape(-1).to(device)
+    return _MIRROR_CACHE[key]
+

@@ -132,6 +132,33 @@ class NufftPlanDiscretized(BaseNufftPlan):
         self._grid_flat = grid.reshape(-1, 3).contiguous()
         self._synced = None
 
+    def set_grid_flat(self, grid_flat: torch.Tensor) -> None:
+        """Sets the plan's points directly as (npts, 3) grid_sample coordinates (the scaled, xyz
+        form ``setpts`` builds), for point sets that are not whole L x L images; use
+        ``execute_forward_flat`` and the aggregated adjoint with them."""
+        self.points = None
+        self.points_L = None
+        self.batch_points = None
+        self._grid_flat = grid_flat.reshape(-1, 3).contiguous()
+        self._synced = None
+
+    def execute_forward_flat(self, volume: torch.Tensor) -> torch.Tensor:
+        """volume (n, D, H, W) complex -> (n, npts) slice values at the plan's points."""
+        vol = self._as_complex_volume(volume)
+        if vol.ndim == 3:
+            vol = vol.unsqueeze(0)
+        _native.require_device(vol, "volume")
+        vol = _native.dense(vol)
+        n, D, H, W = vol.shape
+        npts = self._grid_flat.shape[0]
+        out = torch.empty((n, npts), dtype=vol.dtype, device=vol.device)
+        if npts == 0:
+            return out
+        h = self._slice_plan(D, H, W, vol.dtype, vol.device)
+        _native.check(_native.load().cov3d_slice_plan_forward(h, vol.data_ptr(), out.data_ptr(), n,
+                                                              _native.stream_ptr(vol.device)))
+        return out
+
     def _as_complex_volume(self, volume) -> torch.Tensor:
         if isinstance(volume, tuple):
             volume = torch.complex(volume[0], volume[1])

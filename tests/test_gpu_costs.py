@@ -167,8 +167,9 @@ def test_fourier_domain_ml_equals_dense_likelihood(dev):
     plan = npl.NufftPlanDiscretized((L16,) * 3, upsample_factor=1, mode="bilinear")
     plan.setpts(pts)
     vf = covar.get_vectors_fourier_domain().detach()
+    # full plane: the gradient with respect to the complex volume itself
     a = vf.clone().requires_grad_(True)
-    c = sgd.cost_maximum_liklihood_fourier_domain(a, ims, plan, filt, nv)
+    c = sgd.cost_maximum_liklihood_fourier_domain(a, ims, plan, filt, nv, half_plane=False)
     c.backward()
     b = vf.to(torch.complex128).clone().requires_grad_(True)
     ref = _dense_ml(b, ims.to(torch.complex128), plan, filt.double(), nv, fourier=True)
@@ -176,6 +177,23 @@ def test_fourier_domain_ml_equals_dense_likelihood(dev):
     print(float(c), float(ref))
     assert abs(float(c) - float(ref)) < 1e-3 * abs(float(ref))
     assert rel_l2(a.grad.cpu().numpy(), b.grad.cpu().numpy()) < 1e-3
+    # both planes: the gradient with respect to the real eigenvolumes behind the volume (the
+    # half plane is exact there; its complex-volume gradient differs by an anti-Hermitian part)
+    pf = pkg("projection_funcs")
+    V = covar.get_vectors_spatial_domain().detach()
+    gc = covar.grid_correction
+
+    def fourier(v):
+        return pf.centered_fft3(v / gc.to(v.dtype), padding_size=(L16,) * 3)
+
+    Vb = V.double().clone().requires_grad_(True)
+    _dense_ml(fourier(Vb), ims.to(torch.complex128), plan, filt.double(), nv, fourier=True).backward()
+    for half in (False, True):
+        Va = V.clone().requires_grad_(True)
+        sgd.cost_maximum_liklihood_fourier_domain(fourier(Va), ims, plan, filt, nv, half_plane=half).backward()
+        err = rel_l2(Va.grad.cpu().numpy(), Vb.grad.cpu().numpy())
+        print("half" if half else "full", "plane: real-eigenvolume gradient rel err", err)
+        assert err < 1e-3
 
 
 @pytest.mark.parametrize("objective", ["ml", "ls"])

@@ -27,7 +27,11 @@ def _fourier_batch(dev, L, r, B, seed):
 @pytest.mark.parametrize("L,r,B,mode,us", [(16, 4, 32, "bilinear", 2), (17, 3, 24, "nearest", 2),
                                            (16, 4, 32, "bilinear", 1), (64, 10, 64, "bilinear", 2)])
 @pytest.mark.parametrize("precision,tol", [(0, 1e-4), (1, 1e-3)])
-def test_fused_fourier_ml_matches_autograd(dev, L, r, B, mode, us, precision, tol):
+@pytest.mark.parametrize("half", [True, False])
+def test_fused_fourier_ml_matches_autograd(dev, L, r, B, mode, us, precision, tol, half):
+    """The fused Fourier-domain ML cost (full plane, or half plane with the boundary pairs exact)
+    against the reference's autograd formulation: cost and gradients of the real eigenvolumes and
+    eigenvalues."""
     sgd = pkg("covar_sgd")
     cv = pkg("covar")
     npl = pkg("nufft_plan")
@@ -43,13 +47,14 @@ def test_fused_fourier_ml_matches_autograd(dev, L, r, B, mode, us, precision, to
         covar.init_grid_correction(mode)
         cost = sgd.cost_maximum_liklihood_fourier_domain(covar.get_vectors_fourier_domain(), ims, plan, filt, nv,
                                                          form=form, precision=precision,
-                                                         apply_mean_const_term=True)
+                                                         apply_mean_const_term=True, half_plane=half)
         cost.backward()
         out[form] = (cost.item(), covar.vectors.grad.cpu().numpy(), covar.log_sqrt_eigenvals.grad.cpu().numpy())
     ec = abs(out["fused"][0] - out["autograd"][0]) / abs(out["autograd"][0])
     eg = rel_l2(out["fused"][1], out["autograd"][1])
     el = rel_l2(out["fused"][2], out["autograd"][2])
-    print(f"L={L} r={r} {mode} us={us} precision={precision}: cost {ec:.2e} grad {eg:.2e} eigenvalue grad {el:.2e}")
+    print(f"L={L} r={r} {mode} us={us} precision={precision} half={half}: cost {ec:.2e} grad {eg:.2e} "
+          f"eigenvalue grad {el:.2e}")
     assert ec < tol and eg < tol and el < tol
 
 
