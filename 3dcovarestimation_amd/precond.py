@@ -13,14 +13,23 @@ box autocorrelation (a separable triangle), i.e. two nf^3 FFTs
 
 ``DiagonalMLP`` (in -> 64 -> 64 -> 64 -> 1, tanh-GELU, gfx950 forward and
 backward kernels ``cov3d_mlp_forward`` / ``cov3d_mlp_backward``) predicts
-log T_k from cheap per-voxel / per-shell features:
+log T_k from cheap features (``diagonal_features``):
 
     [log(sampling density x CTF^2 at us*k), log of its box means over
      3^3 / 7^3 / 15^3 us-grid voxels, log(shell mean of the same),
      log(noise variance), |k| / (L/2), 1 / (1 + |k|)]
 
 so the preconditioner needs neither the probe nor extra operator
-applications.  ``train_default`` fits it on seeded acquisitions (Haar poses,
+applications.  The default network (``FEATURE_SETS["shell"]``) reads the
+per-shell inputs only -- the shell mean of density x CTF^2 (T_k depends on the
+two only through their product), the noise variance and the radius -- so its
+weights depend on |k| alone; the shell average of the probed log T_k
+(``shell_average``) is the best shell-constant reference.  The "voxel" network
+reads all eight.
+On the padded mean systems it was measured on, the per-shell weights converge
+in fewer iterations than the per-voxel ones, although the latter fit log T_k
+more closely: the voxel-level variation of T_k inside a shell is not what
+slows CG down.  ``train_default`` fits it on seeded acquisitions (Haar poses,
 radial CTFs, L from 32 to 128) whose probed diagonals are the targets;
 ``save`` / ``load`` keep the weights in a plain ``.npz``.
 
@@ -69,8 +78,12 @@ def mlp_torch_reference(params: torch.Tensor, x: torch.Tensor) -> torch.Tensor:
 class DiagonalMLP:
     """Parameters on the device; forward/backward run on the gfx950 kernels."""
 
-    def __init__(self, n_in: int = 8, device="cuda", seed: int = 0, params: Optional[torch.Tensor] = None):
+    def __init__(self, n_in: int = 8, device="cuda", seed: int = 0, params: Optional[torch.Tensor] = None,
+                 columns: Optional[Sequence[int]] = None):
         self.n_in = n_in
+        # the columns of ``diagonal_features`` the network reads (all of them by default)
+        self.columns = tuple(range(n_in)) if columns is None else tuple(int(c) for c in columns)
+        assert len(self.columns) == n_in
         self.device = torch.device(device)
         if params is None:
             g = torch.Generator().manual_seed(seed)
@@ -148,11 +161,11 @@ class DiagonalMLP:
 
     def state_dict(self) -> dict:
         return {"n_in": self.n_in, "params": self.params.cpu(), "x_mean": self.x_mean.cpu(),
-                "x_std": self.x_std.cpu(), "y_mean": self.y_mean, "y_std": self.y_std}
+                "x_std": self.x_std.cpu(), "y_mean": self.y_mean, "y_std": self.y_std, "columns": self.columns}
 
     @classmethod
     def from_state_dict(cls, sd: dict, device="cuda") -> "DiagonalMLP":
-        m = cls(sd["n_in"], device=device, params=sd["params"])
+        m = cls(sd["n_in"], device=device, params=sd["params"], columns=sd.get("columns"))
         m.x_mean = sd["x_mean"].to(m.device)
         m.x_std = sd["x_std"].to(m.device)
         m.y_mean = float(sd["y_mean"])
@@ -162,7 +175,7 @@ class DiagonalMLP:
     def make_preconditioner(self, lhs: torch.Tensor, L: int, noise_var: Optional[float]) -> Optional[Callable]:
         """Returns r -> ifft3c(w * fft3c(r)) with w = exp(-predicted log T), or None if w is invalid."""
         feats = diagonal_features(lhs, L, 1.0 if noise_var is None else noise_var)
-        logd = self.predict_log_diag(feats)
+        logd = self.predict_log_diag(feats[:, list(self.columns)])
         return preconditioner_from_log_diag(logd, L)
 
 
@@ -205,6 +218,15 @@ def probe_diagonal(lhs: torch.Tensor, L: int) -> torch.Tensor:
 
 N_FEATURES = 8
 SMOOTH_WIDTHS = (3, 7, 15)
+# columns of ``diagonal_features``: "voxel" reads all eight; "shell" reads the per-shell ones only
+# (shell mean of density x CTF^2, noise variance, |k| / (L/2), 1 / (1 + |k|))
+FEATURE_SETS = {"voxel": tuple(range(N_FEATURES)), "shell": (4, 5, 6, 7)}
+
+
+def shell_average(logd: torch.Tensor, L: int) -> torch.Tensor:
+    """The shell average of a log diagonal (flattened L^3), expanded back to L^3: the
+    least-squares best shell-constant prediction of log T_k."""
+    return expand_fourier_shell(average_fourier_shell(logd.reshape(L, L, L)), L, 3).reshape(-1)
 
 
 def _box_smooth(v: torch.Tensor, width: int) -> torch.Tensor:
@@ -318,10 +340,12 @@ def dataset_seed(seed0: int, k: int) -> int:
 
 
 def train_default(device, n_datasets: int = 200, steps: int = 4000, seed: int = 0,
-                  log: Optional[list] = None) -> DiagonalMLP:
+                  log: Optional[list] = None, features: str = "shell") -> DiagonalMLP:
     """Trains the preconditioner MLP (features -> log T_k) on ``n_datasets`` acquisitions."""
+    cols = FEATURE_SETS[features]
     X, Y = training_set(n_datasets, device)
-    model = DiagonalMLP(N_FEATURES, device=device, seed=seed)
+    model = DiagonalMLP(len(cols), device=device, seed=seed, columns=cols)
+    X = X[:, list(cols)].contiguous()
     model.fit(X, Y, steps=steps, log=log)
     return model
 
@@ -329,35 +353,43 @@ def train_default(device, n_datasets: int = 200, steps: int = 4000, seed: int = 
 def save(model: DiagonalMLP, path) -> None:
     sd = model.state_dict()
     np.savez(path, params=sd["params"].numpy(), x_mean=sd["x_mean"].numpy(), x_std=sd["x_std"].numpy(),
-             y_mean=np.float64(sd["y_mean"]), y_std=np.float64(sd["y_std"]), n_in=np.int64(sd["n_in"]))
+             y_mean=np.float64(sd["y_mean"]), y_std=np.float64(sd["y_std"]), n_in=np.int64(sd["n_in"]),
+             columns=np.asarray(sd["columns"], dtype=np.int64))
 
 
 def load(path, device) -> DiagonalMLP:
     z = np.load(path, allow_pickle=False)
     sd = {"n_in": int(z["n_in"]), "params": torch.tensor(z["params"]), "x_mean": torch.tensor(z["x_mean"]),
-          "x_std": torch.tensor(z["x_std"]), "y_mean": float(z["y_mean"]), "y_std": float(z["y_std"])}
+          "x_std": torch.tensor(z["x_std"]), "y_mean": float(z["y_mean"]), "y_std": float(z["y_std"]),
+          "columns": tuple(int(c) for c in z["columns"]) if "columns" in z.files else None}
     return DiagonalMLP.from_state_dict(sd, device=device)
 
 
-DEFAULT_WEIGHTS = Path(__file__).with_name("precond_mlp.npz")
+WEIGHTS = {"shell": Path(__file__).with_name("precond_mlp.npz"),
+           "voxel": Path(__file__).with_name("precond_mlp_voxel.npz")}
 
 
-def load_default(device) -> DiagonalMLP:
-    """The committed network (trained by ``python -m 3dcovarestimation_amd.precond --train``)."""
-    if not DEFAULT_WEIGHTS.exists():
-        raise FileNotFoundError(f"no trained preconditioner at {DEFAULT_WEIGHTS}")
-    return load(DEFAULT_WEIGHTS, device)
+def load_default(device, features: str = "shell") -> DiagonalMLP:
+    """A committed network, trained by ``python -m 3dcovarestimation_amd.precond --train OUT
+    --features {shell,voxel}``: the per-shell one (default) or the per-voxel one."""
+    path = WEIGHTS[features]
+    if not path.exists():
+        raise FileNotFoundError(f"no trained preconditioner at {path}")
+    return load(path, device)
 
 
 def cg_comparison(reg_lhs: torch.Tensor, rhs: torch.Tensor, L: int, model: Optional[DiagonalMLP],
                   noise_var: Optional[float], rtol: float = 1e-6, maxiter: int = 3000,
-                  with_probe: bool = True, repeats: int = 1) -> dict:
+                  with_probe: bool = True, repeats: int = 1, extra_models: Optional[dict] = None,
+                  with_shell_probe: bool = False) -> dict:
     """Plain CG vs learned-diagonal PCG (vs probed-diagonal PCG) on one padded mean system.
 
     Each solve starts from the direct estimate crop(ifft3c(rhs / lhs)), as
     ``reconstruct_mean`` does.  Seconds are wall clock between device
     synchronisations and include building the preconditioner (features and
-    MLP, or the probe).  The best of ``repeats`` timed solves is kept.
+    MLP, or the probe).  The best of ``repeats`` timed solves is kept.  ``extra_models``
+    ({name: network}) adds a solve "learned_<name>" per network; ``with_shell_probe`` adds the
+    shell average of the probed diagonal (the best shell-constant preconditioner).
     """
     import time
 
@@ -391,9 +423,14 @@ def cg_comparison(reg_lhs: torch.Tensor, rhs: torch.Tensor, L: int, model: Optio
     if model is not None:
         out["learned"], v_learn = solve(lambda: model.make_preconditioner(reg_lhs, L, noise_var))
         out["learned"]["rel_diff_vs_plain"] = float((v_learn - v_plain).norm() / v_plain.norm())
+    for name, m in (extra_models or {}).items():
+        out[f"learned_{name}"], _ = solve(lambda: m.make_preconditioner(reg_lhs, L, noise_var))
     if with_probe:
         out["probe"], _ = solve(lambda: preconditioner_from_log_diag(
             torch.log(probe_diagonal(reg_lhs, L).clamp(min=1e-30)), L))
+    if with_shell_probe:
+        out["probe_shell"], _ = solve(lambda: preconditioner_from_log_diag(
+            shell_average(torch.log(probe_diagonal(reg_lhs, L).clamp(min=1e-30)), L), L))
     return out
 
 
@@ -405,6 +442,8 @@ def _main(argv=None) -> int:
     ap.add_argument("--train", metavar="OUT.npz", help="train and write the weights")
     ap.add_argument("--datasets", type=int, default=200)
     ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--features", choices=sorted(FEATURE_SETS), default="shell",
+                    help="shell: the per-shell inputs only; voxel: all eight")
     args = ap.parse_args(argv)
     if not args.train:
         ap.print_help()
@@ -415,7 +454,9 @@ def _main(argv=None) -> int:
     torch.cuda.synchronize()
     print(f"training set: {X.shape[0]} rows from {args.datasets} acquisitions in {time.perf_counter() - t0:.1f} s",
           flush=True)
-    model = DiagonalMLP(N_FEATURES, device=dev, seed=0)
+    cols = FEATURE_SETS[args.features]
+    X = X[:, list(cols)].contiguous()
+    model = DiagonalMLP(len(cols), device=dev, seed=0, columns=cols)
     log: list = []
     t0 = time.perf_counter()
     model.fit(X, Y, steps=args.steps, log=log)

@@ -13,8 +13,9 @@ CPU baseline (the CPU port under oracle/, on a bounded sample).
 
 The second half of the metric is the mean-volume CG at L=128 (100k images,
 r=0, 5% amplitude jitter): time and iterations to relative residual 1e-6 for
-plain CG and for PCG with the learned diagonal (and the probed exact
-diagonal as a reference point), reported under ``mean_cg``.
+plain CG and for PCG with the learned per-shell diagonal (``learned``); the
+per-voxel network, the probed exact diagonal and its shell average are
+reference points, all reported under ``mean_cg``.
 
 ``pipeline_L256`` is the full chosen-form pipeline at L=256 with the images
 streamed from pinned host memory (a bounded sample of the 300k-image workload):
@@ -208,9 +209,16 @@ def mean_cg_run(pkg, L, n_images, dev, cap=20000):
     except FileNotFoundError as e:
         log(f"mean CG: {e}; learned leg skipped")
         model = None
+    try:
+        extra = {"voxel": pc.load_default(dev, "voxel")}
+    except FileNotFoundError as e:
+        log(f"mean CG: {e}; per-voxel leg skipped")
+        extra = {}
     # one untimed pass creates the rocFFT plans of the Toeplitz operator and warms the MLP
-    pc.cg_comparison(reg_lhs, rhs, L, model, data.dataset.noise_var, maxiter=5, with_probe=False)
-    res = pc.cg_comparison(reg_lhs, rhs, L, model, data.dataset.noise_var, rtol=1e-6, maxiter=cap)
+    pc.cg_comparison(reg_lhs, rhs, L, model, data.dataset.noise_var, maxiter=5, with_probe=False,
+                     extra_models=extra)
+    res = pc.cg_comparison(reg_lhs, rhs, L, model, data.dataset.noise_var, rtol=1e-6, maxiter=cap,
+                           extra_models=extra, with_shell_probe=True)
     res["workload"] = (f"padded mean system (2L)^3 Toeplitz CG, L={L}, {n_images} images, r=0, 5% amplitude "
                        f"jitter, nearest back-projection us=2, rtol 1e-6, x0 = direct estimate")
     res["backprojection_s"] = round(t2 - t1, 3)
@@ -218,6 +226,8 @@ def mean_cg_run(pkg, L, n_images, dev, cap=20000):
         res["time_speedup_learned_vs_plain"] = round(res["plain"]["seconds"] / res["learned"]["seconds"], 3)
         res["iteration_ratio_learned_vs_plain"] = round(res["learned"]["iterations"] /
                                                         max(res["plain"]["iterations"], 1), 3)
+    if extra:
+        res["time_speedup_learned_voxel_vs_plain"] = round(res["plain"]["seconds"] / res["learned_voxel"]["seconds"], 3)
     del data, lhs, rhs, reg_lhs
     torch.cuda.empty_cache()
     return res

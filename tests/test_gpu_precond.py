@@ -55,15 +55,18 @@ def _held_out_system(dev, L, n_images, seed):
     return mean.regularize_lhs(lhs), rhs, data.dataset.noise_var
 
 
+@pytest.mark.parametrize("features", ["shell", "voxel"])
 @pytest.mark.parametrize("L,n_images", [(32, 8000), (64, 30000)])
-def test_trained_network_beats_plain_cg(dev, L, n_images):
-    """The committed network vs plain CG on acquisitions outside its training seeds.
+def test_trained_network_beats_plain_cg(dev, L, n_images, features):
+    """The committed networks (per-shell inputs, the default, and per-voxel inputs) vs plain CG on
+    acquisitions outside their training seeds.
 
     The padded mean system converges slowly in its weakly sampled cube corners,
     so the comparison is made at rtol 1e-4 with a 6000-iteration cap.
     """
     pc = pkg("precond")
-    model = pc.load_default(dev)
+    model = pc.load_default(dev, features)
+    assert model.columns == pc.FEATURE_SETS[features]
     seed = 910_000 + L  # training acquisitions use seeds 5001 + 7k
     lhs, rhs, noise_var = _held_out_system(dev, L, n_images, seed)
     cap = 6000
@@ -97,3 +100,29 @@ def test_l128_learned_pcg_to_1e6(dev):
     assert learned["iterations"] < plain["iterations"]
     # both meet the residual bar; the solutions differ only in the ill-conditioned corner modes
     assert learned["rel_diff_vs_plain"] < 1e-2
+
+
+def test_shell_network_matches_best_shell_constant_diagonal(dev):
+    """The per-shell network reads shell statistics and the radius, so its weights depend on |k|
+    only (equal on frequencies of equal |k|).  The shell averages of the probed log diagonal are
+    the best shell-constant weights; on a held-out L=64 acquisition the network's PCG needs at most
+    1.25x their iterations at rtol 1e-6."""
+    pc = pkg("precond")
+    model = pc.load_default(dev)
+    L = 64
+    lhs, rhs, noise_var = _held_out_system(dev, L, 30000, 930_064)
+    feats = pc.diagonal_features(lhs, L, noise_var)
+    pred = model.predict_log_diag(feats[:, list(model.columns)]).double()
+    ax = torch.ceil(torch.arange(-L / 2, L / 2, device=dev, dtype=torch.float64))[:L]
+    r2 = (ax[:, None, None] ** 2 + ax[None, :, None] ** 2 + ax[None, None, :] ** 2).reshape(-1).long()
+    n = int(r2.max()) + 1
+    cnt = torch.zeros(n, dtype=torch.float64, device=dev).index_add_(0, r2, torch.ones_like(pred))
+    avg = torch.zeros(n, dtype=torch.float64, device=dev).index_add_(0, r2, pred) / cnt.clamp(min=1)
+    assert float((pred - avg[r2]).abs().max()) < 1e-4 * float(pred.abs().max())
+    res = pc.cg_comparison(lhs, rhs, L, model, noise_var=noise_var, rtol=1e-6, maxiter=20000, with_probe=False,
+                           with_shell_probe=True)
+    print({k: (v["iterations"], v["seconds"]) for k, v in res.items() if isinstance(v, dict)})
+    learned, best = res["learned"], res["probe_shell"]
+    assert learned["converged"] and learned["fallback"] is None and best["converged"]
+    assert learned["iterations"] <= 1.25 * best["iterations"]
+    assert learned["iterations"] < 0.5 * res["plain"]["iterations"]

@@ -76,3 +76,37 @@ def test_preconditioner_equals_centred_fft_form(L):
     w = torch.exp(-logd).reshape(L, L, L).to(torch.complex64)
     ref = pf.centered_ifft3(w * pf.centered_fft3(r.reshape(L, L, L))).reshape(-1)
     assert (apply(r) - ref).abs().max() < 1e-5 * ref.abs().max()
+
+
+def test_shell_average_is_the_per_shell_mean():
+    """shell_average against a loop over the shells of the integer radius map (even and odd L)."""
+    pc = pkg("precond")
+    for L in (6, 7):
+        rng = np.random.default_rng(L)
+        x = rng.standard_normal(L**3)
+        ax = np.ceil(np.arange(-L / 2, L / 2))[:L]
+        r = np.sqrt(ax[:, None, None] ** 2 + ax[None, :, None] ** 2 + ax[None, None, :] ** 2).ravel()
+        n = int(np.ceil(r.max()))
+        shell = np.minimum(np.floor(r + 0.5), n - 1).astype(int)
+        want = np.empty_like(x)
+        for s in range(n):
+            want[shell == s] = x[shell == s].mean()
+        got = pc.shell_average(torch.tensor(x), L).numpy()
+        assert np.allclose(got, want, atol=1e-12)
+        # idempotent, and it keeps the total
+        assert np.allclose(pc.shell_average(torch.tensor(got), L).numpy(), got, atol=1e-12)
+        assert abs(got.sum() - x.sum()) < 1e-9
+
+
+def test_feature_set_round_trips_through_save_and_load(tmp_path):
+    pc = pkg("precond")
+    cols = pc.FEATURE_SETS["shell"]
+    m = pc.DiagonalMLP(len(cols), device="cpu", seed=3, columns=cols)
+    path = tmp_path / "w.npz"
+    pc.save(m, path)
+    back = pc.load(path, "cpu")
+    assert back.columns == cols and back.n_in == len(cols)
+    assert torch.equal(back.params, m.params)
+    # the committed networks: per-shell (default) and per-voxel
+    assert pc.load_default("cpu").columns == pc.FEATURE_SETS["shell"]
+    assert pc.load_default("cpu", "voxel").columns == pc.FEATURE_SETS["voxel"]
