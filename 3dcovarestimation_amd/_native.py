@@ -57,12 +57,16 @@ _SIGNATURES = {
     "cov3d_vdot_c64_f64": [c_vp, c_vp, c_i64, c_vp, c_vp],
     "cov3d_toeplitz_apply": [c_vp, c_vp, c_vp, c_i64, c_i64, c_vp],
     "cov3d_toeplitz_apply_noalloc": [c_vp, c_vp, c_vp, c_i64, c_i64, c_vp],
+    "cov3d_fft3c_pad_real_work": [c_i64, c_i64, c_i64],
+    "cov3d_fft3c_pad_real": [c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_vp],
+    "cov3d_fft3c_pad_real_adjoint": [c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_vp],
     "cov3d_mlp_forward": [c_vp, c_vp, c_vp, c_i64, c_i64, c_vp],
     "cov3d_mlp_backward": [c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_vp],
     "cov3d_timing_enable": [c_int],
     "cov3d_timing_read": [c_int, ctypes.POINTER(c_dbl), ctypes.POINTER(c_i64)],
 }
-_RESTYPES = {"cov3d_last_error": ctypes.c_char_p, "cov3d_version": ctypes.c_char_p}
+_RESTYPES = {"cov3d_last_error": ctypes.c_char_p, "cov3d_version": ctypes.c_char_p,
+             "cov3d_fft3c_pad_real_work": c_i64}
 
 EXPORTED = tuple(_SIGNATURES)
 

@@ -12,7 +12,7 @@ from typing import Any, Dict, List, Optional, Tuple, Union
 
 import torch
 
-from .projection_funcs import centered_fft3, centered_ifft3
+from .projection_funcs import centered_fft3, centered_ifft3, fft3c_pad_real, fft3c_pad_real_supported
 
 
 def sinc_grid_correction(L: int, upsampling_factor: int, nufft_disc: str) -> torch.Tensor:
@@ -52,6 +52,35 @@ class VolumeBase(torch.nn.Module):
         if self.grid_correction is not None:
             self.grid_correction = self.grid_correction.to(*args, **kwargs)
         return self
+
+    def _axis_weights(self):
+        """1 / the per-axis factor of a separable grid correction (None without one), or False when
+        the correction is not separable.  The sinc corrections are s[i] s[j] s[k] with s = 1 at the
+        centre, so s is the correction's line through the centre."""
+        gc = self.grid_correction
+        if gc is None:
+            return None
+        key = (id(gc), gc.data_ptr(), gc.device, gc.dtype, tuple(gc.shape))
+        cached = getattr(self, "_axis_cache", None)
+        if cached is not None and cached[0] == key:
+            return cached[1]
+        c = gc.shape[-1] // 2
+        s = gc[:, c, c]
+        sep = bool(torch.allclose(s[:, None, None] * s[None, :, None] * s[None, None, :], gc, rtol=1e-6, atol=0))
+        w = (1.0 / s.double()).to(torch.float32) if sep else False
+        self._axis_cache = (key, w)
+        return w
+
+    def _fourier_padded(self, v: torch.Tensor) -> torch.Tensor:
+        """centered_fft3(v / grid_correction, padding_size=(us L,) * 3) (covar.py:440-446, :188-199),
+        on the native pruned transforms for float32 volumes on the device."""
+        nf = self.resolution * self.upsampling_factor
+        w = self._axis_weights()
+        if w is not False and fft3c_pad_real_supported(v, nf):
+            return fft3c_pad_real(v, nf, w)
+        if self.grid_correction is not None:
+            v = v / self.grid_correction
+        return centered_fft3(v, padding_size=(nf,) * 3)
 
     def state_dict(self, *args, **kwargs):
         sd = super().state_dict(*args, **kwargs)
@@ -163,10 +192,7 @@ class Covar(VolumeBase):
         return _ScaleRows.apply(self.vectors, self.log_sqrt_eigenvals)
 
     def get_vectors_fourier_domain(self) -> torch.Tensor:
-        v = self.get_vectors_spatial_domain()
-        if self.grid_correction is not None:
-            v = v / self.grid_correction
-        return centered_fft3(v, padding_size=(self.resolution * self.upsampling_factor,) * 3)
+        return self._fourier_padded(self.get_vectors_spatial_domain())
 
     def get_vectors(self) -> torch.Tensor:
         return self.get_vectors_spatial_domain() if self._in_spatial_domain else self.get_vectors_fourier_domain()
@@ -228,10 +254,7 @@ class Mean(VolumeBase):
         return self.volume * torch.exp(self.log_volume_amplitude).reshape(-1, 1, 1, 1)
 
     def get_volume_fourier_domain(self) -> torch.Tensor:
-        v = self.get_volume_spatial_domain()
-        if self.grid_correction is not None:
-            v = v / self.grid_correction
-        return centered_fft3(v, padding_size=(self.resolution * self.upsampling_factor,) * 3)
+        return self._fourier_padded(self.get_volume_spatial_domain())
 
     def forward(self, dummy_var: Optional[Any] = None) -> torch.Tensor:
         return self.get_volume_spatial_domain() if self._in_spatial_domain else self.get_volume_fourier_domain()

@@ -93,6 +93,69 @@ def centered_fft2(image: torch.Tensor, im_dim: List[int] = [-1, -2], padding_siz
     return _centered(torch.fft.fft2, image, im_dim, padding_size)
 
 
+class _CentredPadFFT3Real(torch.autograd.Function):
+    """fftshift(fftn(ifftshift(pad_nf(v * w0 w1 w2)))) of real volumes v (r, L, L, L) float32 on the
+    pruned line FFTs (``cov3d_fft3c_pad_real``); the backward is the native adjoint, the real part
+    of the centred unnormalised inverse transform, cropped and weighted
+    (``cov3d_fft3c_pad_real_adjoint``)."""
+
+    @staticmethod
+    def forward(ctx, v, nf, ip):
+        from . import _native
+
+        lib = _native.load()
+        r, L = v.shape[0], v.shape[-1]
+        vd = _native.dense(v.detach())
+        out = torch.empty((r, nf, nf, nf), dtype=torch.complex64, device=v.device)
+        work = torch.empty(int(lib.cov3d_fft3c_pad_real_work(L, nf, r)), dtype=torch.complex64, device=v.device)
+        _native.check(lib.cov3d_fft3c_pad_real(vd.data_ptr(), out.data_ptr(), work.data_ptr(), ip.data_ptr(), r, L,
+                                               nf, _native.stream_ptr(v.device)))
+        ctx.save_for_backward(ip)
+        ctx.meta = (r, L, nf)
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        from . import _native
+
+        (ip,) = ctx.saved_tensors
+        r, L, nf = ctx.meta
+        lib = _native.load()
+        gd = _native.dense(g.to(torch.complex64))
+        grad = torch.empty((r, L, L, L), dtype=torch.float32, device=g.device)
+        work = torch.empty(int(lib.cov3d_fft3c_pad_real_work(L, nf, r)), dtype=torch.complex64, device=g.device)
+        _native.check(lib.cov3d_fft3c_pad_real_adjoint(gd.data_ptr(), grad.data_ptr(), work.data_ptr(),
+                                                       ip.data_ptr(), r, L, nf, _native.stream_ptr(g.device)))
+        return grad, None, None
+
+
+def fft3c_pad_real_supported(v: torch.Tensor, nf: int) -> bool:
+    """Whether ``fft3c_pad_real`` runs natively for these volumes: real float32 (r, L, L, L) on a HIP
+    device, nf in {64, 128, 256, 512} and 2L <= nf + 2."""
+    if not (v.is_cuda and v.dtype == torch.float32 and v.ndim == 4 and v.shape[1] == v.shape[2] == v.shape[3]):
+        return False
+    L = v.shape[-1]
+    return nf in (64, 128, 256, 512) and 2 * L <= nf + 2
+
+
+def fft3c_pad_real(v: torch.Tensor, nf: int, axis_weights: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """centered_fft3(v * w[:, None, None] * w[None, :, None] * w[None, None, :], padding_size=(nf,) * 3)
+    for real volumes v (r, L, L, L) (projection_funcs.py:115-128 with the padding of :11-38; the
+    Fourier-domain eigenvolumes of covar.py:440-446, w = 1 / the per-axis grid correction).  Runs on
+    the native pruned transforms when ``fft3c_pad_real_supported``, on torch's FFT otherwise."""
+    L = v.shape[-1]
+    if fft3c_pad_real_supported(v, nf):
+        if axis_weights is None:
+            ip = torch.ones(L, dtype=torch.float32, device=v.device)
+        else:
+            ip = axis_weights.to(device=v.device, dtype=torch.float32).contiguous()
+        return _CentredPadFFT3Real.apply(v, int(nf), ip)
+    if axis_weights is not None:
+        w = axis_weights.to(device=v.device, dtype=v.dtype)
+        v = v * (w[:, None, None] * w[None, :, None] * w[None, None, :])
+    return centered_fft3(v, padding_size=(nf,) * 3)
+
+
 def centered_ifft2(image: torch.Tensor, im_dim: List[int] = [-1, -2], cropping_size=None) -> torch.Tensor:
     t = _centered(torch.fft.ifft2, image, im_dim)
     return crop_tensor(t, cropping_size, im_dim) if cropping_size is not None else t

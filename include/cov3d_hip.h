@@ -288,6 +288,30 @@ int cov3d_toeplitz_apply(const void* x, const void* lhs, void* y, int64_t L,
 int cov3d_toeplitz_apply_noalloc(const void* x, const void* lhs, void* y,
                                  int64_t L, int64_t nf, void* stream);
 
+/* Fourier-domain eigenvolumes of the discretised projection: replaces
+ * centered_fft3(v / grid_correction, padding_size=(nf,)*3) of
+ * cov3d/covar.py:440-446 (and Mean.get_volume_fourier_domain, :188-199;
+ * centered_fft3 is projection_funcs.py:115-128) for real volumes:
+ *   out[t] = fftshift(fftn(ifftshift(pad_nf(v[t] * w0 w1 w2))))
+ * with v (ntrans, L, L, L) float32, out (ntrans, nf, nf, nf) complex64 in the
+ * centred layout, and the separable weights w[i] = ip[i] on each axis (the
+ * inverse of the per-axis sinc grid correction; pass ones for none).  Pruned
+ * real line FFTs on the non-zero lines; nf in {64, 128, 256, 512} with
+ * 2L <= nf + 2.  work: cov3d_fft3c_pad_real_work(L, nf, ntrans) complex64
+ * values (-1 for an unsupported size).  A twiddle table of nf complex64
+ * values is allocated per (device, nf) on first use and kept. */
+int64_t cov3d_fft3c_pad_real_work(int64_t L, int64_t nf, int64_t ntrans);
+int cov3d_fft3c_pad_real(const float* v, void* out, void* work,
+                         const float* ip, int64_t ntrans, int64_t L,
+                         int64_t nf, void* stream);
+/* Its adjoint (what autograd returns for the real input, the reference's
+ * backward through the same expression): v[t] = w0 w1 w2 *
+ * crop_L(Re(conj-transpose of the centred transform applied to g[t])), i.e.
+ * the real part of the unnormalised centred inverse transform, cropped. */
+int cov3d_fft3c_pad_real_adjoint(const void* g, float* v, void* work,
+                                 const float* ip, int64_t ntrans, int64_t L,
+                                 int64_t nf, void* stream);
+
 /* Learned diagonal preconditioner MLP (in -> 64 -> 64 -> 64 -> 1, GELU):
  * forward over n rows; weights packed [W1 b1 W2 b2 W3 b3 W4 b4] f32. */
 int cov3d_mlp_forward(const void* params, const void* x, void* y,

@@ -42,6 +42,22 @@ def test_error_path_returns_status_and_message():
     assert lib.cov3d_nufft_setpts(None, 10, None, None, None, None) == 1
 
 
+def test_fft3c_pad_real_sizes_and_errors():
+    """Work size of the centred padded transform, and its argument checks (which run before any
+    device call)."""
+    native = pkg("_native")
+    lib = native.load()
+    assert lib.cov3d_fft3c_pad_real_work(128, 256, 20) == 20 * 128 * 129 * (128 + 256)
+    assert lib.cov3d_fft3c_pad_real_work(16, 40, 2) == -1   # no pruned grid of 40 cells
+    assert lib.cov3d_fft3c_pad_real_work(40, 64, 2) == -1   # 2L > nf + 2
+    buf = (ctypes.c_float * 4)()
+    p = ctypes.cast(buf, ctypes.c_void_p)
+    assert lib.cov3d_fft3c_pad_real(p, p, p, p, 2, 16, 40, None) == 1
+    assert b"padded grid" in lib.cov3d_last_error()
+    assert lib.cov3d_fft3c_pad_real_adjoint(None, p, p, p, 2, 32, 64, None) == 1
+    assert b"null" in lib.cov3d_last_error()
+
+
 def test_nufft_plan_refuses_cpu():
     nufft_plan = pkg("nufft_plan")
     with pytest.raises(RuntimeError, match="HIP device"):
