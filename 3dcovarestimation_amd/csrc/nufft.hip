@@ -10,8 +10,8 @@
 // stencil position), so every per-bin sum runs over the same points in the same order.
 //   interpolation (k_interp_pipe, FP32; k_interp otherwise): a workgroup stages one
 //     super-bin's fine-grid tile plus its W-wide halo in LDS, two transforms per
-//     16-byte cell, and one thread per point reads its W^3 taps from LDS and writes
-//     the caller's (ntrans, M) layout; the pipelined kernel is persistent, takes its
+//     16-byte cell, and a pair of lanes per point (FP32; one thread per point
+//     otherwise) reads its W^3 taps from LDS and writes the caller's (ntrans, M) layout; the pipelined kernel is persistent, takes its
 //     items from per-XCD counters and loads the next tile while the current taps run;
 //   spreading (k_spread_mfma, FP32; k_spread_gather for FP64): output-driven gather,
 //     one workgroup per 4^3 bin walks the points of the 27 neighbouring bins and forms
@@ -23,6 +23,7 @@
 #include <rocfft/rocfft.h>
 
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "common.h"
@@ -690,6 +691,13 @@ __device__ __forceinline__ void stencil_weights_poly(float u, const EsPoly& P, i
   l0 = j0 + W / 2;
 }
 
+// Element i of a buffer that no kernel writes while the caller runs, through the constant address
+// space: with a wave-uniform index this is a scalar load, which does not wait for the wave's vector
+// loads in flight.
+__device__ __forceinline__ int load_invariant(const int* p, int i) {
+  return ((const __attribute__((address_space(4))) int*)(uintptr_t)p)[i];
+}
+
 // Fine-grid index of an unwrapped coordinate within one period of [0, n).
 __device__ __forceinline__ int wrap1(int m, int n) {
   m += m < 0 ? n : 0;
@@ -777,9 +785,11 @@ __device__ __forceinline__ void stage_tile(const Cplx<T>* __restrict__ fine, Cpl
 }
 
 // Bank-aware visiting order of the points of each interpolation subproblem (FP32 tiles of two
-// transforms per 16-byte cell, W <= 8).  All lanes of k_interp read the same tap offset together,
-// and ds_read_b128 serves 16 lanes per clock from 64 banks (measured), so a read is conflict-free
-// when the 16 lanes' base cells differ mod 16.  Points are bucketed by base cell mod 16 and dealt
+// transforms per 16-byte cell, W <= 8).  ds_read_b128 serves 16 lanes per clock from 64 banks
+// (measured), so a read is conflict-free when the 16 lanes' cells differ mod 16.  The interpolation
+// kernels hold one point per lane pair (interp_pair_f32): a 16-lane read covers 8 consecutive
+// slots, whose two halves read cells 4 planes apart (8 mod 16 at W = 7), so it is conflict-free
+// when the 8 points' base cells differ mod 8.  Points are bucketed by base cell mod 8 and dealt
 // round-robin over the buckets: bucket b, rank r goes to slot sum_b' min(c_b', r) +
 // #{b' < b : c_b' > r}.  The point at slot s is stored as a record irec[p0 + s] = (its three
 // grid coordinates, its sorted index as int bits) with its caller index iidx[p0 + s] = sidx[p]:
@@ -794,10 +804,11 @@ __global__ void __launch_bounds__(kBlock)
   if (sub >= *sd.nsub) return;
   constexpr int E = kSuperEdge + W;
   constexpr int kPer = kMaxSub / kBlock;
-  __shared__ int s_cnt[16];
+  constexpr int NB = 8;  // bank classes
+  __shared__ int s_cnt[NB];
   const int p0 = sd.sub_pt[sub];
   const int np = sd.sub_n[sub];
-  if (threadIdx.x < 16) s_cnt[threadIdx.x] = 0;
+  if (threadIdx.x < NB) s_cnt[threadIdx.x] = 0;
   __syncthreads();
   const float halfw = 0.5f * (float)W;
   int rb[kPer], rr[kPer];
@@ -809,7 +820,7 @@ __global__ void __launch_bounds__(kBlock)
       const int m0 = min(max((int)ceilf(sv0[p] - halfw) + W / 2, 0), kSuperEdge);
       const int m1 = min(max((int)ceilf(sv1[p] - halfw) + W / 2, 0), kSuperEdge);
       const int m2 = min(max((int)ceilf(sv2[p] - halfw) + W / 2, 0), kSuperEdge);
-      rb[k] = (m0 * interp_plane<float, W, 2>() + m1 * E + m2) & 15;
+      rb[k] = (m0 * interp_plane<float, W, 2>() + m1 * E + m2) % NB;
       rr[k] = atomicAdd(&s_cnt[rb[k]], 1);
     }
   }
@@ -820,7 +831,7 @@ __global__ void __launch_bounds__(kBlock)
     if (slot < np) {
       int pos = 0;
 #pragma unroll
-      for (int b = 0; b < 16; ++b) {
+      for (int b = 0; b < NB; ++b) {
         const int cb = s_cnt[b];
         pos += min(cb, rr[k]) + ((b < rb[k] && cb > rr[k]) ? 1 : 0);
       }
@@ -836,6 +847,7 @@ __global__ void __launch_bounds__(kBlock)
 // (a0, a1) go in order, double-buffered: the W cells of the next row are read from LDS while the
 // FMAs of the current row run (occupancy is bounded by the tile, so registers are spent on reads
 // in flight).  Each row's W taps are weighted along z first, then by the row weight w0 w1 once.
+// The FP32 interpolation kernels with point records use interp_pair_f32 instead.
 template <int W, int TC, int PL>
 __device__ __forceinline__ void interp_taps_f32(const float4* __restrict__ t4, int l0, int l1, int l2,
                                                 const float* w0, const float* w1, const float* w2, float* ax,
@@ -884,6 +896,128 @@ __device__ __forceinline__ void interp_taps_f32(const float4* __restrict__ t4, i
   }
 }
 
+// Value of lane (lane ^ 1) of the wave (a quad_perm [1, 0, 3, 2] DPP move).
+__device__ __forceinline__ float swap_pair(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0xB1, 0xF, 0xF, false));
+}
+
+// Taps of one point on the FP32 tile of two transforms per 16-byte cell (W <= 8), split over a lane
+// pair: lanes 2k and 2k + 1 of a wave hold the same point, and lane half h = lane & 1 runs half of
+// its W^2 stencil rows.  Half 0 takes axis-0 planes [0, W/2), half 1 planes [W - W/2, W); for odd W
+// the middle plane W/2 is shared, rows [0, C) to half 0 and [C, W) to half 1 (C = (W + 1) / 2).  Half
+// 1 walks the middle plane from row C - 1, which belongs to half 0, with weight 0, so both halves run
+// C rows at the same static offsets and every read stays inside the point's stencil.  Each plane's
+// rows go in order with the next row's reads in flight, are summed along z and then weighted by
+// w0 w1, as in interp_taps_f32 (the planes of a half are a loop, which keeps the compiler from
+// hoisting later rows' reads).  The two halves' sums are added across the pair (one DPP swap per
+// value) and half h returns transform h.  Against one thread per point this halves the lanes a
+// short subproblem leaves idle, and a 16-lane LDS read covers 8 points whose halves sit 4 planes
+// (8 cells mod 16 at W = 7) apart, so the bank-aware visiting order needs the 8 points' base cells
+// distinct mod 8 only.
+template <int W, int PL>
+__device__ __forceinline__ float2 interp_pair_f32(const float4* __restrict__ t4, int h, int l0, int l1, int l2,
+                                                  const float* wp, float wmid, const float* w1, const float* w2) {
+  static_assert(W <= 8, "one 16-byte word of two transforms per cell");
+  constexpr int E = kSuperEdge + W;
+  constexpr int FULL = W / 2;                    // whole axis-0 planes per half
+  constexpr int C = (W & 1) ? (W + 1) / 2 : 0;  // middle-plane rows per half
+  float ax0 = 0.f, ay0 = 0.f, ax1 = 0.f, ay1 = 0.f;
+  // n rows of one plane from LDS cell pa: row a1 at pa + a1 E, weight wp * wr[a1]
+  auto plane = [&](int pa, float wp, const float* wr, auto nrows) {
+    constexpr int n = decltype(nrows)::value;
+    float4 buf[2][W];
+#pragma unroll
+    for (int a2 = 0; a2 < W; ++a2) buf[0][a2] = t4[pa + a2];
+#pragma unroll
+    for (int a1 = 0; a1 < n; ++a1) {
+      if (a1 + 1 < n) {
+#pragma unroll
+        for (int a2 = 0; a2 < W; ++a2) buf[(a1 + 1) & 1][a2] = t4[pa + (a1 + 1) * E + a2];
+      }
+      float rx0 = w2[0] * buf[a1 & 1][0].x, ry0 = w2[0] * buf[a1 & 1][0].y;
+      float rx1 = w2[0] * buf[a1 & 1][0].z, ry1 = w2[0] * buf[a1 & 1][0].w;
+#pragma unroll
+      for (int a2 = 1; a2 < W; ++a2) {
+        const float4 v = buf[a1 & 1][a2];
+        rx0 += w2[a2] * v.x;
+        ry0 += w2[a2] * v.y;
+        rx1 += w2[a2] * v.z;
+        ry1 += w2[a2] * v.w;
+      }
+      const float ww = wp * wr[a1];
+      ax0 += ww * rx0;
+      ay0 += ww * ry0;
+      ax1 += ww * rx1;
+      ay1 += ww * ry1;
+    }
+  };
+  // this half's plane weights (wp[a]: plane (h ? W - FULL : 0) + a), consumed in order
+  float wq[FULL > 0 ? FULL : 1];
+#pragma unroll
+  for (int a = 0; a < FULL; ++a) wq[a] = wp[a];
+  const int base = (l0 + (h ? W - FULL : 0)) * PL + l1 * E + l2;
+#pragma unroll 1
+  for (int a = 0; a < FULL; ++a) {
+    plane(base + a * PL, wq[0], w1, std::integral_constant<int, W>{});
+#pragma unroll
+    for (int q = 0; q + 1 < FULL; ++q) wq[q] = wq[q + 1];
+  }
+  if constexpr (C > 0) {
+    float wm[C];
+#pragma unroll
+    for (int j = 0; j < C; ++j) wm[j] = h ? (j == 0 ? 0.f : w1[C - 1 + j]) : w1[j];
+    plane((l0 + W / 2) * PL + (l1 + (h ? C - 1 : 0)) * E + l2, wmid, wm, std::integral_constant<int, C>{});
+  }
+  // the partner keeps the other transform: send it this half's sum of that one
+  const float px = swap_pair(h ? ax0 : ax1), py = swap_pair(h ? ay0 : ay1);
+  return make_float2((h ? ax1 : ax0) + px, (h ? ay1 : ay0) + py);
+}
+
+// One point of a lane pair (interp_pair_f32) from its grid coordinates: the stencil weights, then
+// the taps.  With the exp/sqrt weights, half 0 evaluates the axis-1 weights and half 1 the axis-2
+// weights, and the pair swaps them (one DPP move each); each half evaluates only the axis-0
+// weights of its own planes.  The values are those of stencil_weights for the same taps.
+template <int W, int PL>
+__device__ __forceinline__ float2 interp_pair_point_f32(const float4* __restrict__ t4, int h, float u0, float u1,
+                                                        float u2, float kc, float beta, const EsPoly& poly) {
+  constexpr int FULL = W / 2;
+  float w1[W], w2[W], wp[FULL > 0 ? FULL : 1], wmid = 0.f;
+  int l0, l1, l2;
+  if (poly.on) {
+    float w0[W];
+    stencil_weights_poly<W>(u0, poly, l0, w0);
+    stencil_weights_poly<W>(u1, poly, l1, w1);
+    stencil_weights_poly<W>(u2, poly, l2, w2);
+#pragma unroll
+    for (int a = 0; a < FULL; ++a) wp[a] = h ? w0[W - FULL + a] : w0[a];
+    if constexpr ((W & 1) != 0) wmid = w0[W / 2];
+  } else {
+    const float halfw = 0.5f * (float)W;
+    const float beta_l2 = beta * 1.4426950408889634f;
+    const int j0 = (int)ceilf(u0 - halfw), j1 = (int)ceilf(u1 - halfw), j2 = (int)ceilf(u2 - halfw);
+    l0 = j0 + W / 2;
+    l1 = j1 + W / 2;
+    l2 = j2 + W / 2;
+    const float uo = h ? u2 : u1;
+    const int jo = h ? j2 : j1;
+#pragma unroll
+    for (int a = 0; a < W; ++a) {
+      const float own = es_eval_native((float)(jo + a) - uo, kc, beta_l2);
+      const float oth = swap_pair(own);
+      w1[a] = h ? oth : own;
+      w2[a] = h ? own : oth;
+    }
+    const int f0 = j0 + (h ? W - FULL : 0);
+#pragma unroll
+    for (int a = 0; a < FULL; ++a) wp[a] = es_eval_native((float)(f0 + a) - u0, kc, beta_l2);
+    if constexpr ((W & 1) != 0) wmid = es_eval_native((float)(j0 + W / 2) - u0, kc, beta_l2);
+  }
+  l0 = min(max(l0, 0), kSuperEdge);
+  l1 = min(max(l1, 0), kSuperEdge);
+  l2 = min(max(l2, 0), kSuperEdge);
+  return interp_pair_f32<W, PL>(t4, h, l0, l1, l2, wp, wmid, w1, w2);
+}
+
 // Type-2 interpolation: c[t, j] = sum_taps phi * fine[t, tap].
 // One workgroup per (super-bin subproblem, transform chunk).  The 8^3 tile
 // plus a W-cell halo is staged in LDS as [cell][TC/2] 16-byte words holding
@@ -910,46 +1044,32 @@ __global__ void __launch_bounds__(kBlock)
   const int o0 = s0 * kSuperEdge - W / 2, o1 = s1 * kSuperEdge - W / 2, o2 = s2 * kSuperEdge - W / 2;
   const int p0 = sd.sub_pt[sub];
   const int np = sd.sub_n[sub];
-  // Points are visited in the bank-aware order of their records (k_bank_order), when present:
-  // a thread's (at most kMaxSub / kBlock) records are loaded before the tile is staged, so their
-  // latency overlaps the staging loads, and each loop iteration takes the next one.
-  constexpr int kPer = kMaxSub / kBlock;
-  float4 prec[kPer];
-  int pidx[kPer];
-  if (irec) {
-#pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-      const int slot = threadIdx.x + k * kBlock;
-      prec[k] = slot < np ? irec[p0 + slot] : make_float4(0.f, 0.f, 0.f, 0.f);
-      pidx[k] = slot < np ? iidx[p0 + slot] : 0;
-    }
-  }
   // halo cells lie within one period of the grid (nf >= 2W), so one conditional wrap suffices
   constexpr int PL = interp_plane<T, W, TC>();
   stage_tile<T, W, TC, PL, kInterpStageGroup>(fine, tile, g, o0, o1, o2, t0, ntrans, half_h2, conj_all);
   __syncthreads();
-  for (int slot = threadIdx.x; slot < np; slot += kBlock) {
-    int p;
-    int64_t j;
-    T u0, u1, u2;
+  if constexpr (sizeof(T) == 4 && TC == 2 && W <= 8) {
     if (irec) {
-      u0 = (T)prec[0].x;
-      u1 = (T)prec[0].y;
-      u2 = (T)prec[0].z;
-      p = __float_as_int(prec[0].w);
-      j = pidx[0];
-#pragma unroll
-      for (int k = 0; k + 1 < kPer; ++k) {
-        prec[k] = prec[k + 1];
-        pidx[k] = pidx[k + 1];
+      // point records in the bank-aware order of k_bank_order, one point per lane pair
+      const int h = threadIdx.x & 1;
+      for (int slot = threadIdx.x >> 1; slot < np; slot += kBlock / 2) {
+        const float4 rec = irec[p0 + slot];
+        const int p = __float_as_int(rec.w);
+        const float2 v = interp_pair_point_f32<W, PL>(reinterpret_cast<const float4*>(tile), h, rec.x, rec.y, rec.z,
+                                                      kc, beta, poly);
+        if (direct) {
+          if (t0 + h < ntrans) direct[(int64_t)(t0 + h) * M + iidx[p0 + slot]] = Cplx<T>{v.x, v.y};
+        } else {
+          c[((int64_t)blockIdx.y * M + p) * TC + h] = Cplx<T>{v.x, v.y};
+        }
       }
-    } else {
-      p = p0 + slot;
-      u0 = sv0[p];
-      u1 = sv1[p];
-      u2 = sv2[p];
-      j = direct ? sidx[p] : 0;
+      return;
     }
+  }
+  for (int slot = threadIdx.x; slot < np; slot += kBlock) {
+    const int p = p0 + slot;
+    const int64_t j = direct ? sidx[p] : 0;
+    const T u0 = sv0[p], u1 = sv1[p], u2 = sv2[p];
     T w0[W], w1[W], w2[W];
     int l0, l1, l2;
     bool use_poly = false;
@@ -1052,31 +1172,43 @@ __global__ void __launch_bounds__(kBlock)
     s_v[1] = atomicAdd(&ctr[xcd], 1);
   }
   __syncthreads();
-  int64_t item = item_of(s_v[0]);
+  // the item indices come from LDS; readfirstlane makes them wave-uniform, so the subproblem
+  // descriptors are read with scalar loads (a vector load there would wait for every load in flight)
+  int64_t item = item_of(__builtin_amdgcn_readfirstlane(s_v[0]));
   if (item >= nitems) return;
   // Tile staging: thread c < E^2 owns the tile column (e1, e2) = (c / E, c % E) and loads its E
   // cells e0 = 0 .. E-1; the fine-grid row of e0 is wave-uniform, so a cell costs a select and
-  // an add beyond its loads.
+  // an add beyond its loads.  The threads past E^2 load the last column too and store nothing:
+  // with no branch around the loads, nothing joins their registers before the LDS write.
   static_assert(E * E <= kBlock, "one tile column per thread");
   const int col = threadIdx.x;
   const bool has_col = col < E * E;
-  const int ce1 = col / E, ce2 = col % E;
-  float4 tv[E];  // next tile: cells (e0, ce1, ce2), transforms t0 and t0 + 1
+  const int ce1 = min(col, E * E - 1) / E, ce2 = min(col, E * E - 1) % E;
+  // next tile: cells (e0, ce1, ce2) of transforms t0 (tv0) and t0 + 1 (tv1), as loaded; the two
+  // transforms stay apart until the LDS write so that no register move consumes a load early
+  float2 tv0[E], tv1[E];
+  bool tneg = false, ttwo = true;  // ... to be conjugated / with no second transform (zeroed)
   float4 nrec[kPer];
   int nidx[kPer];
   // records first: waiting for the tile's loads then covers them too
   auto fetch = [&](int64_t it) {
-    const int sub = (int)(it % nsub);
-    const int t0 = (int)(it / nsub) * 2;
-    const int p0 = sd.sub_pt[sub];
-    const int np = sd.sub_n[sub];
+    const int sub = __builtin_amdgcn_readfirstlane((int)(it % nsub));
+    const int t0 = __builtin_amdgcn_readfirstlane((int)(it / nsub)) * 2;
+    const int p0 = load_invariant(sd.sub_pt, sub);
+    const int np = load_invariant(sd.sub_n, sub);
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
-      const int slot = threadIdx.x + k * kBlock;
-      nrec[k] = slot < np ? irec[p0 + slot] : make_float4(0.f, 0.f, 0.f, 0.f);
-      nidx[k] = slot < np ? iidx[p0 + slot] : 0;
+      // slots past the subproblem (never read back) repeat its last record: no branch around the loads
+      // (copied by component: a whole-float4 copy here left nrec in scratch memory)
+      const int slot = min((int)threadIdx.x + k * kBlock, np - 1);
+      const float4 r = irec[p0 + slot];
+      nrec[k].x = r.x;
+      nrec[k].y = r.y;
+      nrec[k].z = r.z;
+      nrec[k].w = r.w;
+      nidx[k] = iidx[p0 + slot];
     }
-    const int sb = sd.sub_sb[sub];
+    const int sb = load_invariant(sd.sub_sb, sub);
     const int s2 = sb % g.nsb[2];
     const int s1 = (sb / g.nsb[2]) % g.nsb[1];
     const int s0 = sb / (g.nsb[2] * g.nsb[1]);
@@ -1092,17 +1224,17 @@ __global__ void __launch_bounds__(kBlock)
     const int rowst = g.nf[1] * ld2;
     const Cplx<float>* f0 = fine + (int64_t)t0 * nfine;
     const Cplx<float>* f1 = two ? f0 + nfine : f0;
+    // the values stay as loaded until they are written to LDS (after the current item's taps):
+    // a sign flip here would wait for the loads before the taps start
+    tneg = neg;
+    ttwo = two;
 #pragma unroll
     for (int k = 0; k < E; ++k) {
-      if (has_col) {
-        const int r0 = wrap1(o0 + k, g.nf[0]);
-        const int r0m = r0 ? g.nf[0] - r0 : 0;
-        const uint32_t off = (uint32_t)((mir ? r0m : r0) * rowst + colo);
-        const Cplx<float> x0 = f0[off];
-        Cplx<float> x1 = f1[off];
-        if (!two) x1 = Cplx<float>{0.f, 0.f};
-        tv[k] = neg ? make_float4(x0.x, -x0.y, x1.x, -x1.y) : make_float4(x0.x, x0.y, x1.x, x1.y);
-      }
+      const int r0 = wrap1(o0 + k, g.nf[0]);
+      const int r0m = r0 ? g.nf[0] - r0 : 0;
+      const uint32_t off = (uint32_t)((mir ? r0m : r0) * rowst + colo);
+      tv0[k] = reinterpret_cast<const float2*>(f0)[off];
+      tv1[k] = reinterpret_cast<const float2*>(f1)[off];
     }
   };
   fetch(item);
@@ -1110,55 +1242,48 @@ __global__ void __launch_bounds__(kBlock)
     const int sub = (int)(item % nsub);
     const int pair = (int)(item / nsub);
     const int t0 = pair * 2;
-    const int np = sd.sub_n[sub];
+    const int np = load_invariant(sd.sub_n, sub);
     __syncthreads();  // the previous item's taps are done with the tile
+    // s_v[1] (the next item) was written before the barrier above and is rewritten only after
+    // the barrier below
+    const int64_t nxt = item_of(__builtin_amdgcn_readfirstlane(s_v[1]));
+    // the item after next: its index is needed only at the next iteration's fetch.  The atomic is
+    // issued before the LDS writes below (its latency passes while they wait for the tile loads)
+    // and before the next item's loads (a wait for it leaves those in flight), as an increment
+    // that wraps at 2^32 - 1 (a plain atomicAdd is turned into a wave-aggregated form that waits
+    // for its result at once).
+    int v2;  // assigned and read by thread 0 only, when nxt < nitems (no join that would wait for it)
+    if (threadIdx.x == 0 && nxt < nitems) v2 = (int)atomicInc((unsigned*)&ctr[xcd], 0xffffffffu);
 #pragma unroll
-    for (int k = 0; k < E; ++k)
-      if (has_col) t4[k * PL + col] = tv[k];
-    // the current item's records move to LDS (each thread reads back only its own slots)
+    for (int k = 0; k < E; ++k) {
+      if (has_col) {
+        const float2 a = tv0[k], b = tv1[k];
+        const float sy = tneg ? -a.y : a.y, sw = tneg ? -b.y : b.y;
+        t4[k * PL + col] = make_float4(a.x, sy, ttwo ? b.x : 0.f, ttwo ? sw : 0.f);
+      }
+    }
+    // the current item's records move to LDS as one flat array of slots (read after the barrier below)
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
       s_rec[k][threadIdx.x] = nrec[k];
       s_idx[k][threadIdx.x] = nidx[k];
     }
-    // s_v[1] (the next item) was written before the barrier above and is rewritten only after
-    // the barrier below
-    const int64_t nxt = item_of(s_v[1]);
     __syncthreads();
-    // the item after next: its index is needed only at the next iteration's fetch, so the
-    // atomic's latency hides behind this item's taps
-    int v2 = 0;
-    if (threadIdx.x == 0 && nxt < nitems) v2 = atomicAdd(&ctr[xcd], 1);
-    if (nxt < nitems) fetch(nxt);
-    for (int slot = threadIdx.x, k = 0; slot < np; slot += kBlock, ++k) {
-      const float4 rec = s_rec[k][threadIdx.x];
+    // unconditional (the last item fetches itself again, unused): a branch here would join the
+    // prefetch registers and wait for the loads before the taps
+    fetch(nxt < nitems ? nxt : item);
+    // one point per lane pair (interp_pair_f32); the records are read back as one flat array
+    const int h = threadIdx.x & 1;
+    for (int slot = threadIdx.x >> 1; slot < np; slot += kBlock / 2) {
+      const float4 rec = s_rec[slot / kBlock][slot % kBlock];
       const float u0 = rec.x, u1 = rec.y, u2 = rec.z;
       const int p = __float_as_int(rec.w);
-      const int64_t j = s_idx[k][threadIdx.x];
-      float w0[W], w1[W], w2[W];
-      int l0, l1, l2;
-      if (poly.on) {
-        stencil_weights_poly<W>(u0, poly, l0, w0);
-        stencil_weights_poly<W>(u1, poly, l1, w1);
-        stencil_weights_poly<W>(u2, poly, l2, w2);
-      } else {
-        stencil_weights<float, W>(u0, kc, beta, l0, w0);
-        stencil_weights<float, W>(u1, kc, beta, l1, w1);
-        stencil_weights<float, W>(u2, kc, beta, l2, w2);
-      }
-      l0 = min(max(l0, 0), kSuperEdge);
-      l1 = min(max(l1, 0), kSuperEdge);
-      l2 = min(max(l2, 0), kSuperEdge);
-      float ax[2] = {0.f, 0.f}, ay[2] = {0.f, 0.f};
-      interp_taps_f32<W, 2, PL>(t4, l0, l1, l2, w0, w1, w2, ax, ay);
+      const int64_t j = s_idx[slot / kBlock][slot % kBlock];
+      const float2 v = interp_pair_point_f32<W, PL>(t4, h, u0, u1, u2, kc, beta, poly);
       if (direct) {
-#pragma unroll
-        for (int tt = 0; tt < 2; ++tt)
-          if (t0 + tt < ntrans) direct[(int64_t)(t0 + tt) * M + j] = Cplx<float>{ax[tt], ay[tt]};
+        if (t0 + h < ntrans) direct[(int64_t)(t0 + h) * M + j] = Cplx<float>{v.x, v.y};
       } else {
-        Cplx<float>* dst = c + ((int64_t)pair * M + p) * 2;
-        dst[0] = Cplx<float>{ax[0], ay[0]};
-        dst[1] = Cplx<float>{ax[1], ay[1]};
+        c[((int64_t)pair * M + p) * 2 + h] = Cplx<float>{v.x, v.y};
       }
     }
     if (nxt >= nitems) break;
