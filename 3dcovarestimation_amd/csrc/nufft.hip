@@ -240,16 +240,44 @@ struct BinGeom {
   int nsb[3];  // super-bins per axis (nf / 8)
 };
 
+// Super-bin numbering.  When every axis holds a multiple of 4 super-bins they are numbered in
+// 4x4x4 blocks (block-major, raster inside a block), otherwise in raster order.  Subproblems follow
+// this order, and the interpolation deals runs of consecutive subproblems to one XCD, so a run is
+// a compact block whose tiles share their halos in that XCD's L2 (a run of two raster rows shares
+// them along two axes only).  Every per-point and per-bin sum is independent of the numbering.
+__host__ __device__ __forceinline__ bool super_blocked(const BinGeom& g) {
+  return (g.nsb[0] % 4) == 0 && (g.nsb[1] % 4) == 0 && (g.nsb[2] % 4) == 0;
+}
+
+__host__ __device__ __forceinline__ int super_index(const BinGeom& g, int s0, int s1, int s2) {
+  if (!super_blocked(g)) return (s0 * g.nsb[1] + s1) * g.nsb[2] + s2;
+  const int blk = ((s0 >> 2) * (g.nsb[1] >> 2) + (s1 >> 2)) * (g.nsb[2] >> 2) + (s2 >> 2);
+  return blk * 64 + (((s0 & 3) << 4) | ((s1 & 3) << 2) | (s2 & 3));
+}
+
+__host__ __device__ __forceinline__ void super_coords(const BinGeom& g, int sb, int& s0, int& s1, int& s2) {
+  if (!super_blocked(g)) {
+    s2 = sb % g.nsb[2];
+    s1 = (sb / g.nsb[2]) % g.nsb[1];
+    s0 = sb / (g.nsb[2] * g.nsb[1]);
+    return;
+  }
+  const int blk = sb >> 6, w = sb & 63;
+  const int n1 = g.nsb[1] >> 2, n2 = g.nsb[2] >> 2;
+  s0 = ((blk / (n1 * n2)) << 2) | (w >> 4);
+  s1 = (((blk / n2) % n1) << 2) | ((w >> 2) & 3);
+  s2 = ((blk % n2) << 2) | (w & 3);
+}
+
 __host__ __device__ __forceinline__ int bin_index(const BinGeom& g, int b0, int b1, int b2) {
-  int sb = ((b0 >> 1) * g.nsb[1] + (b1 >> 1)) * g.nsb[2] + (b2 >> 1);
+  const int sb = super_index(g, b0 >> 1, b1 >> 1, b2 >> 1);
   return sb * 8 + ((b0 & 1) << 2) + ((b1 & 1) << 1) + (b2 & 1);
 }
 
 __device__ __forceinline__ void bin_coords(const BinGeom& g, int bin, int& b0, int& b1, int& b2) {
   int sb = bin >> 3, k = bin & 7;
-  int s2 = sb % g.nsb[2];
-  int s1 = (sb / g.nsb[2]) % g.nsb[1];
-  int s0 = sb / (g.nsb[2] * g.nsb[1]);
+  int s0, s1, s2;
+  super_coords(g, sb, s0, s1, s2);
   b0 = 2 * s0 + ((k >> 2) & 1);
   b1 = 2 * s1 + ((k >> 1) & 1);
   b2 = 2 * s2 + (k & 1);
@@ -1036,9 +1064,8 @@ __global__ void __launch_bounds__(kBlock)
   if (sub >= *sd.nsub) return;
   const int t0 = blockIdx.y * TC;
   const int sb = sd.sub_sb[sub];
-  const int s2 = sb % g.nsb[2];
-  const int s1 = (sb / g.nsb[2]) % g.nsb[1];
-  const int s0 = sb / (g.nsb[2] * g.nsb[1]);
+  int s0, s1, s2;
+  super_coords(g, sb, s0, s1, s2);
   constexpr int E = kSuperEdge + W;
   constexpr int ncell = E * E * E;
   const int o0 = s0 * kSuperEdge - W / 2, o1 = s1 * kSuperEdge - W / 2, o2 = s2 * kSuperEdge - W / 2;
@@ -1209,9 +1236,8 @@ __global__ void __launch_bounds__(kBlock)
       nidx[k] = iidx[p0 + slot];
     }
     const int sb = load_invariant(sd.sub_sb, sub);
-    const int s2 = sb % g.nsb[2];
-    const int s1 = (sb / g.nsb[2]) % g.nsb[1];
-    const int s0 = sb / (g.nsb[2] * g.nsb[1]);
+    int s0, s1, s2;
+    super_coords(g, sb, s0, s1, s2);
     const int o0 = s0 * kSuperEdge - W / 2, o1 = s1 * kSuperEdge - W / 2, o2 = s2 * kSuperEdge - W / 2;
     const bool two = t0 + 1 < ntrans;
     // the column's offset within a fine-grid x row, and whether its values are mirrored
@@ -1606,13 +1632,16 @@ __global__ void __launch_bounds__(kGatherBlock) __attribute__((amdgpu_waves_per_
     const int pi = pi_n, pk = pk_n;
     const float v0 = v0n, v1 = v1n, v2 = v2n;
     const bool valid = base + lane < total;
-    if (base + kGatherBlock < total) {
-      locate(base + kGatherBlock, ncur, pi_n, pk_n, ncur);
-      if (base + kGatherBlock + lane < total) {
-        v0n = sv0[pi_n];
-        v1n = sv1[pi_n];
-        v2n = sv2[pi_n];
-      }
+    {
+      // the next chunk's coordinates, loaded unconditionally (lanes past the neighbourhood, and
+      // every lane after the last chunk, read point 0 and never use it): loads under a branch
+      // would be joined, and waited for, before this chunk's work
+      const int nb = base + kGatherBlock;
+      locate(nb < total ? nb : base, ncur, pi_n, pk_n, ncur);
+      const int pl = nb + lane < total ? pi_n : 0;
+      v0n = sv0[pl];
+      v1n = sv1[pl];
+      v2n = sv2[pl];
     }
     bool hit = false;
     if (valid) {
