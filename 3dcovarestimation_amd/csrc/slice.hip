@@ -8,6 +8,7 @@
 // (nufft_plan.py:171-242).  The coordinate unnormalisation, corner rounding,
 // weight products and the corner order follow grid_sample's documented
 // trilinear semantics so the two agree to float rounding.
+#include <algorithm>
 #include <map>
 #include <mutex>
 
@@ -19,6 +20,7 @@ constexpr int kSliceBlock = 256;
 
 hipError_t sort_points_by_bin(void* tmp, size_t& tmp_bytes, const unsigned* keys_in, unsigned* keys_out,
                               int* idx_out, int n, int nbits, hipStream_t st);
+hipError_t exclusive_scan_ints(void* tmp, size_t& tmp_bytes, const int* in, int* out, int n, hipStream_t st);
 
 template <typename T>
 struct Corner8 {
@@ -64,6 +66,24 @@ __device__ __forceinline__ void trilinear_corners(T ix, T iy, T iz, int64_t D, i
   }
 }
 
+// The trilinear sum of one transform's 8 corners (corner order of trilinear_corners; corners outside
+// the volume count as zero).  All 8 loads are issued before the first is used: an outside corner
+// reads cell 0 (its offset) and its value is replaced by zero, so no load waits behind a branch.
+template <typename T>
+__device__ inline Cplx<T> corner_sum(const Cplx<T>* __restrict__ v, const Corner8<T>& cn) {
+  Cplx<T> a[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) a[k] = v[cn.off[k]];
+  Cplx<T> acc = {(T)0, (T)0};
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const T ax = cn.ok[k] ? a[k].x : (T)0, ay = cn.ok[k] ? a[k].y : (T)0;
+    acc.x += ax * cn.w[k];
+    acc.y += ay * cn.w[k];
+  }
+  return acc;
+}
+
 // One thread per (point, transform), the transform the slow grid index: all points of one
 // transform run before the next, so the 8 corner reads of consecutive points hit one transform's
 // volume (134 MB at vL = 256, resident in the 256 MB Infinity Cache) and every output row is written
@@ -88,18 +108,8 @@ __global__ void __launch_bounds__(kSliceBlock)
     Corner8<T> cn;
     trilinear_corners(ix, iy, iz, D, H, W, cn);
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      if (cn.ok[k]) {
-#pragma unroll
-        for (int u = 0; u < kFwdGroup; ++u) {
-          if (t0 + u < ntrans) {
-            const Cplx<T> a = vol[(int64_t)(t0 + u) * vsize + cn.off[k]];
-            acc[u].x += a.x * cn.w[k];
-            acc[u].y += a.y * cn.w[k];
-          }
-        }
-      }
-    }
+    for (int u = 0; u < kFwdGroup; ++u)
+      if (t0 + u < ntrans) acc[u] = corner_sum(vol + (int64_t)(t0 + u) * vsize, cn);
   } else {
     int64_t x = (int64_t)rint(ix), y = (int64_t)rint(iy), z = (int64_t)rint(iz);
     if (x >= 0 && x < W && y >= 0 && y < H && z >= 0 && z < D) {
@@ -249,6 +259,22 @@ __global__ void k_slice_inverse(const int* __restrict__ sidx, int* __restrict__ 
   if (s < n) spos[sidx[s]] = (int)s;
 }
 
+// Point j's values of every transform into its row, 8 transforms' loads in flight at a time
+// (a transform past the last rereads the last one's value and is not stored).
+template <typename T>
+__device__ inline void copy_row_values(const Cplx<T>* __restrict__ sig, Cplx<T>* __restrict__ row, int64_t j,
+                                       int64_t npts, int ntrans) {
+  constexpr int kBatch = 8;
+  for (int t0 = 0; t0 < ntrans; t0 += kBatch) {
+    Cplx<T> v[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) v[u] = sig[(int64_t)min(t0 + u, ntrans - 1) * npts + j];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u)
+      if (t0 + u < ntrans) row[2 + t0 + u] = v[u];
+  }
+}
+
 // Rows of the sorted points: [ix, iy, iz, 0 | value 0 .. value r-1] (RW = 2 + r float2 words), read in
 // the caller's order (coalesced) and written at the point's sorted position.
 template <typename T>
@@ -260,7 +286,7 @@ __global__ void __launch_bounds__(kSliceBlock)
   Cplx<T>* row = rows + (int64_t)spos[j] * rw;
   row[0] = {unnorm(grid[3 * j], W), unnorm(grid[3 * j + 1], H)};
   row[1] = {unnorm(grid[3 * j + 2], D), (T)0};
-  for (int t = 0; t < ntrans; ++t) row[2 + t] = sig[(int64_t)t * npts + j];
+  copy_row_values(sig, row, j, npts, ntrans);
 }
 
 // Rows as k_slice_rows for a slice plan: the coordinates come from the point's sorted record
@@ -277,7 +303,7 @@ __global__ void __launch_bounds__(kSliceBlock)
   Cplx<T>* row = rows + s * rw;
   row[0] = {p.x, p.y};
   row[1] = {p.z, (T)0};
-  for (int t = 0; t < ntrans; ++t) row[2 + t] = sig[(int64_t)t * npts + j];
+  copy_row_values(sig, row, j, npts, ntrans);
 }
 
 template <typename T>
@@ -297,6 +323,7 @@ __global__ void __launch_bounds__(kSliceBlock)
   const int64_t vsize = D * H * W;
   if (q >= vsize) return;
   const int t0 = blockIdx.y * RT;
+  const bool full = t0 + RT <= ntrans;
   const int64_t qx = q % W, qy = (q / W) % H, qz = q / (W * H);
   T ax[RT], ay[RT];
 #pragma unroll
@@ -320,12 +347,21 @@ __global__ void __launch_bounds__(kSliceBlock)
         const T wz = dz ? c1.x - fz : (fz + (T)1) - c1.x;
         w = wx * wy * wz;
       }
+      if (full) {  // every load issued before the sums (a guarded load would wait behind its branch)
 #pragma unroll
-      for (int u = 0; u < RT; ++u) {
-        if (t0 + u < ntrans) {
+        for (int u = 0; u < RT; ++u) {
           const Cplx<T> v = row[2 + t0 + u];
           ax[u] += w * v.x;
           ay[u] += w * v.y;
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < RT; ++u) {
+          if (t0 + u < ntrans) {
+            const Cplx<T> v = row[2 + t0 + u];
+            ax[u] += w * v.x;
+            ay[u] += w * v.y;
+          }
         }
       }
     }
@@ -335,11 +371,141 @@ __global__ void __launch_bounds__(kSliceBlock)
     if (t0 + u < ntrans) vol[(int64_t)(t0 + u) * vsize + q] = {ax[u], ay[u]};
 }
 
+// Points per key (end - start, zero for absent keys), the input of the scan that gives off[].
+__global__ void k_slice_counts(const int* __restrict__ start, const int* __restrict__ end, int* __restrict__ cnt,
+                               int64_t nkeys) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nkeys) cnt[k] = end[k] - start[k];
+  if (k == nkeys) cnt[k] = 0;
+}
+
+// Aggregated trilinear adjoint of a slice plan, tiled (FP32, ntrans <= 20).  A 256-thread
+// workgroup owns a 16 x 4 x 4 (x, y, z) block of output cells.  The points whose floor cells touch
+// the block, floor cells [x0 - 1, x0 + 15] x [y0 - 1, y0 + 3] x [z0 - 1, z0 + 3], are 25 runs of
+// consecutive keys (one per (y, z) floor row), so their rows are 25 contiguous ranges of the sorted
+// rows; taken in run order they form one list of ntot rows.  The list goes through LDS in chunks of
+// 256 rows (each run's part of a chunk copied by the whole workgroup, coalesced), and every cell adds,
+// chunk by chunk, the rows of its 8 floor cells that lie in the chunk, each times its trilinear
+// weight (the products k_slice_adjoint_gather forms).  Each row is read from memory about 1.7 times
+// (the halo of the block) instead of once per cell it reaches (8 times), and the sums run in a fixed
+// order (chunks, then floor cells d in {0,1}^3, then sorted rows): deterministic, and equal to the
+// per-cell gather to rounding.  A dense block (near the centre of the spectrum every slice passes)
+// simply takes more chunks.
+constexpr int kTileX = 16, kTileY = 4, kTileZ = 4;
+constexpr int kTileRuns = (kTileY + 1) * (kTileZ + 1);
+constexpr int kTileRowCap = kSliceBlock;  // rows per chunk: one per thread
+
+template <int RT>
+__global__ void __launch_bounds__(kSliceBlock)
+    k_slice_adjoint_tile(const float2* __restrict__ rows, const int* __restrict__ off, float2* __restrict__ vol,
+                         int64_t D, int64_t H, int64_t W, int ntrans, int rw) {
+  static_assert(kTileX * kTileY * kTileZ == kSliceBlock, "one cell per thread");
+  __shared__ float2 s_rows[kTileRowCap * (2 + RT)];
+  __shared__ int s_gstart[kTileRuns], s_len[kTileRuns], s_lbase[kTileRuns + 1];
+  __shared__ int s_src[kTileRowCap];
+  const int64_t nbx = W / kTileX, nby = H / kTileY;
+  const int64_t blk = blockIdx.x;
+  const int64_t x0 = (blk % nbx) * kTileX, y0 = ((blk / nbx) % nby) * kTileY, z0 = (blk / (nbx * nby)) * kTileZ;
+  const int tx = threadIdx.x % kTileX, ty = (threadIdx.x / kTileX) % kTileY, tz = threadIdx.x / (kTileX * kTileY);
+  auto key_of = [&](int64_t fx, int64_t fy, int64_t fz) { return ((fz + 1) * (H + 1) + (fy + 1)) * (W + 1) + (fx + 1); };
+  if (threadIdx.x < kTileRuns) {
+    const int64_t fy = y0 - 1 + threadIdx.x % (kTileY + 1), fz = z0 - 1 + threadIdx.x / (kTileY + 1);
+    const int a = off[key_of(x0 - 1, fy, fz)], b = off[key_of(x0 + kTileX - 1, fy, fz) + 1];
+    s_gstart[threadIdx.x] = a;
+    s_len[threadIdx.x] = b - a;
+  }
+  __syncthreads();
+  if (threadIdx.x <= kTileRuns) {  // exclusive prefix of the run lengths, one entry per thread
+    int acc = 0;
+    for (int i = 0; i < (int)threadIdx.x; ++i) acc += s_len[i];
+    s_lbase[threadIdx.x] = acc;
+  }
+  __syncthreads();
+  const int ntot = s_lbase[kTileRuns];
+  const int64_t qx = x0 + tx, qy = y0 + ty, qz = z0 + tz;
+  // this cell's 8 floor cells as ranges [lo, hi) of the run-ordered list
+  int lo[8], hi[8];
+#pragma unroll
+  for (int nb = 0; nb < 8; ++nb) {
+    const int dx = nb & 1, dy = (nb >> 1) & 1, dz = (nb >> 2) & 1;
+    const int64_t key = key_of(qx - dx, qy - dy, qz - dz);
+    const int run = (tz - dz + 1) * (kTileY + 1) + (ty - dy + 1);
+    const int shift = s_lbase[run] - s_gstart[run];
+    lo[nb] = off[key] + shift;
+    hi[nb] = off[key + 1] + shift;
+  }
+  const float inv_rw = 1.f / (float)rw;
+  float ax[RT], ay[RT];
+#pragma unroll
+  for (int u = 0; u < RT; ++u) ax[u] = ay[u] = 0.f;
+  for (int c0 = 0; c0 < ntot; c0 += kTileRowCap) {
+    const int n = min(kTileRowCap, ntot - c0);
+    __syncthreads();  // the previous chunk is consumed
+    if ((int)threadIdx.x < n) {
+      // chunk row t is list entry g = c0 + t; run r holds entries [s_lbase[r], s_lbase[r + 1])
+      const int gl = c0 + threadIdx.x;
+      int r = 0;
+#pragma unroll
+      for (int step = 16; step >= 1; step >>= 1)
+        if (r + step < kTileRuns && s_lbase[r + step] <= gl) r += step;
+      s_src[threadIdx.x] = s_gstart[r] + (gl - s_lbase[r]);
+    }
+    __syncthreads();
+    // the chunk as n * rw flat float2s, all loads issued before the first LDS write (a run's
+    // rows are contiguous, so consecutive lanes read consecutive addresses)
+    const int ne = n * rw;
+    float2 v[2 + RT];
+#pragma unroll
+    for (int i = 0; i < 2 + RT; ++i) {
+      const int e = threadIdx.x + i * kSliceBlock;
+      if (e < ne) {
+        const int row = (int)(((float)e + 0.5f) * inv_rw);  // exact: e + 0.5 is 0.5 / rw from a multiple
+        v[i] = rows[(int64_t)s_src[row] * rw + (e - row * rw)];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 2 + RT; ++i) {
+      const int e = threadIdx.x + i * kSliceBlock;
+      if (e < ne) s_rows[e] = v[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int nb = 0; nb < 8; ++nb) {
+      const int dx = nb & 1, dy = (nb >> 1) & 1, dz = (nb >> 2) & 1;
+      const int a = max(lo[nb], c0), b = min(hi[nb], c0 + n);
+      for (int g = a; g < b; ++g) {
+        const float2* row = s_rows + (g - c0) * rw;
+        const float2 c0v = row[0], c1v = row[1];
+        const float fx = floorf(c0v.x), fy = floorf(c0v.y), fz = floorf(c1v.x);
+        const float wx = dx ? c0v.x - fx : (fx + 1.f) - c0v.x;
+        const float wy = dy ? c0v.y - fy : (fy + 1.f) - c0v.y;
+        const float wz = dz ? c1v.x - fz : (fz + 1.f) - c1v.x;
+        const float w = wx * wy * wz;
+        // all RT values read (beyond ntrans they are other LDS words, summed and never written)
+#pragma unroll
+        for (int u = 0; u < RT; ++u) {
+          const float2 v = row[2 + u];
+          ax[u] += w * v.x;
+          ay[u] += w * v.y;
+        }
+      }
+    }
+  }
+  const int64_t vsize = D * H * W;
+  const int64_t q = (qz * H + qy) * W + qx;
+#pragma unroll
+  for (int u = 0; u < RT; ++u)
+    if (u < ntrans) vol[(int64_t)u * vsize + q] = make_float2(ax[u], ay[u]);
+}
+
 // Sort workspace of the gather adjoint, per device, grown on demand and kept.
 struct AdjWork {
   size_t cap_pts = 0, cap_keys = 0, cap_tmp = 0, cap_rows = 0;
   unsigned *keys = nullptr, *skeys = nullptr;
   int *sidx = nullptr, *spos = nullptr, *start = nullptr, *end = nullptr;
+  int* off = nullptr;  // slice plans: first sorted position of every key (nkeys + 1 entries)
+  int* cnt = nullptr;  // ... and the points per key it is scanned from
+  size_t cap_off = 0;
   void* tmp = nullptr;
   void* rows = nullptr;
 };
@@ -510,14 +676,7 @@ __global__ void __launch_bounds__(kSliceBlock)
   if (MODE == 0) {
     Corner8<T> cn;
     trilinear_corners(p.x, p.y, p.z, D, H, W, cn);
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      if (cn.ok[k]) {
-        const Cplx<T> a = v[cn.off[k]];
-        acc.x += a.x * cn.w[k];
-        acc.y += a.y * cn.w[k];
-      }
-    }
+    acc = corner_sum(v, cn);
   } else {
     const int64_t x = (int64_t)rint(p.x), y = (int64_t)rint(p.y), z = (int64_t)rint(p.z);
     if (x >= 0 && x < W && y >= 0 && y < H && z >= 0 && z < D) acc = v[(z * H + y) * W + x];
@@ -573,7 +732,18 @@ static int slice_plan_setpts(SlicePlan* pl, const void* grid, int64_t npts, int6
     COV3D_HIP(hipMalloc(&w.end, nkeys * sizeof(int)));
     w.cap_keys = nkeys;
   }
-  if (int rc = plan_grow(&w.tmp, w.cap_tmp, tmp_bytes)) return rc;
+  // the key scan of the tiled adjoint shares the sort's work area: size it for both before either runs
+  const bool want_off = sizeof(T) == 4 && mode == 0;
+  size_t scan_bytes = 0;
+  if (want_off) COV3D_HIP(exclusive_scan_ints(nullptr, scan_bytes, nullptr, nullptr, (int)(nkeys + 1), st));
+  if (int rc = plan_grow(&w.tmp, w.cap_tmp, std::max(tmp_bytes, scan_bytes))) return rc;
+  if (want_off && (size_t)(nkeys + 1) > w.cap_off) {
+    hipFree(w.off), hipFree(w.cnt);
+    w.off = w.cnt = nullptr, w.cap_off = 0;
+    COV3D_HIP(hipMalloc(&w.off, (nkeys + 1) * sizeof(int)));
+    COV3D_HIP(hipMalloc(&w.cnt, (nkeys + 1) * sizeof(int)));
+    w.cap_off = nkeys + 1;
+  }
   if (int rc = plan_grow(&pl->sp, pl->cap_sp, (size_t)npts * sizeof(SortedPoint<T>))) return rc;
   const unsigned nbp = (unsigned)ceil_div(npts, kSliceBlock);
   if (mode == 0)
@@ -594,6 +764,14 @@ static int slice_plan_setpts(SlicePlan* pl, const void* grid, int64_t npts, int6
   hipLaunchKernelGGL((k_slice_sorted_points<T>), dim3(nbp), dim3(kSliceBlock), 0, st, (const T*)grid, w.sidx,
                      (SortedPoint<T>*)pl->sp, D, H, W, npts);
   COV3D_LAUNCH_CHECK();
+  if (want_off) {
+    // first sorted position of every key, for the tiled adjoint: an exclusive scan of the counts
+    hipLaunchKernelGGL(k_slice_counts, dim3((unsigned)ceil_div(nkeys + 1, kSliceBlock)), dim3(kSliceBlock), 0, st,
+                       w.start, w.end, w.cnt, nkeys);
+    COV3D_LAUNCH_CHECK();
+    size_t sb = w.cap_tmp;
+    COV3D_HIP(exclusive_scan_ints(w.tmp, sb, w.cnt, w.off, (int)(nkeys + 1), st));
+  }
   pl->npts = npts, pl->D = D, pl->H = H, pl->W = W, pl->nkeys = nkeys, pl->mode = mode;
   pl->ready = true;
   return 0;
@@ -631,7 +809,16 @@ static int slice_plan_adjoint(SlicePlan* pl, const void* sig, void* vol, int64_t
   COV3D_LAUNCH_CHECK();
   constexpr int RT = 20;
   dim3 g((unsigned)ceil_div(D * H * W, kSliceBlock), (unsigned)ceil_div(ntrans, RT));
-  if (pl->mode == 0)
+  if (sizeof(T) == 4 && pl->mode == 0 && w.off && ntrans <= RT && W % kTileX == 0 && H % kTileY == 0 &&
+      D % kTileZ == 0) {
+    const unsigned nblk = (unsigned)((W / kTileX) * (H / kTileY) * (D / kTileZ));
+    if (ntrans == 1)
+      hipLaunchKernelGGL((k_slice_adjoint_tile<1>), dim3(nblk), dim3(kSliceBlock), 0, st, (const float2*)w.rows,
+                         w.off, (float2*)vol, D, H, W, 1, rw);
+    else
+      hipLaunchKernelGGL((k_slice_adjoint_tile<RT>), dim3(nblk), dim3(kSliceBlock), 0, st, (const float2*)w.rows,
+                         w.off, (float2*)vol, D, H, W, (int)ntrans, rw);
+  } else if (pl->mode == 0)
     hipLaunchKernelGGL((k_slice_adjoint_gather<T, 0, RT>), g, dim3(kSliceBlock), 0, st, (const Cplx<T>*)w.rows,
                        w.start, w.end, (Cplx<T>*)vol, D, H, W, (int)ntrans, rw);
   else
@@ -746,7 +933,7 @@ extern "C" int cov3d_slice_plan_destroy(void* plan) {
   auto* pl = static_cast<cov3d::SlicePlan*>(plan);
   cov3d::AdjWork& w = pl->w;
   hipFree(w.keys), hipFree(w.skeys), hipFree(w.sidx), hipFree(w.spos), hipFree(w.start), hipFree(w.end);
-  hipFree(w.tmp), hipFree(w.rows), hipFree(pl->sp);
+  hipFree(w.tmp), hipFree(w.rows), hipFree(w.off), hipFree(w.cnt), hipFree(pl->sp);
   delete pl;
   return 0;
 }

@@ -1,7 +1,9 @@
 """The slice plan (cov3d_slice_plan_*: points sorted once per setpts, sorted-order forward, shared
 sort for the gather adjoint) against the stateless kernels (cov3d_slice_forward / _adjoint), bit
 for bit, in both modes and precisions, with points outside the volume, and after a second setpts
-on the same plan (nufft_plan.py:94-200)."""
+on the same plan (nufft_plan.py:94-200).  The FP32 trilinear adjoint of a plan whose volume tiles
+into 16 x 4 x 4 blocks runs block-wise through LDS and sums in another order: it is checked to
+rounding, with dense clusters that need many LDS chunks per block."""
 
 import ctypes
 
@@ -55,6 +57,39 @@ def test_slice_plan_matches_stateless_kernels(dev, dtype, mode):
             assert torch.equal(back, ref_b)
     finally:
         lib.cov3d_slice_plan_destroy(h)
+
+
+@pytest.mark.parametrize("shape", [(32, 32, 32), (16, 8, 48)])
+def test_slice_plan_tiled_adjoint_matches_gather(dev, shape):
+    nat = pkg("_native")
+    D, H, W = shape
+    n = 20
+    g = torch.Generator(device=dev).manual_seed(11)
+    uniform = (torch.rand((7000, 3), device=dev, generator=g) * 2 - 1) * 1.05
+    cluster = torch.randn((40000, 3), device=dev, generator=g) * 0.12  # thousands of points per block
+    edge = torch.rand((3000, 3), device=dev, generator=g) * 0.1 + 0.93  # near and past the upper faces
+    grid = torch.cat([uniform, cluster, edge]).contiguous()
+    npts = grid.shape[0]
+    sig = torch.randn((n, npts), dtype=torch.complex64, device=dev, generator=g)
+    vol = torch.zeros((n, D, H, W), dtype=torch.complex64, device=dev)
+    _, ref = _stateless(nat, vol, grid, 0, sig)
+    lib = nat.load()
+    h = ctypes.c_void_p()
+    nat.check(lib.cov3d_slice_plan_create(ctypes.byref(h)))
+    try:
+        st = nat.stream_ptr(dev)
+        nat.check(lib.cov3d_slice_plan_setpts(h, grid.data_ptr(), npts, D, H, W, 0, 0, st))
+        back = torch.empty_like(ref)
+        nat.check(lib.cov3d_slice_plan_adjoint(h, sig.data_ptr(), back.data_ptr(), n, st))
+        torch.cuda.synchronize()
+    finally:
+        lib.cov3d_slice_plan_destroy(h)
+    # the block around the centre holds far more than one LDS chunk (256 rows)
+    cell = torch.floor((grid + 1) * torch.tensor([W - 1, H - 1, D - 1], device=dev) / 2)
+    centre = torch.tensor([(W - 1) // 2, (H - 1) // 2, (D - 1) // 2], device=dev)
+    assert int(((cell - centre).abs() <= 1).all(dim=1).sum()) > 4 * 256
+    scale = float(ref.abs().max())
+    torch.testing.assert_close(back, ref, rtol=1e-5, atol=1e-5 * scale)
 
 
 def test_discretized_plan_resyncs_on_setpts_and_volume_size(dev):
