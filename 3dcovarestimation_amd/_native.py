@@ -40,6 +40,8 @@ _SIGNATURES = {
     "cov3d_slice_plan_setpts": [c_vp, c_vp, c_i64, c_i64, c_i64, c_i64, c_int, c_int, c_vp],
     "cov3d_slice_plan_forward": [c_vp, c_vp, c_vp, c_i64, c_vp],
     "cov3d_slice_plan_adjoint": [c_vp, c_vp, c_vp, c_i64, c_vp],
+    "cov3d_slice_plan_forward_split": [c_vp, c_vp, c_vp, c_i64, c_vp, c_i64, c_vp],
+    "cov3d_slice_plan_adjoint_split": [c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_i64, c_vp],
     "cov3d_slice_plan_destroy": [c_vp],
     "cov3d_ml_grad_images_complex": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_i64, c_i64,
                                      c_i64, c_dbl, c_dbl, c_vp],

@@ -259,19 +259,22 @@ __global__ void k_slice_inverse(const int* __restrict__ sidx, int* __restrict__ 
   if (s < n) spos[sidx[s]] = (int)s;
 }
 
-// Point j's values of every transform into its row, 8 transforms' loads in flight at a time
-// (a transform past the last rereads the last one's value and is not stored).
+// Point j's values of every transform (rows of length npts) into its row, times sc when scaled,
+// 8 transforms' loads in flight at a time (a transform past the last rereads the last one's value
+// and is not stored).
 template <typename T>
 __device__ inline void copy_row_values(const Cplx<T>* __restrict__ sig, Cplx<T>* __restrict__ row, int64_t j,
-                                       int64_t npts, int ntrans) {
+                                       int64_t npts, int ntrans, T sc = (T)1, bool scaled = false) {
   constexpr int kBatch = 8;
   for (int t0 = 0; t0 < ntrans; t0 += kBatch) {
     Cplx<T> v[kBatch];
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) v[u] = sig[(int64_t)min(t0 + u, ntrans - 1) * npts + j];
 #pragma unroll
-    for (int u = 0; u < kBatch; ++u)
+    for (int u = 0; u < kBatch; ++u) {
+      if (scaled) v[u] = {v[u].x * sc, v[u].y * sc};
       if (t0 + u < ntrans) row[2 + t0 + u] = v[u];
+    }
   }
 }
 
@@ -291,10 +294,13 @@ __global__ void __launch_bounds__(kSliceBlock)
 
 // Rows as k_slice_rows for a slice plan: the coordinates come from the point's sorted record
 // (the plan keeps no grid pointer); the values are read in the caller's order (coalesced) and
-// the row is written at the point's sorted position.
+// the row is written at the point's sorted position.  The values of the caller's points [0, n_a)
+// come from sig_a (ntrans, n_a), the rest from sig_b (ntrans, npts - n_a); with a scale pointer
+// every value is multiplied by *scale.
 template <typename T>
 __global__ void __launch_bounds__(kSliceBlock)
-    k_slice_rows_plan(const Cplx<T>* __restrict__ sig, const SortedPoint<T>* __restrict__ sp,
+    k_slice_rows_plan(const Cplx<T>* __restrict__ sig_a, int64_t n_a, const Cplx<T>* __restrict__ sig_b,
+                      const T* __restrict__ scale, const SortedPoint<T>* __restrict__ sp,
                       const int* __restrict__ spos, Cplx<T>* __restrict__ rows, int64_t npts, int ntrans, int rw) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= npts) return;
@@ -303,7 +309,9 @@ __global__ void __launch_bounds__(kSliceBlock)
   Cplx<T>* row = rows + s * rw;
   row[0] = {p.x, p.y};
   row[1] = {p.z, (T)0};
-  copy_row_values(sig, row, j, npts, ntrans);
+  const bool in_a = j < n_a;
+  copy_row_values(in_a ? sig_a : sig_b, row, in_a ? j : j - n_a, in_a ? n_a : npts - n_a, ntrans,
+                  scale ? *scale : (T)1, scale != nullptr);
 }
 
 template <typename T>
@@ -661,11 +669,13 @@ __global__ void __launch_bounds__(kSliceBlock)
 }
 
 // The forward of k_slice_forward (same arithmetic, bit for bit) over the points in sorted order,
-// one thread per (sorted point, transform), the transform the slow grid index.
+// one thread per (sorted point, transform), the transform the slow grid index.  The caller's points
+// [0, n_a) write out_a (ntrans, n_a), the rest out_b (ntrans, npts - n_a).
 template <typename T, int MODE>
 __global__ void __launch_bounds__(kSliceBlock)
     k_slice_forward_sorted(const Cplx<T>* __restrict__ vol, const SortedPoint<T>* __restrict__ sp,
-                           Cplx<T>* __restrict__ out, int64_t D, int64_t H, int64_t W, int64_t npts) {
+                           Cplx<T>* __restrict__ out_a, int64_t n_a, Cplx<T>* __restrict__ out_b, int64_t D,
+                           int64_t H, int64_t W, int64_t npts) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= npts) return;
   const int64_t t = blockIdx.y;
@@ -681,7 +691,10 @@ __global__ void __launch_bounds__(kSliceBlock)
     const int64_t x = (int64_t)rint(p.x), y = (int64_t)rint(p.y), z = (int64_t)rint(p.z);
     if (x >= 0 && x < W && y >= 0 && y < H && z >= 0 && z < D) acc = v[(z * H + y) * W + x];
   }
-  out[t * npts + p.j] = acc;
+  if (p.j < n_a)
+    out_a[t * n_a + p.j] = acc;
+  else
+    out_b[t * (npts - n_a) + (p.j - n_a)] = acc;
 }
 
 struct SlicePlan {
@@ -778,21 +791,25 @@ static int slice_plan_setpts(SlicePlan* pl, const void* grid, int64_t npts, int6
 }
 
 template <typename T>
-static int slice_plan_forward(SlicePlan* pl, const void* vol, void* out, int64_t ntrans, hipStream_t st) {
+static int slice_plan_forward(SlicePlan* pl, const void* vol, void* out_a, int64_t n_a, void* out_b, int64_t ntrans,
+                              hipStream_t st) {
   if (pl->npts == 0) return 0;
   dim3 g((unsigned)ceil_div(pl->npts, kSliceBlock), (unsigned)ntrans);
   if (pl->mode == 0)
     hipLaunchKernelGGL((k_slice_forward_sorted<T, 0>), g, dim3(kSliceBlock), 0, st, (const Cplx<T>*)vol,
-                       (const SortedPoint<T>*)pl->sp, (Cplx<T>*)out, pl->D, pl->H, pl->W, pl->npts);
+                       (const SortedPoint<T>*)pl->sp, (Cplx<T>*)out_a, n_a, (Cplx<T>*)out_b, pl->D, pl->H, pl->W,
+                       pl->npts);
   else
     hipLaunchKernelGGL((k_slice_forward_sorted<T, 1>), g, dim3(kSliceBlock), 0, st, (const Cplx<T>*)vol,
-                       (const SortedPoint<T>*)pl->sp, (Cplx<T>*)out, pl->D, pl->H, pl->W, pl->npts);
+                       (const SortedPoint<T>*)pl->sp, (Cplx<T>*)out_a, n_a, (Cplx<T>*)out_b, pl->D, pl->H, pl->W,
+                       pl->npts);
   COV3D_LAUNCH_CHECK();
   return 0;
 }
 
 template <typename T>
-static int slice_plan_adjoint(SlicePlan* pl, const void* sig, void* vol, int64_t ntrans, hipStream_t st) {
+static int slice_plan_adjoint(SlicePlan* pl, const void* sig_a, int64_t n_a, const void* sig_b, const void* scale,
+                              void* vol, int64_t ntrans, hipStream_t st) {
   AdjWork& w = pl->w;
   const int64_t D = pl->D, H = pl->H, W = pl->W, npts = pl->npts;
   if (npts == 0) {
@@ -804,8 +821,9 @@ static int slice_plan_adjoint(SlicePlan* pl, const void* sig, void* vol, int64_t
   const int rw = 2 + (int)ntrans;
   if (int rc = plan_grow(&w.rows, w.cap_rows, (size_t)npts * rw * sizeof(Cplx<T>))) return rc;
   const unsigned nbp = (unsigned)ceil_div(npts, kSliceBlock);
-  hipLaunchKernelGGL((k_slice_rows_plan<T>), dim3(nbp), dim3(kSliceBlock), 0, st, (const Cplx<T>*)sig,
-                     (const SortedPoint<T>*)pl->sp, w.spos, (Cplx<T>*)w.rows, npts, (int)ntrans, rw);
+  hipLaunchKernelGGL((k_slice_rows_plan<T>), dim3(nbp), dim3(kSliceBlock), 0, st, (const Cplx<T>*)sig_a, n_a,
+                     (const Cplx<T>*)sig_b, (const T*)scale, (const SortedPoint<T>*)pl->sp, w.spos,
+                     (Cplx<T>*)w.rows, npts, (int)ntrans, rw);
   COV3D_LAUNCH_CHECK();
   constexpr int RT = 20;
   dim3 g((unsigned)ceil_div(D * H * W, kSliceBlock), (unsigned)ceil_div(ntrans, RT));
@@ -905,27 +923,43 @@ extern "C" int cov3d_slice_plan_setpts(void* plan, const void* grid, int64_t npt
   return cov3d::slice_plan_setpts<float>(pl, grid, npts, D, H, W, mode, st);
 }
 
-extern "C" int cov3d_slice_plan_forward(void* plan, const void* vol, void* out, int64_t ntrans, void* stream) {
+extern "C" int cov3d_slice_plan_forward_split(void* plan, const void* vol, void* out_a, int64_t n_a, void* out_b,
+                                              int64_t ntrans, void* stream) {
   COV3D_CHECK_ARG(plan, "NULL plan");
   auto* pl = static_cast<cov3d::SlicePlan*>(plan);
   COV3D_CHECK_ARG(pl->ready, "slice plan: setpts first");
   COV3D_CHECK_ARG(ntrans >= 1 && ntrans < 65536, "bad ntrans");
+  COV3D_CHECK_ARG(n_a >= 0 && n_a <= pl->npts, "n_a must lie in [0, npts]");
   if (pl->npts == 0) return 0;
-  COV3D_CHECK_ARG(vol && out, "NULL pointer");
+  COV3D_CHECK_ARG(vol && (out_a || n_a == 0) && (out_b || n_a == pl->npts), "NULL pointer");
   hipStream_t st = cov3d::as_stream(stream);
-  if (pl->dtype == 1) return cov3d::slice_plan_forward<double>(pl, vol, out, ntrans, st);
-  return cov3d::slice_plan_forward<float>(pl, vol, out, ntrans, st);
+  if (pl->dtype == 1) return cov3d::slice_plan_forward<double>(pl, vol, out_a, n_a, out_b, ntrans, st);
+  return cov3d::slice_plan_forward<float>(pl, vol, out_a, n_a, out_b, ntrans, st);
+}
+
+extern "C" int cov3d_slice_plan_forward(void* plan, const void* vol, void* out, int64_t ntrans, void* stream) {
+  COV3D_CHECK_ARG(plan, "NULL plan");
+  return cov3d_slice_plan_forward_split(plan, vol, out, static_cast<cov3d::SlicePlan*>(plan)->npts, nullptr, ntrans,
+                                        stream);
+}
+
+extern "C" int cov3d_slice_plan_adjoint_split(void* plan, const void* sig_a, int64_t n_a, const void* sig_b,
+                                              const void* scale, void* vol_out, int64_t ntrans, void* stream) {
+  COV3D_CHECK_ARG(plan, "NULL plan");
+  auto* pl = static_cast<cov3d::SlicePlan*>(plan);
+  COV3D_CHECK_ARG(pl->ready, "slice plan: setpts first");
+  COV3D_CHECK_ARG(ntrans >= 1 && ntrans < 65536, "bad ntrans");
+  COV3D_CHECK_ARG(n_a >= 0 && n_a <= pl->npts, "n_a must lie in [0, npts]");
+  COV3D_CHECK_ARG(vol_out && (sig_a || n_a == 0) && (sig_b || n_a == pl->npts), "NULL pointer");
+  hipStream_t st = cov3d::as_stream(stream);
+  if (pl->dtype == 1) return cov3d::slice_plan_adjoint<double>(pl, sig_a, n_a, sig_b, scale, vol_out, ntrans, st);
+  return cov3d::slice_plan_adjoint<float>(pl, sig_a, n_a, sig_b, scale, vol_out, ntrans, st);
 }
 
 extern "C" int cov3d_slice_plan_adjoint(void* plan, const void* sig, void* vol_out, int64_t ntrans, void* stream) {
   COV3D_CHECK_ARG(plan, "NULL plan");
-  auto* pl = static_cast<cov3d::SlicePlan*>(plan);
-  COV3D_CHECK_ARG(pl->ready, "slice plan: setpts first");
-  COV3D_CHECK_ARG(ntrans >= 1 && ntrans < 65536, "bad ntrans");
-  COV3D_CHECK_ARG(vol_out && (sig || pl->npts == 0), "NULL pointer");
-  hipStream_t st = cov3d::as_stream(stream);
-  if (pl->dtype == 1) return cov3d::slice_plan_adjoint<double>(pl, sig, vol_out, ntrans, st);
-  return cov3d::slice_plan_adjoint<float>(pl, sig, vol_out, ntrans, st);
+  return cov3d_slice_plan_adjoint_split(plan, sig, static_cast<cov3d::SlicePlan*>(plan)->npts, nullptr, nullptr,
+                                        vol_out, ntrans, stream);
 }
 
 extern "C" int cov3d_slice_plan_destroy(void* plan) {

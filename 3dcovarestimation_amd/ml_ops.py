@@ -310,34 +310,38 @@ class MLCostDiscretized(torch.autograd.Function):
         if half:
             pix = half_plane_pixels(L, dev)
             npix = pix.numel()
-            hp = _disc_companion(plan, "_disc_half")
             gfull = plan._grid_flat.detach().reshape(B, L * L, 3)
-            hp.set_grid_flat(gfull[:, pix.long()])
-            c = hp.execute_forward_flat(vhat.detach()).reshape(r, B, npix)
+            # boundary pairs: h in the half plane, o = its mirror pixel
+            in_h = torch.zeros(L * L, dtype=torch.bool, device=dev)
+            in_h[pix.long()] = True
+            bh = bd & in_h.unsqueeze(0)
+            nb = int(bh.sum(dim=1).max())
+            gh = gfull[:, pix.long()].reshape(-1, 3)
+            if nb > 0:
+                order = torch.sort(bh.to(torch.int8), dim=1, descending=True, stable=True).indices[:, :nb]
+                valid = torch.gather(bh, 1, order)  # (B, nb)
+                oidx = _mirror_index(L, dev)[order]  # mirror pixels (full-plane indices)
+                go = torch.gather(gfull, 1, oidx.unsqueeze(-1).expand(-1, -1, 3)).clone()
+                go[~valid] = 4.0  # outside the volume: padded entries slice to zero
+                gh = torch.cat([gh, go.reshape(-1, 3)])
+            # one plan over the half-plane points followed by the mirror points of the boundary
+            # pairs: one sort, one forward pass, and one adjoint that sums both sets
+            hp = _disc_companion(plan, "_disc_hb")
+            hp.set_grid_flat(gh)
+            c, co = hp.execute_forward_split(vhat.detach(), B * npix)
+            c = c.reshape(r, B, npix)
             res = ml_forward(c, yhat, L, nv_s, ctf=ctf, mean_const=False, precision=precision, want_m=True,
                              want_minv=False, want_b=True, pix_map=pix)
             M = res["m"].double()
             bvec = res["b"].double()
             Mc = torch.complex(M, torch.zeros_like(M))
             bc = torch.complex(bvec, torch.zeros_like(bvec))
-            # boundary pairs: h in the half plane, o = its mirror pixel
-            in_h = torch.zeros(L * L, dtype=torch.bool, device=dev)
-            in_h[pix.long()] = True
-            bh = bd & in_h.unsqueeze(0)
-            nb = int(bh.sum(dim=1).max())
             bnd = None
             if nb > 0:
-                order = torch.sort(bh.to(torch.int8), dim=1, descending=True, stable=True).indices[:, :nb]
-                valid = torch.gather(bh, 1, order)  # (B, nb)
                 hpos = torch.full((L * L,), -1, dtype=torch.long, device=dev)
                 hpos[pix.long()] = torch.arange(npix, device=dev)
                 hidx = hpos[order].clamp(min=0)  # positions in the half set
-                oidx = _mirror_index(L, dev)[order]  # mirror pixels (full-plane indices)
-                go = torch.gather(gfull, 1, oidx.unsqueeze(-1).expand(-1, -1, 3)).clone()
-                go[~valid] = 4.0  # outside the volume: padded entries slice to zero
-                bp = _disc_companion(plan, "_disc_bnd")
-                bp.set_grid_flat(go)
-                co = bp.execute_forward_flat(vhat.detach()).reshape(r, B, nb)
+                co = co.reshape(r, B, nb)
                 vf = valid.to(torch.float32)
                 wh = torch.gather(scale, 1, order) * vf / float(L * L)
                 wo = torch.gather(scale, 1, oidx) * vf / float(L * L)
@@ -409,13 +413,14 @@ class MLCostDiscretized(torch.autograd.Function):
             _native.check(_native.load().cov3d_ml_grad_images_complex(
                 c.data_ptr(), _native.ptr(ctf_t), yh.data_ptr(), z1.data_ptr(), m1.data_ptr(), z2.data_ptr(),
                 m2.data_ptr(), None, 0, g.data_ptr(), B, r, L, float(nv_s), 1.0 / B, _native.stream_ptr(c.device)))
-            grad_v = ctx.plan._adjoint(g.reshape(r, -1), aggregated=True)
+            grad_v = ctx.plan.adjoint_split(g.reshape(r, -1), None, scale=grad_out)
         else:
             # A +-k pair of mirrored slices contributes g(k) + conj(g(-k)) = 2 g1(k) once the volume
             # gradient reaches the real eigenvolumes: the imaginary pass cancels within the pair, so
             # the half plane needs the real-coefficient pass only, with the pair weight.
             ml_grad_images(c, yhat, a.to(torch.float32), A1, L, nv_s, 1.0 / B, ctf=ctf_arg, out=g, pix_map=pix)
             bnd = ctx.bnd
+            gov = None
             if bnd is not None:
                 hidx, valid, co, wh, wo, order, oidx = bnd
                 yf = yhat.reshape(B, -1)
@@ -437,13 +442,11 @@ class MLCostDiscretized(torch.autograd.Function):
                 flat = (torch.arange(B, device=g.device).unsqueeze(1) * npix + hidx)[valid]
                 gv = g.view(r, B * npix)
                 gv[:, flat] = gh[:, valid]
-                gov = exact(qo, yo, wo).contiguous()
-                grad_o = ctx.plan._disc_bnd._adjoint(gov.reshape(r, -1), aggregated=True)
-            grad_v = ctx.plan._disc_half._adjoint(g.reshape(r, -1), aggregated=True)
-            if bnd is not None:
-                grad_v = grad_v + grad_o
-        grad_v = grad_v.reshape(shape) * grad_out
-        return grad_v, None, None, None, None, None, None, None
+                gov = exact(qo, yo, wo).contiguous().reshape(r, -1)
+            # the half-plane values and the mirror values of the boundary pairs into one volume,
+            # both scaled by the upstream gradient on the device
+            grad_v = ctx.plan._disc_hb.adjoint_split(g.reshape(r, -1), gov, scale=grad_out)
+        return grad_v.reshape(shape), None, None, None, None, None, None, None
 
 
 def _disc_companion(plan, name: str):

@@ -22,7 +22,7 @@ import logging
 import math
 from dataclasses import dataclass
 from enum import Enum
-from typing import Any, Tuple, Union
+from typing import Any, Optional, Tuple, Union
 
 import torch
 
@@ -157,6 +157,58 @@ class NufftPlanDiscretized(BaseNufftPlan):
         h = self._slice_plan(D, H, W, vol.dtype, vol.device)
         _native.check(_native.load().cov3d_slice_plan_forward(h, vol.data_ptr(), out.data_ptr(), n,
                                                               _native.stream_ptr(vol.device)))
+        return out
+
+    def execute_forward_split(self, volume: torch.Tensor, n_a: int) -> Tuple[torch.Tensor, torch.Tensor]:
+        """volume (n, D, H, W) complex -> the slice values at the plan's points [0, n_a) as (n, n_a)
+        and at the rest as (n, npts - n_a): the points of two sets, set as one flat grid, sliced in
+        one sorted pass (``cov3d_slice_plan_forward_split``)."""
+        vol = self._as_complex_volume(volume)
+        if vol.ndim == 3:
+            vol = vol.unsqueeze(0)
+        _native.require_device(vol, "volume")
+        vol = _native.dense(vol)
+        n, D, H, W = vol.shape
+        npts = self._grid_flat.shape[0]
+        if not 0 <= n_a <= npts:
+            raise ValueError("n_a must lie in [0, npts]")
+        out_a = torch.empty((n, n_a), dtype=vol.dtype, device=vol.device)
+        out_b = torch.empty((n, npts - n_a), dtype=vol.dtype, device=vol.device)
+        if npts == 0:
+            return out_a, out_b
+        h = self._slice_plan(D, H, W, vol.dtype, vol.device)
+        _native.check(_native.load().cov3d_slice_plan_forward_split(
+            h, vol.data_ptr(), out_a.data_ptr() if n_a else None, n_a,
+            out_b.data_ptr() if n_a < npts else None, n, _native.stream_ptr(vol.device)))
+        return out_a, out_b
+
+    def adjoint_split(self, sig_a: torch.Tensor, sig_b: Optional[torch.Tensor],
+                      scale: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """Aggregated adjoint of values given in two parts, sig_a (n, n_a) for the plan's points
+        [0, n_a) and sig_b (n, npts - n_a) for the rest, summed into one (n, vL, vL, vL) volume; with
+        ``scale`` (one real on the device) every value is multiplied by it first
+        (``cov3d_slice_plan_adjoint_split``)."""
+        L = self.sz[0]
+        vL = L * self.upsample_factor
+        sig_a = _native.dense(sig_a if sig_a.is_complex() else sig_a.to(_complex_of(sig_a.dtype)))
+        _native.require_device(sig_a, "signal")
+        n = sig_a.shape[0]
+        sig_a = sig_a.reshape(n, -1)
+        n_a = sig_a.shape[1]
+        npts = self._grid_flat.shape[0]
+        if sig_b is not None:
+            sig_b = _native.dense(sig_b.to(sig_a.dtype)).reshape(n, -1)
+        nb = 0 if sig_b is None else sig_b.shape[1]
+        if n_a + nb != npts:
+            raise ValueError("signal sizes do not match the plan's points")
+        sc = None
+        if scale is not None:
+            sc = _native.dense(scale.detach().to(device=sig_a.device, dtype=_real_of(sig_a.dtype)).reshape(1))
+        out = torch.empty((n, vL, vL, vL), dtype=sig_a.dtype, device=sig_a.device)
+        h = self._slice_plan(vL, vL, vL, sig_a.dtype, sig_a.device)
+        _native.check(_native.load().cov3d_slice_plan_adjoint_split(
+            h, sig_a.data_ptr() if n_a else None, n_a, sig_b.data_ptr() if nb else None, _native.ptr(sc),
+            out.data_ptr(), n, _native.stream_ptr(sig_a.device)))
         return out
 
     def _as_complex_volume(self, volume) -> torch.Tensor:

@@ -127,9 +127,12 @@ int cov3d_slice_adjoint(const void* sig, const void* grid, void* vol_out,
 /* Slice plan: the points of NufftPlanDiscretized.setpts (nufft_plan.py:94-118) sorted once
  * by the grid cell they reach, shared by the forward (nufft_plan.py:120-169) and the aggregated
  * adjoint (:171-200).  setpts takes the same (npts, 3) grid as cov3d_slice_forward and keeps no
- * pointer to it; forward / adjoint then take the volumes and values only and give the results of
- * cov3d_slice_forward / cov3d_slice_adjoint (aggregated) bit for bit.  A plan stays on the device
- * of its first setpts; it is not thread-safe. */
+ * pointer to it; forward / adjoint then take the volumes and values only.  The forward gives the
+ * result of cov3d_slice_forward bit for bit; the adjoint that of cov3d_slice_adjoint (aggregated)
+ * bit for bit, except for complex64 trilinear plans whose volume tiles into 16 x 4 x 4 blocks
+ * (W % 16 == H % 4 == D % 4 == 0, ntrans <= 20), which sum block-wise through LDS in another
+ * fixed order: equal to rounding and still deterministic.  A plan stays on the device of its
+ * first setpts; it is not thread-safe. */
 int cov3d_slice_plan_create(void** plan);
 int cov3d_slice_plan_setpts(void* plan, const void* grid, int64_t npts, int64_t D,
                             int64_t H, int64_t W, int mode, int dtype, void* stream);
@@ -137,6 +140,17 @@ int cov3d_slice_plan_forward(void* plan, const void* vol, void* out, int64_t ntr
                              void* stream);
 int cov3d_slice_plan_adjoint(void* plan, const void* sig, void* vol_out, int64_t ntrans,
                              void* stream);
+/* The forward / adjoint of a plan whose points are two sets set as one (no counterpart in the
+ * reference, which slices each point set with its own grid_sample call): points [0, n_a) of the
+ * setpts grid go with out_a / sig_a (ntrans, n_a), points [n_a, npts) with out_b / sig_b
+ * (ntrans, npts - n_a); the adjoint sums both sets into one volume.  scale, when not NULL, is a
+ * real of the plan's precision on the device that multiplies every value before the adjoint
+ * sums (the upstream gradient of a backward pass, read without a host synchronisation).  With
+ * n_a = npts and scale NULL they are cov3d_slice_plan_forward / _adjoint. */
+int cov3d_slice_plan_forward_split(void* plan, const void* vol, void* out_a, int64_t n_a,
+                                   void* out_b, int64_t ntrans, void* stream);
+int cov3d_slice_plan_adjoint_split(void* plan, const void* sig_a, int64_t n_a, const void* sig_b,
+                                   const void* scale, void* vol_out, int64_t ntrans, void* stream);
 int cov3d_slice_plan_destroy(void* plan);
 
 /* Gradient of cov3d_slice_forward with respect to its grid: for each point j,

@@ -92,6 +92,37 @@ def test_slice_plan_tiled_adjoint_matches_gather(dev, shape):
     torch.testing.assert_close(back, ref, rtol=1e-5, atol=1e-5 * scale)
 
 
+@pytest.mark.parametrize("dtype", [torch.complex64, torch.complex128])
+def test_slice_plan_split_matches_whole(dev, dtype):
+    """cov3d_slice_plan_forward_split / _adjoint_split (two point sets set as one) against the plain
+    forward / adjoint of the same plan: the two outputs are the whole output cut at n_a, and the
+    adjoint of the two signal parts, scaled by a device scalar, is the adjoint of the joined signal
+    times that scalar (0.5: exact in binary, so bit for bit)."""
+    npl = pkg("nufft_plan")
+    rdt = torch.float64 if dtype == torch.complex128 else torch.float32
+    g = torch.Generator(device=dev).manual_seed(5)
+    L, us, n = 16, 2, 4
+    vL = L * us
+    npts, n_a = 6000, 4700
+    plan = npl.NufftPlanDiscretized((L,) * 3, upsample_factor=us, mode="bilinear")
+    plan.set_grid_flat(((torch.rand((npts, 3), dtype=rdt, device=dev, generator=g) * 2 - 1) * 1.05).contiguous())
+    vol = torch.randn((n, vL, vL, vL), dtype=dtype, device=dev, generator=g)
+    whole = plan.execute_forward_flat(vol)
+    a, b = plan.execute_forward_split(vol, n_a)
+    assert torch.equal(a, whole[:, :n_a]) and torch.equal(b, whole[:, n_a:])
+    a0, b0 = plan.execute_forward_split(vol, npts)
+    assert torch.equal(a0, whole) and b0.shape == (n, 0)
+    sig = torch.randn((n, npts), dtype=dtype, device=dev, generator=g)
+    ref = plan._adjoint(sig, aggregated=True)
+    got = plan.adjoint_split(sig[:, :n_a].contiguous(), sig[:, n_a:].contiguous())
+    assert torch.equal(got, ref)
+    half = torch.tensor(0.5, device=dev)
+    got = plan.adjoint_split(sig[:, :n_a].contiguous(), sig[:, n_a:].contiguous(), scale=half)
+    assert torch.equal(got, ref * 0.5)
+    with pytest.raises(ValueError):
+        plan.adjoint_split(sig[:, :n_a].contiguous(), None)
+
+
 def test_discretized_plan_resyncs_on_setpts_and_volume_size(dev):
     """NufftPlanDiscretized re-sorts after setpts and for a volume of another size."""
     npl = pkg("nufft_plan")
