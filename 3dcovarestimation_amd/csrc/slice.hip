@@ -62,15 +62,26 @@ __device__ __forceinline__ void trilinear_corners(T ix, T iy, T iz, int64_t D, i
   for (int k = 0; k < 8; ++k) {
     cn.w[k] = wxs[k] * wys[k] * wzs[k];
     cn.ok[k] = xs[k] >= 0 && xs[k] < W && ys[k] >= 0 && ys[k] < H && zs[k] >= 0 && zs[k] < D;
-    cn.off[k] = cn.ok[k] ? (zs[k] * H + ys[k]) * W + xs[k] : 0;
+    // a corner outside the volume gets the offset of the nearest cell inside (a valid address
+    // near the point's own cells; its value is not used)
+    const int64_t xc = xs[k] < 0 ? 0 : (xs[k] >= W ? W - 1 : xs[k]);
+    const int64_t yc = ys[k] < 0 ? 0 : (ys[k] >= H ? H - 1 : ys[k]);
+    const int64_t zc = zs[k] < 0 ? 0 : (zs[k] >= D ? D - 1 : zs[k]);
+    cn.off[k] = (zc * H + yc) * W + xc;
   }
 }
 
 // The trilinear sum of one transform's 8 corners (corner order of trilinear_corners; corners outside
 // the volume count as zero).  All 8 loads are issued before the first is used: an outside corner
-// reads cell 0 (its offset) and its value is replaced by zero, so no load waits behind a branch.
+// reads its nearest inside cell and its value is replaced by zero, so no load waits behind a
+// branch.  A point with no corner inside reads nothing (many such points would otherwise all
+// read the same edge cell).
 template <typename T>
 __device__ inline Cplx<T> corner_sum(const Cplx<T>* __restrict__ v, const Corner8<T>& cn) {
+  bool any = false;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) any = any || cn.ok[k];
+  if (!any) return {(T)0, (T)0};
   Cplx<T> a[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) a[k] = v[cn.off[k]];
