@@ -34,6 +34,7 @@ import torch
 from . import ml_ops
 from .covar import Covar
 from .fsc_utils import covar_fsc, expand_fourier_shell
+from .newton_opt import BlockwiseLBFGS
 from .nufft_plan import NufftPlan, NufftPlanDiscretized
 from .projection_funcs import centered_fft2, centered_fft3, vol_forward
 from .wiener_coords import compute_latentMAP_batch
@@ -574,8 +575,26 @@ class CovarPoseTrainer(CovarTrainer):
 
     def restart_optimizer(self) -> None:
         super().restart_optimizer()
-        self.pose_optimizer = torch.optim.SparseAdam([{"params": list(self.pose.parameters()),
-                                                       "lr": self.pose_lr_ratio * self.lr}])
+        if self.rotation_est_method == "SGD":
+            self.pose_optimizer = torch.optim.SparseAdam([{"params": list(self.pose.parameters()),
+                                                           "lr": self.pose_lr_ratio * self.lr}])
+        elif self.rotation_est_method == "LBFGS":
+            # covar_sgd.py:980-981: a row-wise L-BFGS on the rotation vectors, steps limited to 0.1
+            self.pose_optimizer = BlockwiseLBFGS([{"params": list(self.pose.parameters()), "lr": 1,
+                                                   "step_size_limit": 1e-1}])
+        else:
+            raise ValueError(f"unknown rotation_est_method {self.rotation_est_method!r}")
+
+    def set_pose_module(self, pose) -> None:
+        """Takes a new pose module (covar_sgd.py:645-658), e.g. the one merged from two half-set
+        modules; offsets and contrasts stay out of the gradient as in ``__init__``."""
+        self.pose = pose.to(self.device)
+        for prm in self.pose.offsets.parameters():
+            prm.requires_grad = False
+        if self.pose.use_contrast:
+            for prm in self.pose.contrasts.parameters():
+                prm.requires_grad = False
+        self._updated_idx = torch.zeros(len(self.dataset), device=self.device)
 
     def train_epochs(self, max_epochs: int, **training_kwargs) -> None:
         while self.downsample_factor >= 0:
@@ -612,8 +631,6 @@ def trainCovar(covar_model, dataset, batch_size, optimize_pose=False, mean_model
     from .dataset import create_dataloader
 
     use_halfsets = kwargs.pop("use_halfsets")
-    if optimize_pose and use_halfsets:
-        raise NotImplementedError("pose refinement with half-sets (PoseModule.split_module) is not built")
     num_reg_update_iters = kwargs.pop("num_reg_update_iters", None) or 0
     num_epochs = kwargs.pop("max_epochs")
     device = covar_model.device
@@ -636,9 +653,19 @@ def trainCovar(covar_model, dataset, batch_size, optimize_pose=False, mean_model
     with torch.no_grad():
         covar_copy.set_vectors(torch.randn_like(covar_copy.get_vectors_spatial_domain()) *
                                float(covar_model.pixel_var_estimate) ** 0.5)
-    half1, half2 = dataset.half_split()
-    t1 = CovarTrainer(covar_model, create_dataloader(half1, batch_size=batch_size), device, savepath, gt_data=gt_data)
-    t2 = CovarTrainer(covar_copy, create_dataloader(half2, batch_size=batch_size), device, None, gt_data=gt_data)
+    half1, half2, permutation = dataset.half_split()
+    dl1 = create_dataloader(half1, batch_size=batch_size)
+    dl2 = create_dataloader(half2, batch_size=batch_size)
+    if not optimize_pose:
+        t1 = CovarTrainer(covar_model, dl1, device, savepath, gt_data=gt_data)
+        t2 = CovarTrainer(covar_copy, dl2, device, None, gt_data=gt_data)
+    else:
+        # covar_sgd.py:1557-1572: each half-set refines the poses of its own images, with its own
+        # copy of the mean
+        pose1, pose2 = pose.split_module(permutation)
+        gt1, gt2 = gt_data.half_split(permutation) if hasattr(gt_data, "half_split") else (gt_data, gt_data)
+        t1 = CovarPoseTrainer(covar_model, dl1, device, mean_model, pose1, savepath, gt_data=gt1)
+        t2 = CovarPoseTrainer(covar_copy, dl2, device, copy.deepcopy(mean_model), pose2, None, gt_data=gt2)
     t1.setup_training(**kwargs)
     t2.setup_training(**kwargs)
     for _ in range(num_reg_update_iters):
@@ -647,6 +674,11 @@ def trainCovar(covar_model, dataset, batch_size, optimize_pose=False, mean_model
         update_fourier_reg(t1, t2)
     t2.complete_training()
     t1.train_data = create_dataloader(dataset, batch_size=batch_size)
+    if optimize_pose:
+        # covar_sgd.py:1603-1607: the full set continues from the two half-sets' poses
+        pose.update_from_modules(pose1, pose2, permutation)
+        t1.set_pose_module(pose)
+        t1.gt_data = gt_data
     t1.train(max_epochs=num_epochs, **kwargs)
     return t1
 

@@ -263,11 +263,14 @@ class CovarDataset:
         sub._in_spatial_domain = self._in_spatial_domain
         return sub
 
-    def half_split(self) -> Tuple["CovarDataset", "CovarDataset"]:
-        """Even/odd split into two half-sets."""
+    def half_split(self, permute: bool = True) -> Tuple["CovarDataset", "CovarDataset", torch.Tensor]:
+        """Two half-sets, the images permutation[:n // 2] and permutation[n // 2:] of a random
+        permutation (identity order with ``permute=False``), and that permutation
+        (dataset.py:302-320)."""
         n = len(self)
-        dev = self.device
-        return self._subset(torch.arange(0, n, 2, device=dev)), self._subset(torch.arange(1, n, 2, device=dev))
+        permutation = torch.randperm(n) if permute else torch.arange(n)
+        idx = permutation.to(self.device)
+        return self._subset(idx[: n // 2]), self._subset(idx[n // 2:]), permutation
 
     def filter_gain(self, batch_size: int = 4096) -> torch.Tensor:
         """Mean CTF^2 over the stack, (L, L)."""

@@ -141,6 +141,63 @@ class PoseModule(torch.nn.Module):
             else:
                 self.contrasts.weight[idx] = contrasts
 
+    def _rows(self, idx: torch.Tensor) -> "PoseModule":
+        """A new module on this module's device holding copies of rows ``idx``."""
+        idx = idx.to(self.rotvec.weight.device)
+        m = PoseModule(self.rotvec.weight.data[idx].detach().clone(), self.offsets.weight.data[idx].detach().clone(),
+                       self.resolution, dtype=self.dtype, use_contrast=self.use_contrast)
+        if self.use_contrast:
+            m.set_contrasts(self.contrasts.weight.data[idx].detach().clone())
+        return m.to(self.device)
+
+    def split_module(self, permutation: Optional[torch.Tensor] = None) -> Tuple["PoseModule", "PoseModule"]:
+        """Two modules holding the rows permutation[:n // 2] and permutation[n // 2:] (identity order
+        when no permutation is given), for the two half-sets of the same split (poses.py:350-389)."""
+        n = self.rotvec.weight.shape[0]
+        if permutation is None:
+            permutation = torch.arange(n)
+        permutation = torch.as_tensor(permutation, dtype=torch.long)
+        return self._rows(permutation[: n // 2]), self._rows(permutation[n // 2:])
+
+    @staticmethod
+    def _joined(module1: "PoseModule", module2: "PoseModule", permutation: torch.Tensor):
+        """The rows of two split modules put back in the original order: row permutation[k] of the
+        result is row k of the two modules stacked."""
+        if module1.use_contrast != module2.use_contrast:
+            raise ValueError("both modules must have the same use_contrast setting")
+        dev = module1.rotvec.weight.device
+        perm = torch.as_tensor(permutation, dtype=torch.long).to(dev)
+
+        def place(a, b):
+            stacked = torch.cat([a.detach(), b.detach().to(dev)], dim=0)
+            out = torch.empty_like(stacked)
+            out[perm] = stacked
+            return out
+
+        rot = place(module1.rotvec.weight.data, module2.rotvec.weight.data)
+        off = place(module1.offsets.weight.data, module2.offsets.weight.data)
+        con = (place(module1.contrasts.weight.data, module2.contrasts.weight.data) if module1.use_contrast
+               else None)
+        return rot, off, con
+
+    @staticmethod
+    def merge_modules(module1: "PoseModule", module2: "PoseModule", permutation: torch.Tensor) -> "PoseModule":
+        """The module that ``split_module(permutation)`` split into module1 and module2
+        (poses.py:391-429)."""
+        rot, off, con = PoseModule._joined(module1, module2, permutation)
+        m = PoseModule(rot, off, module1.resolution, dtype=module1.dtype, use_contrast=module1.use_contrast)
+        if con is not None:
+            m.set_contrasts(con)
+        return m.to(module1.device)
+
+    def update_from_modules(self, module1: "PoseModule", module2: "PoseModule", permutation: torch.Tensor) -> None:
+        """Takes the rows of two split modules back in place (poses.py:431-461)."""
+        rot, off, con = PoseModule._joined(module1, module2, permutation)
+        self.set_rotvecs(rot.to(self.rotvec.weight.device))
+        self.set_offsets(off.to(self.offsets.weight.device))
+        if con is not None and self.use_contrast:
+            self.set_contrasts(con.to(self.contrasts.weight.device))
+
 
 def estimate_image_offsets_newton(images: torch.Tensor, reference: torch.Tensor, mask: Optional[torch.Tensor] = None,
                                   init_offsets: Optional[torch.Tensor] = None, in_fourier_domain: bool = False,

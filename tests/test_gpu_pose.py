@@ -165,3 +165,79 @@ def test_pose_trainer_refines_rotations(dev, disc):
     print(f"mean rotation error {before:.3f} -> {after:.3f} degrees; costs {tr.training_log['cost_val']}")
     assert after < 0.7 * before
     assert torch.isfinite(pose.get_offsets()).all() and pose.get_offsets().abs().max() < 2.0
+
+
+def _perturbed_problem(dev, n=1024, L=32, seed=6, deg=3.0):
+    ds = pkg("dataset")
+    ps = pkg("poses")
+    cv = pkg("covar")
+    cfg = ds.SyntheticConfig(L, n, 2, [4.0, 1.0], 3.0, 2.0, (0.8, 2.5), shift_px=0.0, seed=seed)
+    data = ds.make_synthetic(cfg, dev, subtract_mean=False)
+    R_true = data.dataset.rotations.clone()
+    rv = _rotvecs(R_true.cpu())
+    noise = torch.randn(rv.shape, generator=torch.Generator().manual_seed(3)) * math.radians(deg) / math.sqrt(3)
+    pose = ps.PoseModule(rv + noise, torch.zeros((n, 2)), L)
+    mean = cv.Mean(data.mean, L)
+    covar = cv.Covar(L, 2, vectors=(data.eigenvecs * data.eigenvals.sqrt().reshape(-1, 1, 1, 1)).cpu())
+
+    def err(rot):
+        rel = rot.transpose(1, 2) @ R_true
+        return float(torch.rad2deg(torch.arccos(torch.clamp((rel.diagonal(dim1=1, dim2=2).sum(1) - 1) / 2, -1, 1))).mean())
+
+    return data, pose, mean, covar, err
+
+
+def test_pose_trainer_lbfgs_rotations(dev):
+    """rotation_est_method = "LBFGS" (covar_sgd.py:980-981): the row-wise L-BFGS on the rotation
+    vectors (steps limited to 0.1 rad) moves 3-degree perturbed rotations toward the truth."""
+    ds = pkg("dataset")
+    ps = pkg("poses")
+    sgd = pkg("covar_sgd")
+    data, pose, mean, covar, err = _perturbed_problem(dev)
+    tr = sgd.CovarPoseTrainer(covar, ds.create_dataloader(data.dataset, batch_size=128), dev, mean, pose)
+    tr.rotation_est_method = "LBFGS"
+    tr.mean_update_frequency = 100
+    tr.setup_training(nufft_disc="nufft", objective_func="ml", lr=2e-5)
+    assert type(tr.pose_optimizer).__name__ == "BlockwiseLBFGS"
+    before = err(ps.rotvec_to_rotmat(pose.get_rotvecs()))
+    tr.train_epochs(4)
+    after = err(ps.rotvec_to_rotmat(pose.get_rotvecs()))
+    print(f"LBFGS mean rotation error {before:.3f} -> {after:.3f} degrees; costs {tr.training_log['cost_val']}")
+    assert after < 0.8 * before
+
+
+def test_train_covar_half_sets_with_pose(dev):
+    """trainCovar(optimize_pose=True, use_halfsets=True) (covar_sgd.py:1541-1607): each half-set
+    refines its own images' poses (PoseModule.split_module of the dataset's permutation), and the
+    full-set pass continues from both halves' poses put back in place (update_from_modules)."""
+    ps = pkg("poses")
+    sgd = pkg("covar_sgd")
+    data, pose, mean, covar, err = _perturbed_problem(dev, n=512)
+    covar = covar.to(dev)  # trainCovar runs on the model's device (covar_sgd.py:1494)
+    rv0 = pose.get_rotvecs().clone().to(dev)
+    before = err(ps.rotvec_to_rotmat(rv0))
+    seen = {}
+    orig = ps.PoseModule.update_from_modules
+
+    def spy(self, m1, m2, permutation):
+        seen["halves"] = (m1.get_rotvecs().clone(), m2.get_rotvecs().clone(), permutation.clone())
+        return orig(self, m1, m2, permutation)
+
+    ps.PoseModule.update_from_modules = spy
+    try:
+        trainer = sgd.trainCovar(covar, data.dataset, 128, optimize_pose=True, mean_model=mean, pose=pose,
+                                 use_halfsets=True, num_reg_update_iters=1, max_epochs=2, nufft_disc="nufft",
+                                 objective_func="ml", lr=2e-5)
+    finally:
+        ps.PoseModule.update_from_modules = orig
+    m1, m2, perm = seen["halves"]
+    n = rv0.shape[0]
+    assert m1.shape[0] == n // 2 and m2.shape[0] == n - n // 2
+    # both halves moved their own rows, and the full-set trainer started from them
+    assert not torch.equal(m1, rv0[perm[: n // 2].to(rv0.device)])
+    assert not torch.equal(m2, rv0[perm[n // 2:].to(rv0.device)])
+    assert trainer.pose is pose
+    assert "covariance_fsc_halfset" in trainer.training_log
+    after = err(ps.rotvec_to_rotmat(pose.get_rotvecs()))
+    print(f"half-set pose rounds: mean rotation error {before:.3f} -> {after:.3f} degrees")
+    assert after < before
