@@ -325,6 +325,53 @@ __global__ void __launch_bounds__(kSliceBlock)
                   scale ? *scale : (T)1, scale != nullptr);
 }
 
+// k_slice_rows_plan through LDS, for rows of at most kRowsT complex words: a workgroup gathers
+// the rows of 256 consecutive caller points (their values read coalesced, as above) into LDS and
+// then writes them out cooperatively, consecutive lanes storing consecutive words of a row, so
+// every row leaves as one contiguous stretch instead of as 8-byte pieces from 22 store
+// instructions.
+constexpr int kRowsT = 22;
+
+template <typename T>
+__global__ void __launch_bounds__(kSliceBlock)
+    k_slice_rows_plan_lds(const Cplx<T>* __restrict__ sig_a, int64_t n_a, const Cplx<T>* __restrict__ sig_b,
+                          const T* __restrict__ scale, const SortedPoint<T>* __restrict__ sp,
+                          const int* __restrict__ spos, Cplx<T>* __restrict__ rows, int64_t npts, int ntrans,
+                          int rw) {
+  __shared__ Cplx<T> s_rows[kSliceBlock * kRowsT];
+  __shared__ int s_pos[kSliceBlock];
+  const int64_t j0 = (int64_t)blockIdx.x * kSliceBlock;
+  const int64_t j = j0 + threadIdx.x;
+  if (j < npts) {
+    const int s = spos[j];
+    const SortedPoint<T> p = sp[s];
+    s_pos[threadIdx.x] = s;
+    Cplx<T> row[kRowsT];
+    row[0] = {p.x, p.y};
+    row[1] = {p.z, (T)0};
+    const bool in_a = j < n_a;
+    const Cplx<T>* src = in_a ? sig_a : sig_b;
+    const int64_t jj = in_a ? j : j - n_a, stride = in_a ? n_a : npts - n_a;
+    const T sc = scale ? *scale : (T)1;
+#pragma unroll
+    for (int u = 0; u < kRowsT - 2; ++u) {
+      Cplx<T> v = src[(int64_t)min(u, ntrans - 1) * stride + jj];
+      if (scale) v = {v.x * sc, v.y * sc};
+      row[2 + u] = v;
+    }
+#pragma unroll
+    for (int u = 0; u < kRowsT; ++u)
+      if (u < rw) s_rows[threadIdx.x * rw + u] = row[u];
+  }
+  __syncthreads();
+  const int n = (int)min<int64_t>(kSliceBlock, npts - j0);
+  const int ne = n * rw;
+  for (int e = threadIdx.x; e < ne; e += kSliceBlock) {
+    const int r = e / rw;
+    rows[(int64_t)s_pos[r] * rw + (e - r * rw)] = s_rows[e];
+  }
+}
+
 template <typename T>
 __global__ void k_zero_vals(T* p, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -702,10 +749,12 @@ __global__ void __launch_bounds__(kSliceBlock)
     const int64_t x = (int64_t)rint(p.x), y = (int64_t)rint(p.y), z = (int64_t)rint(p.z);
     if (x >= 0 && x < W && y >= 0 && y < H && z >= 0 && z < D) acc = v[(z * H + y) * W + x];
   }
-  if (p.j < n_a)
-    out_a[t * n_a + p.j] = acc;
+  // one destination and one vector store (two alternative stores were split into 4-byte pieces)
+  Cplx<T>* dst = p.j < n_a ? out_a + (t * n_a + p.j) : out_b + (t * (npts - n_a) + (p.j - n_a));
+  if constexpr (sizeof(T) == 4)
+    *reinterpret_cast<float2*>(dst) = make_float2((float)acc.x, (float)acc.y);
   else
-    out_b[t * (npts - n_a) + (p.j - n_a)] = acc;
+    *reinterpret_cast<double2*>(dst) = make_double2((double)acc.x, (double)acc.y);
 }
 
 struct SlicePlan {
@@ -832,9 +881,14 @@ static int slice_plan_adjoint(SlicePlan* pl, const void* sig_a, int64_t n_a, con
   const int rw = 2 + (int)ntrans;
   if (int rc = plan_grow(&w.rows, w.cap_rows, (size_t)npts * rw * sizeof(Cplx<T>))) return rc;
   const unsigned nbp = (unsigned)ceil_div(npts, kSliceBlock);
-  hipLaunchKernelGGL((k_slice_rows_plan<T>), dim3(nbp), dim3(kSliceBlock), 0, st, (const Cplx<T>*)sig_a, n_a,
-                     (const Cplx<T>*)sig_b, (const T*)scale, (const SortedPoint<T>*)pl->sp, w.spos,
-                     (Cplx<T>*)w.rows, npts, (int)ntrans, rw);
+  if (rw <= kRowsT)
+    hipLaunchKernelGGL((k_slice_rows_plan_lds<T>), dim3(nbp), dim3(kSliceBlock), 0, st, (const Cplx<T>*)sig_a,
+                       n_a, (const Cplx<T>*)sig_b, (const T*)scale, (const SortedPoint<T>*)pl->sp, w.spos,
+                       (Cplx<T>*)w.rows, npts, (int)ntrans, rw);
+  else
+    hipLaunchKernelGGL((k_slice_rows_plan<T>), dim3(nbp), dim3(kSliceBlock), 0, st, (const Cplx<T>*)sig_a, n_a,
+                       (const Cplx<T>*)sig_b, (const T*)scale, (const SortedPoint<T>*)pl->sp, w.spos,
+                       (Cplx<T>*)w.rows, npts, (int)ntrans, rw);
   COV3D_LAUNCH_CHECK();
   constexpr int RT = 20;
   dim3 g((unsigned)ceil_div(D * H * W, kSliceBlock), (unsigned)ceil_div(ntrans, RT));
