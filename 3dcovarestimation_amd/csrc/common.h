@@ -22,6 +22,16 @@ struct Cplx {
   T x, y;
 };
 
+// One 8-byte (FP64: 16-byte) store of a complex value.  Cplx<T> is only T-aligned, and a
+// member-wise store of one may be emitted as two separate stores (two partial writes of one word).
+template <typename T>
+__device__ __forceinline__ void store_cplx(Cplx<T>* p, T x, T y) {
+  if constexpr (sizeof(T) == 4)
+    *reinterpret_cast<float2*>(p) = make_float2((float)x, (float)y);
+  else
+    *reinterpret_cast<double2*>(p) = make_double2((double)x, (double)y);
+}
+
 template <typename T>
 __device__ __forceinline__ Cplx<T> cadd(Cplx<T> a, Cplx<T> b) {
   return {a.x + b.x, a.y + b.y};

@@ -1084,11 +1084,9 @@ __global__ void __launch_bounds__(kBlock)
         const int p = __float_as_int(rec.w);
         const float2 v = interp_pair_point_f32<W, PL>(reinterpret_cast<const float4*>(tile), h, rec.x, rec.y, rec.z,
                                                       kc, beta, poly);
-        if (direct) {
-          if (t0 + h < ntrans) direct[(int64_t)(t0 + h) * M + iidx[p0 + slot]] = Cplx<T>{v.x, v.y};
-        } else {
-          c[((int64_t)blockIdx.y * M + p) * TC + h] = Cplx<T>{v.x, v.y};
-        }
+        Cplx<T>* dst = direct ? direct + (int64_t)(t0 + h) * M + iidx[p0 + slot]
+                              : c + ((int64_t)blockIdx.y * M + p) * TC + h;
+        if (!direct || t0 + h < ntrans) store_cplx<T>(dst, v.x, v.y);
       }
       return;
     }
@@ -1155,7 +1153,7 @@ __global__ void __launch_bounds__(kBlock)
       // the caller's (ntrans, M) layout at the point's original index j
 #pragma unroll
       for (int tt = 0; tt < TC; ++tt)
-        if (t0 + tt < ntrans) direct[(int64_t)(t0 + tt) * M + j] = Cplx<T>{ax[tt], ay[tt]};
+        if (t0 + tt < ntrans) store_cplx<T>(direct + (int64_t)(t0 + tt) * M + j, ax[tt], ay[tt]);
     } else {
       // sorted, chunk-major output: cs[chunk][p][TC]
       Cplx<T>* dst = c + ((int64_t)blockIdx.y * M + p) * TC;
@@ -1306,11 +1304,10 @@ __global__ void __launch_bounds__(kBlock)
       const int p = __float_as_int(rec.w);
       const int64_t j = s_idx[slot / kBlock][slot % kBlock];
       const float2 v = interp_pair_point_f32<W, PL>(t4, h, u0, u1, u2, kc, beta, poly);
-      if (direct) {
-        if (t0 + h < ntrans) direct[(int64_t)(t0 + h) * M + j] = Cplx<float>{v.x, v.y};
-      } else {
-        c[((int64_t)pair * M + p) * 2 + h] = Cplx<float>{v.x, v.y};
-      }
+      // one destination and one store (stores in both arms of a branch were merged into a
+      // shared 4-byte half)
+      Cplx<float>* dst = direct ? direct + (int64_t)(t0 + h) * M + j : c + ((int64_t)pair * M + p) * 2 + h;
+      if (!direct || t0 + h < ntrans) store_cplx<float>(dst, v.x, v.y);
     }
     if (nxt >= nitems) break;
     if (threadIdx.x == 0) s_v[1] = v2;
