@@ -911,6 +911,80 @@ static int slice_plan_adjoint(SlicePlan* pl, const void* sig_a, int64_t n_a, con
   return 0;
 }
 
+// grid_sample coordinates of slice points from rotated frequencies (B, 3, npix) (zyx rows):
+// grid[(b npix + k), c] = pts[b, 2 - c, k] * scale + shift, the per-point form of
+// NufftPlanDiscretized.setpts' transpose, flip and affine map.
+template <typename T>
+__global__ void k_slice_grid_from_pts(const T* __restrict__ pts, T* __restrict__ grid, int64_t n, int64_t npix,
+                                      T scale, T shift) {
+  // a product and a sum, each rounded, as the two tensor operations round them (no fused multiply-add)
+#pragma clang fp contract(off)
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // output element
+  if (e >= n * npix * 3) return;
+  const int64_t j = e / 3;
+  const int c = (int)(e - j * 3);
+  const int64_t b = j / npix, k = j - b * npix;
+  const T a = pts[(b * 3 + (2 - c)) * npix + k] * scale;
+  grid[e] = shift != (T)0 ? a + shift : a;
+}
+
+// Boundary pixels of the Fourier-domain ML on a slice plan: pixel k of image b is flagged when its
+// trilinear stencil can reach cell 0 or cell vL of the padded volume on some axis (cell coordinate
+// within one of 0 or vL), OR-ed with the flag of its mirror pixel -k, and kept where keep[k] is set
+// (the half plane, or every pixel).  One workgroup per image writes the kept flagged pixels in
+// ascending order to order[b, 0 .. count[b]) (at most cap of them; count[b] is the full number).
+constexpr int kBndThreads = 256;
+
+__device__ __forceinline__ bool near_nyquist(const float* g, float half_cells, float vL) {
+  bool near = false;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const float x = (g[c] + 1.f) * half_cells;
+    near = near || (x > -1.f && x < 1.f) || (x > vL - 1.f && x < vL + 1.f);
+  }
+  return near;
+}
+
+__global__ void __launch_bounds__(kBndThreads)
+    k_slice_boundary(const float* __restrict__ grid, const unsigned char* __restrict__ keep, int L, float vL,
+                     int* __restrict__ order, int* __restrict__ count, int cap) {
+  __shared__ int s_scan[kBndThreads];
+  const int64_t b = blockIdx.x;
+  const int np = L * L;
+  const float* gb = grid + b * (int64_t)np * 3;
+  const float half_cells = 0.5f * (vL - 1.f);
+  // each thread owns a contiguous run of pixels, so a block scan of the counts orders them
+  const int per = (np + kBndThreads - 1) / kBndThreads;
+  const int k0 = threadIdx.x * per, k1 = min(np, k0 + per);
+  auto flagged = [&](int k) {
+    if (!keep[k]) return false;
+    const int i = k / L, j = k - i * L;
+    const int mi = (L % 2 == 0) ? (L - i) % L : L - 1 - i, mj = (L % 2 == 0) ? (L - j) % L : L - 1 - j;
+    return near_nyquist(gb + (int64_t)k * 3, half_cells, vL) ||
+           near_nyquist(gb + (int64_t)(mi * L + mj) * 3, half_cells, vL);
+  };
+  int n = 0;
+  for (int k = k0; k < k1; ++k) n += flagged(k) ? 1 : 0;
+  s_scan[threadIdx.x] = n;
+  __syncthreads();
+  // inclusive scan (Hillis-Steele) over the threads' counts
+  for (int off = 1; off < kBndThreads; off <<= 1) {
+    const int v = threadIdx.x >= off ? s_scan[threadIdx.x - off] : 0;
+    __syncthreads();
+    s_scan[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int pos = s_scan[threadIdx.x] - n;
+  if (n > 0) {
+    for (int k = k0; k < k1; ++k)
+      if (flagged(k)) {
+        if (pos < cap) order[b * cap + pos] = k;
+        ++pos;
+      }
+  }
+  if (threadIdx.x == kBndThreads - 1) count[b] = s_scan[kBndThreads - 1];
+}
+
 }  // namespace cov3d
 
 extern "C" int cov3d_slice_forward(const void* vol, const void* grid, void* out, int64_t ntrans,
@@ -1034,5 +1108,37 @@ extern "C" int cov3d_slice_plan_destroy(void* plan) {
   hipFree(w.keys), hipFree(w.skeys), hipFree(w.sidx), hipFree(w.spos), hipFree(w.start), hipFree(w.end);
   hipFree(w.tmp), hipFree(w.rows), hipFree(w.off), hipFree(w.cnt), hipFree(pl->sp);
   delete pl;
+  return 0;
+}
+
+extern "C" int cov3d_slice_grid_from_pts(const void* pts, void* grid, int64_t n, int64_t npix, double scale,
+                                         double shift, int dtype, void* stream) {
+  COV3D_CHECK_ARG(n >= 0 && npix >= 0, "negative size");
+  COV3D_CHECK_ARG(dtype == 0 || dtype == 1, "dtype must be 0 (float32) or 1 (float64)");
+  if (n == 0 || npix == 0) return 0;
+  COV3D_CHECK_ARG(pts && grid, "NULL pointer");
+  hipStream_t st = cov3d::as_stream(stream);
+  const int64_t ne = n * npix * 3;
+  const unsigned nb = (unsigned)cov3d::ceil_div(ne, cov3d::kSliceBlock);
+  if (dtype == 1)
+    hipLaunchKernelGGL((cov3d::k_slice_grid_from_pts<double>), dim3(nb), dim3(cov3d::kSliceBlock), 0, st,
+                       (const double*)pts, (double*)grid, n, npix, scale, shift);
+  else
+    hipLaunchKernelGGL((cov3d::k_slice_grid_from_pts<float>), dim3(nb), dim3(cov3d::kSliceBlock), 0, st,
+                       (const float*)pts, (float*)grid, n, npix, (float)scale, (float)shift);
+  COV3D_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int cov3d_slice_boundary_pixels(const float* grid, const unsigned char* keep, int64_t B, int64_t L,
+                                           int64_t vL, int* order, int* count, int64_t cap, void* stream) {
+  COV3D_CHECK_ARG(B >= 0 && L >= 1 && vL >= 2 && cap >= 0, "bad sizes");
+  COV3D_CHECK_ARG(B < 2147483647 && L * L < 2147483647 && cap < 2147483647, "sizes too large");
+  if (B == 0) return 0;
+  COV3D_CHECK_ARG(grid && keep && count && (order || cap == 0), "NULL pointer");
+  hipStream_t st = cov3d::as_stream(stream);
+  hipLaunchKernelGGL(cov3d::k_slice_boundary, dim3((unsigned)B), dim3(cov3d::kBndThreads), 0, st, grid, keep, (int)L,
+                     (float)vL, order, count, (int)cap);
+  COV3D_LAUNCH_CHECK();
   return 0;
 }

@@ -232,25 +232,36 @@ class MLCostNufft(torch.autograd.Function):
         return grad_v, None, None, None, None, None, None, None
 
 
-def _boundary_pixels(plan, B: int, L: int) -> torch.Tensor:
-    """(B, L*L) mask of the pixels whose discretised slice is not the mirror image of its -k partner.
+def _boundary_order(plan, B: int, L: int, keep: torch.Tensor) -> Tuple[torch.Tensor, torch.Tensor]:
+    """The pixels whose discretised slice is not the mirror image of its -k partner, per image in
+    ascending order (``cov3d_slice_boundary_pixels``): (order (B, L*L) int32, valid in the first
+    count[b] entries; count (B,) int32).
 
     The slice of pixel k reads the centred volume at index x = p vL / (2 pi) + vL / 2 per axis,
     and its -k partner at vL - x; cell j mirrors to cell vL - j, so everything is mirrored except
     the Nyquist cell 0, whose mirror vL lies outside the volume.  A pixel is flagged when its
     stencil can reach cell 0 or cell vL on any axis (x within one cell of 0 or of vL; a superset of
-    both interpolation modes), and the flag is made symmetric by OR-ing it with its partner's, so
-    the set is closed under k -> -k.
+    both interpolation modes), and the flag is OR-ed with its partner's, so the set is closed under
+    k -> -k; only pixels with ``keep`` (uint8, L*L) set are listed.
     """
     vL = plan.sz[0] * plan.upsample_factor
-    x = (plan._grid_flat.to(torch.float32) + 1) * (0.5 * (vL - 1))  # (B*L*L, 3) cell coordinates
-    near = ((x > -1.0) & (x < 1.0)) | ((x > vL - 1.0) & (x < vL + 1.0))
-    f = near.any(dim=1).reshape(B, L, L)
-    if L % 2 == 0:
-        m = torch.roll(torch.flip(f, dims=(1, 2)), shifts=(1, 1), dims=(1, 2))  # (i, j) -> (L - i, L - j) mod L
-    else:
-        m = torch.flip(f, dims=(1, 2))  # (i, j) -> (L - 1 - i, L - 1 - j)
-    return (f | m).reshape(B, L * L)
+    grid = _native.dense(plan._grid_flat.detach().to(torch.float32))
+    dev = grid.device
+    order = torch.empty((B, L * L), dtype=torch.int32, device=dev)
+    count = torch.empty(B, dtype=torch.int32, device=dev)
+    keep = _native.dense(keep.to(device=dev, dtype=torch.uint8))
+    _native.check(_native.load().cov3d_slice_boundary_pixels(
+        grid.data_ptr(), keep.data_ptr(), B, L, vL, order.data_ptr(), count.data_ptr(), L * L,
+        _native.stream_ptr(dev)))
+    return order, count
+
+
+def _first_entries(order: torch.Tensor, count: torch.Tensor, n: int) -> Tuple[torch.Tensor, torch.Tensor]:
+    """The first n entries of every row of a compacted list as int64 indices (0 past a row's
+    count) and the mask of the valid ones."""
+    valid = torch.arange(n, device=order.device).unsqueeze(0) < count.unsqueeze(1)
+    idx = torch.where(valid, order[:, :n].long(), torch.zeros((), dtype=torch.long, device=order.device))
+    return idx, valid
 
 
 class MLCostDiscretized(torch.autograd.Function):
@@ -267,7 +278,7 @@ class MLCostDiscretized(torch.autograd.Function):
     the fused kernel (split-BF16 MFMA Gram, raw values c = S(vhat), images yhat, spatial noise
     variance noise_var / L^2) gives Re M and Re b over all L^2 pixels.  Their imaginary parts cancel
     between every pixel and its -k partner except where the slice touches the padded volume's
-    Nyquist cell (``_boundary_pixels``), so they are summed over those pixels only.  The r x r
+    Nyquist cell (``_boundary_order``), so they are summed over those pixels only.  The r x r
     complex solve and log-determinant run in FP64.  Backward: the gradient with respect to the raw
     values, conj(M^-1) Q + conj(z) (z^T Q - y~) in the kernel's units, splits into two passes of the
     fused gradient kernel with real coefficients (conj(M^-1) = R - iJ, z = a + ib):
@@ -305,21 +316,19 @@ class MLCostDiscretized(torch.autograd.Function):
         scale = nyquist_mask(L, dev).reshape(1, -1).expand(B, -1)
         if ctf is not None:
             scale = scale * ctf.reshape(B, -1).to(torch.float32)
-        bd = _boundary_pixels(plan, B, L)
         ctx.half = bool(half)
         if half:
             pix = half_plane_pixels(L, dev)
             npix = pix.numel()
             gfull = plan._grid_flat.detach().reshape(B, L * L, 3)
             # boundary pairs: h in the half plane, o = its mirror pixel
-            in_h = torch.zeros(L * L, dtype=torch.bool, device=dev)
-            in_h[pix.long()] = True
-            bh = bd & in_h.unsqueeze(0)
-            nb = int(bh.sum(dim=1).max())
+            in_h = torch.zeros(L * L, dtype=torch.uint8, device=dev)
+            in_h[pix.long()] = 1
+            border, bcount = _boundary_order(plan, B, L, in_h)
+            nb = int(bcount.max())
             gh = gfull[:, pix.long()].reshape(-1, 3)
             if nb > 0:
-                order = torch.sort(bh.to(torch.int8), dim=1, descending=True, stable=True).indices[:, :nb]
-                valid = torch.gather(bh, 1, order)  # (B, nb)
+                order, valid = _first_entries(border, bcount, nb)  # (B, nb)
                 oidx = _mirror_index(L, dev)[order]  # mirror pixels (full-plane indices)
                 go = torch.gather(gfull, 1, oidx.unsqueeze(-1).expand(-1, -1, 3)).clone()
                 go[~valid] = 4.0  # outside the volume: padded entries slice to zero
@@ -361,12 +370,12 @@ class MLCostDiscretized(torch.autograd.Function):
             M = res["m"].double()
             bvec = res["b"].double()
             # imaginary parts from the pixels whose -k partner is not their mirror image
-            nbd = int(bd.sum(dim=1).max())
+            border, bcount = _boundary_order(plan, B, L, torch.ones(L * L, dtype=torch.uint8, device=dev))
+            nbd = int(bcount.max())
             Mi = torch.zeros_like(M)
             bi = torch.zeros_like(bvec)
             if nbd > 0:
-                order = torch.sort(bd.to(torch.int8), dim=1, descending=True, stable=True).indices[:, :nbd]
-                valid = torch.gather(bd, 1, order)  # (B, nbd)
+                order, valid = _first_entries(border, bcount, nbd)  # (B, nbd)
                 w = torch.gather(scale, 1, order) * valid / float(L * L)  # Q = mask ctf c / L^2
                 q = torch.gather(c, 2, order.unsqueeze(0).expand(r, -1, -1)) * w.unsqueeze(0)  # (r, B, nbd)
                 q = q.permute(1, 0, 2).to(torch.complex128)  # (B, r, nbd)

@@ -121,15 +121,30 @@ class NufftPlanDiscretized(BaseNufftPlan):
         """points: (n, 3, Lp**2) zyx frequencies in radians (nufft_plan.py:94-118)."""
         L = self.sz[0]
         self.points_L = int(round(points.shape[-1] ** 0.5))
-        grid = points.transpose(-2, -1).reshape(1, -1, self.points_L, self.points_L, 3).flip(-1)
         vol_L = L * self.upsample_factor
         # grid_sample's [-1, 1] spans the vol_L samples; frequency k*2pi/L maps to index k*us + vol_L//2
-        grid = grid * ((vol_L / (vol_L - 1)) / torch.pi)
-        if vol_L % 2 == 0:
-            grid = grid + 1 / (vol_L - 1)
+        scale = (vol_L / (vol_L - 1)) / torch.pi
+        shift = 1 / (vol_L - 1) if vol_L % 2 == 0 else 0.0
+        if (points.ndim == 3 and points.shape[1] == 3 and points.is_cuda and not points.requires_grad
+                and points.dtype in (torch.float32, torch.float64)):
+            # one pass on the device (cov3d_slice_grid_from_pts); points that carry a gradient (the
+            # pose step) take the tensor operations below, which autograd differentiates
+            n, _, npix = points.shape
+            pts = _native.dense(points)
+            flat = torch.empty((n * npix, 3), dtype=points.dtype, device=points.device)
+            _native.check(_native.load().cov3d_slice_grid_from_pts(
+                pts.data_ptr(), flat.data_ptr(), n, npix, scale, shift, 1 if points.dtype == torch.float64 else 0,
+                _native.stream_ptr(points.device)))
+            grid = flat.reshape(1, n, self.points_L, self.points_L, 3)
+        else:
+            grid = points.transpose(-2, -1).reshape(1, -1, self.points_L, self.points_L, 3).flip(-1)
+            grid = grid * scale
+            if vol_L % 2 == 0:
+                grid = grid + shift
+            flat = grid.reshape(-1, 3).contiguous()
         self.points = grid
         self.batch_points = grid.shape[1]
-        self._grid_flat = grid.reshape(-1, 3).contiguous()
+        self._grid_flat = flat
         self._synced = None
 
     def set_grid_flat(self, grid_flat: torch.Tensor) -> None:

@@ -152,6 +152,21 @@ int cov3d_slice_plan_forward_split(void* plan, const void* vol, void* out_a, int
 int cov3d_slice_plan_adjoint_split(void* plan, const void* sig_a, int64_t n_a, const void* sig_b,
                                    const void* scale, void* vol_out, int64_t ntrans, void* stream);
 int cov3d_slice_plan_destroy(void* plan);
+/* The flat (n * npix, 3) grid_sample coordinates NufftPlanDiscretized.setpts forms from rotated
+ * frequencies pts (n, 3, npix) (zyx rows; nufft_plan.py:94-118): grid[j, c] = pts[b, 2 - c, k] *
+ * scale + shift for j = b * npix + k, the product and the sum each rounded (dtype 0: float32,
+ * 1: float64). */
+int cov3d_slice_grid_from_pts(const void* pts, void* grid, int64_t n, int64_t npix, double scale,
+                              double shift, int dtype, void* stream);
+/* The pixels of the Fourier-domain ML on a slice plan (covar_sgd.py:1350-1399) whose discretised
+ * slice is not the mirror of its -k partner's (no counterpart in the reference, which evaluates
+ * every pixel): pixel k of image b (grid (B * L * L, 3) float32, as cov3d_slice_grid_from_pts
+ * makes it) is flagged when a cell coordinate (g + 1) (vL - 1) / 2 lies within one of 0 or vL on
+ * some axis, or when that of its mirror pixel does ((L - i) mod L, (L - j) mod L for even L;
+ * (L - 1 - i, L - 1 - j) for odd L); flagged pixels with keep[k] != 0 are written in ascending
+ * order to order[b * cap + 0 ..), at most cap of them, and count[b] is their number. */
+int cov3d_slice_boundary_pixels(const float* grid, const unsigned char* keep, int64_t B, int64_t L,
+                                int64_t vL, int* order, int* count, int64_t cap, void* stream);
 
 /* Gradient of cov3d_slice_forward with respect to its grid: for each point j,
  *   grid_grad[j, a] = sum_t Re(conj(grad_out[t, j]) * d out[t, j] / d grid[j, a])

@@ -142,3 +142,69 @@ def test_discretized_plan_resyncs_on_setpts_and_volume_size(dev):
             assert torch.equal(a, fresh.execute_forward(vol))
             outs.append(a)
     assert not torch.equal(outs[0], outs[2])
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_grid_from_pts_matches_tensor_ops(dev, dtype):
+    """cov3d_slice_grid_from_pts (the native half of NufftPlanDiscretized.setpts) equals the
+    transpose, flip and affine map of nufft_plan.py:94-118 bit for bit."""
+    npl = pkg("nufft_plan")
+    ds = pkg("dataset")
+    L, us = 16, 2
+    rots = ds.haar_rotations(7, torch.Generator().manual_seed(4)).to(dev)
+    pts = ds.pts_rot_from_rotations(rots, L).to(dtype)
+    plan = npl.NufftPlanDiscretized((L,) * 3, upsample_factor=us, mode="bilinear")
+    plan.setpts(pts)
+    vL = L * us
+    ref = pts.transpose(-2, -1).reshape(1, -1, L, L, 3).flip(-1) * ((vL / (vL - 1)) / torch.pi) + 1 / (vL - 1)
+    assert torch.equal(plan._grid_flat, ref.reshape(-1, 3))
+    assert plan.points.shape == (1, 7, L, L, 3) and plan.batch_points == 7
+
+
+@pytest.mark.parametrize("L", [16, 15])
+def test_boundary_pixels_cover_the_broken_mirror_pairs(dev, L):
+    """The pixels cov3d_slice_boundary_pixels lists are the ones whose discretised slice of a
+    Hermitian volume (the centred transform of a real volume) may not be the conjugate of the
+    -k partner's: every pair that breaks the symmetry is listed, the list is closed under the
+    mirror, ascending, and restricted by keep."""
+    npl = pkg("nufft_plan")
+    ds = pkg("dataset")
+    ml = pkg("ml_ops")
+    pf = pkg("projection_funcs")
+    us, B = 2, 24
+    vL = L * us
+    g = torch.Generator().manual_seed(9)
+    rots = ds.haar_rotations(B, g).to(dev)
+    plan = npl.NufftPlanDiscretized((L,) * 3, upsample_factor=us, mode="bilinear")
+    plan.setpts(ds.pts_rot_from_rotations(rots, L))
+    vol = torch.randn((1, L, L, L), generator=g, dtype=torch.float64).to(dev)
+    vhat = pf.centered_fft3(vol, padding_size=(vL,) * 3)
+    c = plan.execute_forward(vhat).reshape(B, L * L)
+    mirror = ml._mirror_index(L, dev)
+    # the float32 grid makes x(-p) = vL - x(p) hold to ~1e-7 relative only; a pair that reaches the
+    # Nyquist cell differs by the weight of that cell's value
+    broken = (c - c[:, mirror].conj()).abs() > 1e-4 * c.abs().max()
+    if L % 2 == 0:
+        # the image's own Nyquist row and column pair with themselves, not with -k (the ML zeroes them)
+        k = torch.arange(L * L, device=dev)
+        broken &= ~((k // L == 0) | (k % L == 0)).unsqueeze(0)
+    order, count = ml._boundary_order(plan, B, L, torch.ones(L * L, dtype=torch.uint8, device=dev))
+    flags = torch.zeros((B, L * L), dtype=torch.bool, device=dev)
+    for b in range(B):
+        row = order[b, : int(count[b])].long()
+        assert torch.all(row[1:] > row[:-1])  # ascending
+        flags[b, row] = True
+    assert broken.any(), "the points never reach the Nyquist cells; the test would be vacuous"
+    assert not (broken & ~flags).any()
+    assert torch.equal(flags, flags[:, mirror])
+    # the cell-distance definition, restated on the host
+    x = (plan._grid_flat.double().cpu() + 1) * (0.5 * (vL - 1))
+    near = (((x > -1) & (x < 1)) | ((x > vL - 1) & (x < vL + 1))).any(dim=1).reshape(B, L * L)
+    near = near | near[:, mirror.cpu()]
+    assert torch.equal(flags.cpu(), near)
+    keep = torch.zeros(L * L, dtype=torch.uint8, device=dev)
+    keep[ml.half_plane_pixels(L, dev).long()] = 1
+    order_h, count_h = ml._boundary_order(plan, B, L, keep)
+    for b in range(B):
+        row = order_h[b, : int(count_h[b])].long()
+        assert torch.equal(row, torch.nonzero(flags[b] & keep.bool()).reshape(-1))
