@@ -523,6 +523,94 @@ __global__ void __launch_bounds__(kMlBlock)
   }
 }
 
+// Batched Hermitian positive definite solve in FP64, one 64-lane workgroup per r x r matrix
+// (r <= kHermMax): M = L L^H (Cholesky in LDS, column k's rows in parallel), log det M =
+// 2 sum log L_kk, Y = L^-1 (one column per lane by forward substitution), M^-1 = Y^H Y and
+// z = M^-1 b.  Complex values are (re, im) double pairs; the matrices are read and written
+// row-major.  A non-positive pivot (M not positive definite) gives NaN outputs for that matrix.
+constexpr int kHermMax = 32;
+constexpr int kHermThreads = 64;
+
+__global__ void __launch_bounds__(kHermThreads)
+    k_herm_solve(const double2* __restrict__ M, const double2* __restrict__ b, double2* __restrict__ minv,
+                 double2* __restrict__ z, double* __restrict__ logdet, int r) {
+  __shared__ double2 s_L[kHermMax][kHermMax + 1];
+  __shared__ double2 s_Y[kHermMax][kHermMax + 1];
+  __shared__ double2 s_b[kHermMax];
+  __shared__ double s_ld[kHermMax];
+  const int64_t m = blockIdx.x;
+  const int tid = threadIdx.x;
+  const double2* Mm = M + m * r * r;
+  for (int e = tid; e < r * r; e += kHermThreads) s_L[e / r][e % r] = Mm[e];
+  if (tid < r) s_b[tid] = b[m * r + tid];
+  __syncthreads();
+  // right-looking Cholesky on the lower triangle: after step k, column k holds L(:, k)
+  for (int k = 0; k < r; ++k) {
+    if (tid == 0) s_L[k][k].x = sqrt(s_L[k][k].x), s_L[k][k].y = 0.0;
+    __syncthreads();
+    const double d = s_L[k][k].x;
+    for (int i = k + 1 + tid; i < r; i += kHermThreads) {
+      s_L[i][k].x /= d;
+      s_L[i][k].y /= d;
+    }
+    __syncthreads();
+    // trailing update A(i, j) -= L(i, k) conj(L(j, k)), k < j <= i
+    for (int e = tid; e < (r - k - 1) * (r - k - 1); e += kHermThreads) {
+      const int i = k + 1 + e / (r - k - 1), j = k + 1 + e % (r - k - 1);
+      if (j <= i) {
+        const double2 a = s_L[i][k], c = s_L[j][k];
+        s_L[i][j].x -= a.x * c.x + a.y * c.y;
+        s_L[i][j].y -= a.y * c.x - a.x * c.y;
+      }
+    }
+    __syncthreads();
+  }
+  if (tid < r) s_ld[tid] = log(s_L[tid][tid].x);
+  // Y = L^-1: lane c solves L y = e_c
+  if (tid < r) {
+    const int c = tid;
+    for (int i = 0; i < r; ++i) {
+      double2 acc = make_double2(i == c ? 1.0 : 0.0, 0.0);
+      for (int j = c; j < i; ++j) {
+        const double2 l = s_L[i][j], y = s_Y[j][c];
+        acc.x -= l.x * y.x - l.y * y.y;
+        acc.y -= l.x * y.y + l.y * y.x;
+      }
+      const double d = s_L[i][i].x;
+      s_Y[i][c] = i < c ? make_double2(0.0, 0.0) : make_double2(acc.x / d, acc.y / d);
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double acc = 0.0;
+    for (int k = 0; k < r; ++k) acc += s_ld[k];
+    logdet[m] = 2.0 * acc;
+  }
+  // M^-1 = Y^H Y: element (i, j) = sum_k conj(Y(k, i)) Y(k, j), k >= max(i, j)
+  double2* Im = minv + m * r * r;
+  for (int e = tid; e < r * r; e += kHermThreads) {
+    const int i = e / r, j = e % r;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int k = max(i, j); k < r; ++k) {
+      const double2 a = s_Y[k][i], c = s_Y[k][j];
+      acc.x += a.x * c.x + a.y * c.y;
+      acc.y += a.x * c.y - a.y * c.x;
+    }
+    Im[e] = acc;
+    s_L[i][j] = acc;  // L is no longer needed: keep M^-1 for z
+  }
+  __syncthreads();
+  if (tid < r) {
+    double2 acc = make_double2(0.0, 0.0);
+    for (int j = 0; j < r; ++j) {
+      const double2 a = s_L[tid][j], c = s_b[j];
+      acc.x += a.x * c.x - a.y * c.y;
+      acc.y += a.x * c.y + a.y * c.x;
+    }
+    z[m * r + tid] = acc;
+  }
+}
+
 }  // namespace cov3d
 
 extern "C" int cov3d_ml_forward(const void* c, const void* ctf, const void* yhat,
@@ -680,6 +768,19 @@ extern "C" int cov3d_ml_grad_images_complex(const void* c, const void* ctf, cons
   else
     COV3D_GRADC(32);
 #undef COV3D_GRADC
+  COV3D_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int cov3d_herm_solve_c128(const void* M, const void* b, void* minv, void* z, void* logdet, int64_t n,
+                                     int64_t r, void* stream) {
+  COV3D_CHECK_ARG(n >= 0 && r >= 1 && r <= cov3d::kHermMax, "need n >= 0 and 1 <= r <= %d", cov3d::kHermMax);
+  COV3D_CHECK_ARG(n < 2147483647, "too many matrices");
+  if (n == 0) return 0;
+  COV3D_CHECK_ARG(M && b && minv && z && logdet, "NULL pointer");
+  hipStream_t st = cov3d::as_stream(stream);
+  hipLaunchKernelGGL(cov3d::k_herm_solve, dim3((unsigned)n), dim3(cov3d::kHermThreads), 0, st, (const double2*)M,
+                     (const double2*)b, (double2*)minv, (double2*)z, (double*)logdet, (int)r);
   COV3D_LAUNCH_CHECK();
   return 0;
 }

@@ -239,6 +239,13 @@ int cov3d_ml_grad_images_complex(const void* c, const void* ctf, const void* yha
                                  void* g, int64_t B, int64_t r, int64_t L, double noise_var,
                                  double grad_scale, void* stream);
 
+/* The r x r complex solve of the Fourier-domain ML (covar_sgd.py:1381-1399, where torch's
+ * linalg.solve and logdet run it on M = I + conj(P) P^T / noise_var, Hermitian positive definite):
+ * for n matrices M (n, r, r) complex128 and right-hand sides b (n, r), r <= 32, writes M^-1,
+ * z = M^-1 b and log det M (float64) from one FP64 Cholesky per matrix (lower triangle read). */
+int cov3d_herm_solve_c128(const void* M, const void* b, void* minv, void* z, void* logdet,
+                          int64_t n, int64_t r, void* stream);
+
 /* cov3d_ml_grad_images with the output written to a plan's sorted type-1
  * input (cov3d_nufft_sorted_input): value t of pixel k of image i goes to
  * g_sorted[spos[i * np + k] * ldc + t], zeros for r <= t < ldc (ldc even,

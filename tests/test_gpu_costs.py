@@ -253,3 +253,20 @@ def test_nufft_point_gradient_finite_differences(dev, L):
         an = float((grad * d).sum())
         print(f"L={L}: analytic {an:.10e} finite difference {float(fd):.10e}")
         assert abs(an - float(fd)) < 1e-6 * abs(float(fd)) + 1e-9
+
+
+@pytest.mark.parametrize("r", [1, 5, 20, 31])
+def test_herm_solve_matches_torch(dev, r):
+    """cov3d_herm_solve_c128 (one FP64 Cholesky per matrix) against torch's complex128 inverse,
+    solve and log-determinant on Hermitian positive definite matrices of the ML's form."""
+    ml = pkg("ml_ops")
+    g = torch.Generator(device=dev).manual_seed(r)
+    n, d = 37, 3 * r + 2
+    P = torch.randn((n, r, d), dtype=torch.complex128, device=dev, generator=g)
+    M = torch.eye(r, dtype=torch.complex128, device=dev) + P.conj() @ P.transpose(1, 2) / 0.7
+    b = torch.randn((n, r), dtype=torch.complex128, device=dev, generator=g)
+    minv, z, logdet = ml.herm_solve(M, b)
+    ref_inv = torch.linalg.inv(M)
+    torch.testing.assert_close(minv, ref_inv, rtol=1e-11, atol=1e-13)
+    torch.testing.assert_close(z, torch.linalg.solve(M, b.unsqueeze(-1)).squeeze(-1), rtol=1e-11, atol=1e-12)
+    torch.testing.assert_close(logdet, torch.linalg.slogdet(M)[1], rtol=1e-12, atol=1e-12)
