@@ -41,6 +41,9 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+# LDS: a conflict-free ds_read_b128 stream measured on the MI355X (16 waves per CU, all CUs);
+# AMD publishes no chip-wide LDS bandwidth figure
+LDS_STREAM_TBS = 97.0
 
 
 def log(msg: str) -> None:
@@ -512,6 +515,15 @@ def main():
         "avg_launch_ms": round(avg_ms, 3),
         "timing": "HIP events on the launch stream around every launch in the timed steps (cov3d_timing_*)",
     }
+    if kern["interp"]["avg_ms"]:
+        # the interpolation is LDS-bound (SQ counters: LDS active ~65 %, VALU ~68 %): every point reads its
+        # w^3 taps of every transform from the LDS tile once, 8 bytes per tap and transform
+        lds_bytes = B * d * w**3 * 8 * r
+        lds_tbs = lds_bytes / (kern["interp"]["avg_ms"] * 1e-3) / 1e12
+        roof["interp_lds"] = {"kernel": "k_interp_pipe", "bytes_per_launch": lds_bytes,
+                              "achieved_TBs": round(lds_tbs, 1), "peak_TBs": LDS_STREAM_TBS,
+                              "frac": round(lds_tbs / LDS_STREAM_TBS, 4),
+                              "peak_source": "measured conflict-free ds_read_b128 stream (no published figure)"}
     if args.copy_probe:
         src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
         dst = torch.empty_like(src)
