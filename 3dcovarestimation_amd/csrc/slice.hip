@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <map>
 #include <mutex>
+#include <vector>
 
 #include "common.h"
 
@@ -464,13 +465,15 @@ constexpr int kTileRowCap = kSliceBlock;  // rows per chunk: one per thread
 template <int RT>
 __global__ void __launch_bounds__(kSliceBlock)
     k_slice_adjoint_tile(const float2* __restrict__ rows, const int* __restrict__ off, float2* __restrict__ vol,
-                         int64_t D, int64_t H, int64_t W, int ntrans, int rw) {
+                         int64_t D, int64_t H, int64_t W, int ntrans, int rw, const int* __restrict__ border) {
   static_assert(kTileX * kTileY * kTileZ == kSliceBlock, "one cell per thread");
   __shared__ float2 s_rows[kTileRowCap * (2 + RT)];
   __shared__ int s_gstart[kTileRuns], s_len[kTileRuns], s_lbase[kTileRuns + 1];
   __shared__ int s_src[kTileRowCap];
   const int64_t nbx = W / kTileX, nby = H / kTileY;
-  const int64_t blk = blockIdx.x;
+  // blocks run nearest the volume centre first: the central blocks hold the most rows (slices
+  // all pass through the centre), and the light outer ones then fill the end of the launch
+  const int64_t blk = border[blockIdx.x];
   const int64_t x0 = (blk % nbx) * kTileX, y0 = ((blk / nbx) % nby) * kTileY, z0 = (blk / (nbx * nby)) * kTileZ;
   const int tx = threadIdx.x % kTileX, ty = (threadIdx.x / kTileX) % kTileY, tz = threadIdx.x / (kTileX * kTileY);
   auto key_of = [&](int64_t fx, int64_t fy, int64_t fz) { return ((fz + 1) * (H + 1) + (fy + 1)) * (W + 1) + (fx + 1); };
@@ -572,6 +575,9 @@ struct AdjWork {
   int* off = nullptr;  // slice plans: first sorted position of every key (nkeys + 1 entries)
   int* cnt = nullptr;  // ... and the points per key it is scanned from
   size_t cap_off = 0;
+  int* border = nullptr;  // tile blocks by distance from the volume centre, for border_shape
+  int64_t border_shape[3] = {0, 0, 0};
+  std::vector<int> border_host;  // the uploaded order (kept alive for the asynchronous copy)
   void* tmp = nullptr;
   void* rows = nullptr;
 };
@@ -913,13 +919,35 @@ static int slice_plan_adjoint(SlicePlan* pl, const void* sig_a, int64_t n_a, con
   dim3 g((unsigned)ceil_div(D * H * W, kSliceBlock), (unsigned)ceil_div(ntrans, RT));
   if (sizeof(T) == 4 && pl->mode == 0 && w.off && ntrans <= RT && W % kTileX == 0 && H % kTileY == 0 &&
       D % kTileZ == 0) {
-    const unsigned nblk = (unsigned)((W / kTileX) * (H / kTileY) * (D / kTileZ));
+    const int64_t nbx = W / kTileX, nby = H / kTileY, nbz = D / kTileZ;
+    const unsigned nblk = (unsigned)(nbx * nby * nbz);
+    if (w.border_shape[0] != D || w.border_shape[1] != H || w.border_shape[2] != W) {
+      // block order by squared distance of the block centre from the volume centre (ties by index);
+      // a previous order's upload must be finished before its host copy is rewritten
+      if (w.border) COV3D_HIP(hipStreamSynchronize(st));
+      std::vector<int64_t> key(nblk);
+      w.border_host.resize(nblk);
+      for (int64_t b = 0; b < (int64_t)nblk; ++b) {
+        const int64_t bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
+        const int64_t dx = 2 * (bx * kTileX) + kTileX - W, dy = 2 * (by * kTileY) + kTileY - H,
+                      dz = 2 * (bz * kTileZ) + kTileZ - D;
+        key[b] = dx * dx + dy * dy + dz * dz;
+        w.border_host[b] = (int)b;
+      }
+      std::stable_sort(w.border_host.begin(), w.border_host.end(), [&](int a, int b) { return key[a] < key[b]; });
+      hipFree(w.border);
+      w.border = nullptr;
+      w.border_shape[0] = w.border_shape[1] = w.border_shape[2] = 0;
+      COV3D_HIP(hipMalloc(&w.border, (size_t)nblk * sizeof(int)));
+      COV3D_HIP(hipMemcpyAsync(w.border, w.border_host.data(), (size_t)nblk * sizeof(int), hipMemcpyHostToDevice, st));
+      w.border_shape[0] = D, w.border_shape[1] = H, w.border_shape[2] = W;
+    }
     if (ntrans == 1)
       hipLaunchKernelGGL((k_slice_adjoint_tile<1>), dim3(nblk), dim3(kSliceBlock), 0, st, (const float2*)w.rows,
-                         w.off, (float2*)vol, D, H, W, 1, rw);
+                         w.off, (float2*)vol, D, H, W, 1, rw, (const int*)w.border);
     else
       hipLaunchKernelGGL((k_slice_adjoint_tile<RT>), dim3(nblk), dim3(kSliceBlock), 0, st, (const float2*)w.rows,
-                         w.off, (float2*)vol, D, H, W, (int)ntrans, rw);
+                         w.off, (float2*)vol, D, H, W, (int)ntrans, rw, (const int*)w.border);
   } else if (pl->mode == 0)
     hipLaunchKernelGGL((k_slice_adjoint_gather<T, 0, RT>), g, dim3(kSliceBlock), 0, st, (const Cplx<T>*)w.rows,
                        w.start, w.end, (Cplx<T>*)vol, D, H, W, (int)ntrans, rw);
@@ -1125,7 +1153,7 @@ extern "C" int cov3d_slice_plan_destroy(void* plan) {
   auto* pl = static_cast<cov3d::SlicePlan*>(plan);
   cov3d::AdjWork& w = pl->w;
   hipFree(w.keys), hipFree(w.skeys), hipFree(w.sidx), hipFree(w.spos), hipFree(w.start), hipFree(w.end);
-  hipFree(w.tmp), hipFree(w.rows), hipFree(w.off), hipFree(w.cnt), hipFree(pl->sp);
+  hipFree(w.tmp), hipFree(w.rows), hipFree(w.off), hipFree(w.cnt), hipFree(w.border), hipFree(pl->sp);
   delete pl;
   return 0;
 }
