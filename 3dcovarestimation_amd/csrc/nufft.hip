@@ -238,6 +238,9 @@ struct BinGeom {
   int nf[3];
   int nb[3];   // bins per axis (nf / 4)
   int nsb[3];  // super-bins per axis (nf / 8)
+  // spreading: for XCD x, xcd_tiles[x * n + j] is the tile ordinal (gather_bin's numbering) it
+  // takes j-th, nearest the grid centre first; NULL keeps the ordinal order
+  const int* xcd_tiles;
 };
 
 // Super-bin numbering.  When every axis holds a multiple of 4 super-bins they are numbered in
@@ -666,7 +669,11 @@ __device__ __forceinline__ void gather_bin(int bid, int nblk, const BinGeom& g, 
     return;
   }
   const int x = bid & 7, k = bid >> 3;
-  const int j = k / (TB * TB * TB);  // tile ordinal on this XCD
+  // tile ordinal on this XCD; with an order table the tiles nearest the grid centre (the ones
+  // holding the most points: every slice passes through the centre) go first, so the light outer
+  // tiles fill the end of the launch instead of the heavy ones
+  const int jpos = k / (TB * TB * TB);
+  const int j = g.xcd_tiles ? g.xcd_tiles[x * (ntile / 8) + jpos] : jpos;
   const int i = k % (TB * TB * TB);
   const int m = j / (T1 * T2), rr = j % (T1 * T2);
   const int t1 = rr / T2, t2 = rr % T2;
@@ -2330,6 +2337,7 @@ static int destroy_plan(Plan* p) {
   if (!p) return 0;
   free_points(p);
   hipFree(p->fine);
+  hipFree((void*)p->geom.xcd_tiles);
   hipFree(p->pipe_ctr);
   for (int k = 0; k < 3; ++k) hipFree(p->invp[k]);
   hipFree(p->bin_count);
@@ -2389,11 +2397,44 @@ static int make_plan(int type, const int64_t* n, int iflag, int64_t ntrans, doub
   }
   p->nsuper = (int64_t)p->geom.nsb[0] * p->geom.nsb[1] * p->geom.nsb[2];
   p->nbins = 8 * p->nsuper;
+  p->geom.xcd_tiles = nullptr;
   const int64_t nfine = p->nf[0] * p->nf[1] * p->nf[2];
   auto fail = [&](int rc) {
     destroy_plan(p);
     return rc;
   };
+  {
+    // spreading's per-XCD tile order (gather_bin): tiles by wrapped distance from frequency 0, the
+    // fine grid's corner, where the slices' points are densest; nearest first
+    constexpr int TB = 8;
+    const int nb0 = p->geom.nb[0], nb1 = p->geom.nb[1], nb2 = p->geom.nb[2];
+    if (nb0 % TB == 0 && nb1 % TB == 0 && nb2 % TB == 0 && (nb0 / TB) % 8 == 0) {
+      const int T0 = nb0 / TB, T1 = nb1 / TB, T2 = nb2 / TB;
+      const int per = T0 * T1 * T2 / 8;
+      std::vector<int> tab((size_t)8 * per);
+      auto wrapped = [](int t, int T) {  // twice the distance of tile t's centre from 0 (mod T)
+        const int c = 2 * t + 1;
+        return std::min(c, 2 * T - c);
+      };
+      for (int x = 0; x < 8; ++x) {
+        std::vector<std::pair<int64_t, int>> key((size_t)per);
+        for (int j = 0; j < per; ++j) {
+          const int m = j / (T1 * T2), rr = j % (T1 * T2), t1 = rr / T2, t2 = rr % T2;
+          const int t0 = ((x - 2 * t1 - 4 * t2) & 7) + 8 * m;
+          const int64_t d0 = wrapped(t0, T0), d1 = wrapped(t1, T1), d2 = wrapped(t2, T2);
+          key[j] = {d0 * d0 + d1 * d1 + d2 * d2, j};
+        }
+        std::stable_sort(key.begin(), key.end(),
+                         [](const std::pair<int64_t, int>& a, const std::pair<int64_t, int>& b) { return a.first < b.first; });
+        for (int j = 0; j < per; ++j) tab[(size_t)x * per + j] = key[j].second;
+      }
+      int* d = nullptr;
+      if (hipMalloc(&d, tab.size() * sizeof(int)) != hipSuccess) return fail(COV3D_ERR_HIP);
+      p->geom.xcd_tiles = d;
+      if (hipMemcpy(d, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(COV3D_ERR_HIP);
+    }
+  }
   if (hipMalloc(&p->fine, nfine * ntrans * csize) != hipSuccess) {
     set_error("hipMalloc of %lld-byte fine grid failed", (long long)(nfine * ntrans * csize));
     return fail(COV3D_ERR_HIP);
