@@ -43,7 +43,7 @@ _SIGNATURES = {
     "cov3d_slice_plan_forward_split": [c_vp, c_vp, c_vp, c_i64, c_vp, c_i64, c_vp],
     "cov3d_slice_plan_adjoint_split": [c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_i64, c_vp],
     "cov3d_slice_grid_from_pts": [c_vp, c_vp, c_i64, c_i64, ctypes.c_double, ctypes.c_double, c_int, c_vp],
-    "cov3d_slice_boundary_pixels": [c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_i64, c_vp],
+    "cov3d_slice_boundary_pixels": [c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp, c_i64, c_vp],
     "cov3d_herm_solve_c128": [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_vp],
     "cov3d_slice_plan_destroy": [c_vp],
     "cov3d_ml_grad_images_complex": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_i64, c_i64,

@@ -966,13 +966,18 @@ __global__ void k_slice_grid_from_pts(const T* __restrict__ pts, T* __restrict__
                                       T scale, T shift) {
   // a product and a sum, each rounded, as the two tensor operations round them (no fused multiply-add)
 #pragma clang fp contract(off)
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // output element
-  if (e >= n * npix * 3) return;
-  const int64_t j = e / 3;
-  const int c = (int)(e - j * 3);
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // point: three coalesced loads
+  if (j >= n * npix) return;
   const int64_t b = j / npix, k = j - b * npix;
-  const T a = pts[(b * 3 + (2 - c)) * npix + k] * scale;
-  grid[e] = shift != (T)0 ? a + shift : a;
+  const T* pb = pts + b * 3 * npix + k;
+  T out[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const T a = pb[(2 - c) * npix] * scale;
+    out[c] = shift != (T)0 ? a + shift : a;
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) grid[3 * j + c] = out[c];
 }
 
 // Boundary pixels of the Fourier-domain ML on a slice plan: pixel k of image b is flagged when its
@@ -992,26 +997,39 @@ __device__ __forceinline__ bool near_nyquist(const float* g, float half_cells, f
   return near;
 }
 
+// pass 1, one thread per pixel: flag[b, k] = kept and (k or its mirror reaches a Nyquist cell)
+__global__ void k_slice_boundary_flags(const float* __restrict__ grid, const unsigned char* __restrict__ keep,
+                                       int64_t B, int L, float vL, unsigned char* __restrict__ flag) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int np = L * L;
+  if (e >= B * np) return;
+  const int64_t b = e / np;
+  const int k = (int)(e - b * np);
+  unsigned char f = 0;
+  if (keep[k]) {
+    const int i = k / L, j = k - i * L;
+    const int mi = (L % 2 == 0) ? (L - i) % L : L - 1 - i, mj = (L % 2 == 0) ? (L - j) % L : L - 1 - j;
+    const float* gb = grid + b * (int64_t)np * 3;
+    const float half_cells = 0.5f * (vL - 1.f);
+    f = (near_nyquist(gb + (int64_t)k * 3, half_cells, vL) || near_nyquist(gb + (int64_t)(mi * L + mj) * 3, half_cells, vL))
+            ? 1 : 0;
+  }
+  flag[e] = f;
+}
+
+// pass 2, one workgroup per image: the flagged pixels in ascending order (a block scan of the
+// threads' counts over contiguous pixel runs read as bytes)
 __global__ void __launch_bounds__(kBndThreads)
-    k_slice_boundary(const float* __restrict__ grid, const unsigned char* __restrict__ keep, int L, float vL,
-                     int* __restrict__ order, int* __restrict__ count, int cap) {
+    k_slice_boundary(const unsigned char* __restrict__ flag, int L, int* __restrict__ order, int* __restrict__ count,
+                     int cap) {
   __shared__ int s_scan[kBndThreads];
   const int64_t b = blockIdx.x;
   const int np = L * L;
-  const float* gb = grid + b * (int64_t)np * 3;
-  const float half_cells = 0.5f * (vL - 1.f);
-  // each thread owns a contiguous run of pixels, so a block scan of the counts orders them
+  const unsigned char* fb = flag + b * (int64_t)np;
   const int per = (np + kBndThreads - 1) / kBndThreads;
   const int k0 = threadIdx.x * per, k1 = min(np, k0 + per);
-  auto flagged = [&](int k) {
-    if (!keep[k]) return false;
-    const int i = k / L, j = k - i * L;
-    const int mi = (L % 2 == 0) ? (L - i) % L : L - 1 - i, mj = (L % 2 == 0) ? (L - j) % L : L - 1 - j;
-    return near_nyquist(gb + (int64_t)k * 3, half_cells, vL) ||
-           near_nyquist(gb + (int64_t)(mi * L + mj) * 3, half_cells, vL);
-  };
   int n = 0;
-  for (int k = k0; k < k1; ++k) n += flagged(k) ? 1 : 0;
+  for (int k = k0; k < k1; ++k) n += fb[k];
   s_scan[threadIdx.x] = n;
   __syncthreads();
   // inclusive scan (Hillis-Steele) over the threads' counts
@@ -1024,7 +1042,7 @@ __global__ void __launch_bounds__(kBndThreads)
   int pos = s_scan[threadIdx.x] - n;
   if (n > 0) {
     for (int k = k0; k < k1; ++k)
-      if (flagged(k)) {
+      if (fb[k]) {
         if (pos < cap) order[b * cap + pos] = k;
         ++pos;
       }
@@ -1165,8 +1183,7 @@ extern "C" int cov3d_slice_grid_from_pts(const void* pts, void* grid, int64_t n,
   if (n == 0 || npix == 0) return 0;
   COV3D_CHECK_ARG(pts && grid, "NULL pointer");
   hipStream_t st = cov3d::as_stream(stream);
-  const int64_t ne = n * npix * 3;
-  const unsigned nb = (unsigned)cov3d::ceil_div(ne, cov3d::kSliceBlock);
+  const unsigned nb = (unsigned)cov3d::ceil_div(n * npix, cov3d::kSliceBlock);
   if (dtype == 1)
     hipLaunchKernelGGL((cov3d::k_slice_grid_from_pts<double>), dim3(nb), dim3(cov3d::kSliceBlock), 0, st,
                        (const double*)pts, (double*)grid, n, npix, scale, shift);
@@ -1178,14 +1195,19 @@ extern "C" int cov3d_slice_grid_from_pts(const void* pts, void* grid, int64_t n,
 }
 
 extern "C" int cov3d_slice_boundary_pixels(const float* grid, const unsigned char* keep, int64_t B, int64_t L,
-                                           int64_t vL, int* order, int* count, int64_t cap, void* stream) {
+                                           int64_t vL, unsigned char* flags, int* order, int* count, int64_t cap,
+                                           void* stream) {
   COV3D_CHECK_ARG(B >= 0 && L >= 1 && vL >= 2 && cap >= 0, "bad sizes");
   COV3D_CHECK_ARG(B < 2147483647 && L * L < 2147483647 && cap < 2147483647, "sizes too large");
   if (B == 0) return 0;
-  COV3D_CHECK_ARG(grid && keep && count && (order || cap == 0), "NULL pointer");
+  COV3D_CHECK_ARG(grid && keep && flags && count && (order || cap == 0), "NULL pointer");
   hipStream_t st = cov3d::as_stream(stream);
-  hipLaunchKernelGGL(cov3d::k_slice_boundary, dim3((unsigned)B), dim3(cov3d::kBndThreads), 0, st, grid, keep, (int)L,
-                     (float)vL, order, count, (int)cap);
+  const int64_t npx = B * L * L;
+  hipLaunchKernelGGL(cov3d::k_slice_boundary_flags, dim3((unsigned)cov3d::ceil_div(npx, cov3d::kSliceBlock)),
+                     dim3(cov3d::kSliceBlock), 0, st, grid, keep, B, (int)L, (float)vL, flags);
+  COV3D_LAUNCH_CHECK();
+  hipLaunchKernelGGL(cov3d::k_slice_boundary, dim3((unsigned)B), dim3(cov3d::kBndThreads), 0, st,
+                     (const unsigned char*)flags, (int)L, order, count, (int)cap);
   COV3D_LAUNCH_CHECK();
   return 0;
 }

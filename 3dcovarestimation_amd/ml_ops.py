@@ -249,9 +249,10 @@ def _boundary_order(plan, B: int, L: int, keep: torch.Tensor) -> Tuple[torch.Ten
     dev = grid.device
     order = torch.empty((B, L * L), dtype=torch.int32, device=dev)
     count = torch.empty(B, dtype=torch.int32, device=dev)
+    flags = torch.empty(B * L * L, dtype=torch.uint8, device=dev)
     keep = _native.dense(keep.to(device=dev, dtype=torch.uint8))
     _native.check(_native.load().cov3d_slice_boundary_pixels(
-        grid.data_ptr(), keep.data_ptr(), B, L, vL, order.data_ptr(), count.data_ptr(), L * L,
+        grid.data_ptr(), keep.data_ptr(), B, L, vL, flags.data_ptr(), order.data_ptr(), count.data_ptr(), L * L,
         _native.stream_ptr(dev)))
     return order, count
 

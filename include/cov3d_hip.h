@@ -164,9 +164,11 @@ int cov3d_slice_grid_from_pts(const void* pts, void* grid, int64_t n, int64_t np
  * makes it) is flagged when a cell coordinate (g + 1) (vL - 1) / 2 lies within one of 0 or vL on
  * some axis, or when that of its mirror pixel does ((L - i) mod L, (L - j) mod L for even L;
  * (L - 1 - i, L - 1 - j) for odd L); flagged pixels with keep[k] != 0 are written in ascending
- * order to order[b * cap + 0 ..), at most cap of them, and count[b] is their number. */
+ * order to order[b * cap + 0 ..), at most cap of them, and count[b] is their number.  flags is a
+ * caller workspace of B * L * L bytes (the per-pixel flags between the two passes). */
 int cov3d_slice_boundary_pixels(const float* grid, const unsigned char* keep, int64_t B, int64_t L,
-                                int64_t vL, int* order, int* count, int64_t cap, void* stream);
+                                int64_t vL, unsigned char* flags, int* order, int* count, int64_t cap,
+                                void* stream);
 
 /* Gradient of cov3d_slice_forward with respect to its grid: for each point j,
  *   grid_grad[j, a] = sum_t Re(conj(grad_out[t, j]) * d out[t, j] / d grid[j, a])
