@@ -22,6 +22,16 @@ struct Cplx {
   T x, y;
 };
 
+// Workgroup index dealt XCD-contiguously: the hardware hands consecutive workgroups to the 8 XCDs
+// in turn, so workgroup b of n runs on XCD x = b mod 8 as that XCD's k-th (k = b / 8); XCD x holds
+// q + (x < r) of them (q = n / 8, r = n mod 8), and renumbered as start(x) + k, start(x) =
+// x q + min(x, r), each XCD takes one contiguous range of logical indices (a bijection of
+// [0, n)), so neighbouring logical workgroups, which share cache lines, share that XCD's L2.
+__device__ __forceinline__ int64_t xcd_contiguous(int64_t b, int64_t n) {
+  const int64_t q = n >> 3, r = n & 7, x = b & 7;
+  return x * q + (x < r ? x : r) + (b >> 3);
+}
+
 // One 8-byte (FP64: 16-byte) store of a complex value.  Cplx<T> is only T-aligned, and a
 // member-wise store of one may be emitted as two separate stores (two partial writes of one word).
 template <typename T>

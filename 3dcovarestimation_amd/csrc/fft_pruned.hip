@@ -113,8 +113,11 @@ __global__ void __launch_bounds__(kFftThreads)
   __shared__ float2 s_buf[kFftLines * LS];
   __shared__ float2 s_tw[NF];
   load_twiddles<NF>(s_tw, tw);
-  const int ct = blockIdx.x % NCT;
-  const int64_t pos = blockIdx.x / NCT;  // (outer, mid) position
+  // the NCT column tiles of a row position (its 128-byte pieces, which straddle cache lines since
+  // rows are H complex long) and neighbouring positions run on one XCD and meet in its L2
+  const int64_t lb = xcd_contiguous(blockIdx.x, gridDim.x);
+  const int ct = (int)(lb % NCT);
+  const int64_t pos = lb / NCT;  // (outer, mid) position
   const int c0 = ct * kFftLines;
   const int ncol = min(kFftLines, H - c0);
   const int col = threadIdx.x % kFftLines, rsub = threadIdx.x / kFftLines;  // 16 columns x 16 rows

@@ -745,6 +745,7 @@ __global__ void __launch_bounds__(kSliceBlock)
     k_slice_forward_sorted(const Cplx<T>* __restrict__ vol, const SortedPoint<T>* __restrict__ sp,
                            Cplx<T>* __restrict__ out_a, int64_t n_a, Cplx<T>* __restrict__ out_b, int64_t D,
                            int64_t H, int64_t W, int64_t npts, int ntrans) {
+  // (dealing consecutive sorted points to one XCD, xcd_contiguous, measured slower: 3.19 -> 3.41 ms)
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= npts) return;
   const int64_t t0 = (int64_t)blockIdx.y * TB;
