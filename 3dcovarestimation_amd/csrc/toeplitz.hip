@@ -100,8 +100,10 @@ __global__ void __launch_bounds__(kFftThreads)
   __shared__ float2 s_buf[kFftLines * LS];
   __shared__ float2 s_tw[NF];
   make_twiddles<NF>(s_tw);
-  const int c0 = (int)(blockIdx.x % NCT) * kFftLines;
-  const int64_t pos = blockIdx.x / NCT;
+  // a position's column tiles and neighbouring positions on one XCD (shared L2 lines)
+  const int64_t lb = xcd_contiguous(blockIdx.x, gridDim.x);
+  const int c0 = (int)(lb % NCT) * kFftLines;
+  const int64_t pos = lb / NCT;
   const int col = threadIdx.x % kFftLines, rsub = threadIdx.x / kFftLines;
   // row r of this workgroup's lines sits at B[base + r * stride + c0 + col]
   const int64_t stride = MODE == 1 ? (int64_t)NF * NF : NF;
