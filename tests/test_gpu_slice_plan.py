@@ -92,8 +92,9 @@ def test_slice_plan_tiled_adjoint_matches_gather(dev, shape):
     torch.testing.assert_close(back, ref, rtol=1e-5, atol=1e-5 * scale)
 
 
+@pytest.mark.parametrize("mode", ["bilinear", "nearest"])
 @pytest.mark.parametrize("dtype", [torch.complex64, torch.complex128])
-def test_slice_plan_split_matches_whole(dev, dtype):
+def test_slice_plan_split_matches_whole(dev, dtype, mode):
     """cov3d_slice_plan_forward_split / _adjoint_split (two point sets set as one) against the plain
     forward / adjoint of the same plan: the two outputs are the whole output cut at n_a, and the
     adjoint of the two signal parts, scaled by a device scalar, is the adjoint of the joined signal
@@ -104,7 +105,7 @@ def test_slice_plan_split_matches_whole(dev, dtype):
     L, us, n = 16, 2, 4
     vL = L * us
     npts, n_a = 6000, 4700
-    plan = npl.NufftPlanDiscretized((L,) * 3, upsample_factor=us, mode="bilinear")
+    plan = npl.NufftPlanDiscretized((L,) * 3, upsample_factor=us, mode=mode)
     plan.set_grid_flat(((torch.rand((npts, 3), dtype=rdt, device=dev, generator=g) * 2 - 1) * 1.05).contiguous())
     vol = torch.randn((n, vL, vL, vL), dtype=dtype, device=dev, generator=g)
     whole = plan.execute_forward_flat(vol)
