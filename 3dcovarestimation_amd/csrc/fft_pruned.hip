@@ -162,13 +162,17 @@ __global__ void __launch_bounds__(kFftThreads)
     __syncthreads();
   }
   if (col < ncol) {
-    // rows rsub, rsub + 16, ...: the loads of kColGroup rows are issued before any is stored.
-    // Measured: the passes whose rows are H apart (B, Y) gain (0.40 -> 0.37, 0.39 -> 0.26 ms at
-    // NF = 256), the passes whose rows are NF H apart (C, X) lose (0.93 -> 1.06, 0.98 -> 1.05 ms)
-    // and keep one row per iteration.
+    // rows rsub, rsub + 16, ...: GB rows' loads are issued before any is stored.  Measured with
+    // the XCD-contiguous dealing: the passes whose rows are H apart (B, Y) take kColGroup; pass C
+    // (and its centred form, mode 4), rows NF H apart, takes 4 at NF = 256 (0.75 -> 0.64 ms) and 8
+    // at NF = 512 (8.78 -> 6.06 ms, where 65 KB of LDS leave two workgroups per CU); pass X keeps
+    // one row at NF = 256 (4: 0.877 -> 0.889 ms) and takes 8 at NF = 512 (7.45 -> 7.26 ms).
     constexpr int RS = kFftThreads / kFftLines;
     constexpr int NIT = NF / RS;
-    constexpr int GB = (MODE == 0 || MODE == 3) ? kColGroup : 1;
+    constexpr int GB = (MODE == 0 || MODE == 3) ? kColGroup
+                       : (MODE == 1 || MODE == 4) ? (NF >= 512 ? 8 : 4)
+                       : (MODE == 2 && NF >= 512) ? 8
+                                                   : 1;
     for (int g0 = 0; g0 < NIT; g0 += GB) {
       float2 va[GB], vb[GB];
 #pragma unroll
