@@ -114,9 +114,20 @@ __global__ void __launch_bounds__(kFftThreads)
     __syncthreads();
   }
   const int nin = pruned_in ? L : NF;
-  for (int r = rsub; r < nin; r += RS) {
-    const int m = pruned_in ? wrap_mode(r, L, NF) : r;
-    s_buf[col * LS + m] = B[base + r * stride + c0 + col];
+  // kColGroup rows' loads in flight before their LDS stores (a store right after each load would
+  // wait for it)
+  for (int r0 = rsub; r0 < nin; r0 += RS * kColGroup) {
+    float2 v[kColGroup];
+#pragma unroll
+    for (int u = 0; u < kColGroup; ++u) {
+      const int r = r0 + u * RS;
+      if (r < nin) v[u] = B[base + r * stride + c0 + col];
+    }
+#pragma unroll
+    for (int u = 0; u < kColGroup; ++u) {
+      const int r = r0 + u * RS;
+      if (r < nin) s_buf[col * LS + (pruned_in ? wrap_mode(r, L, NF) : r)] = v[u];
+    }
   }
   __syncthreads();
   lds_fft<NF, LS>(s_buf, s_tw, MODE == 2 ? 1.f : -1.f);
@@ -124,10 +135,21 @@ __global__ void __launch_bounds__(kFftThreads)
     // frequency (k0, k1 = pos, k2 = c0 + col) takes lhs at the centred index (k + NF/2) mod NF
     const int64_t c1 = (pos + NF / 2) % NF;
     const int c2 = (c0 + col + NF / 2) % NF;
-    for (int k0 = rsub; k0 < NF; k0 += RS) {
-      const float2 a = s_buf[col * LS + k0];
-      const float2 b = lhs[((int64_t)((k0 + NF / 2) % NF) * NF + c1) * NF + c2];
-      s_buf[col * LS + k0] = make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    for (int k00 = rsub; k00 < NF; k00 += RS * kColGroup) {
+      float2 bv[kColGroup];
+#pragma unroll
+      for (int u = 0; u < kColGroup; ++u) {
+        const int k0 = k00 + u * RS;
+        if (k0 < NF) bv[u] = lhs[((int64_t)((k0 + NF / 2) % NF) * NF + c1) * NF + c2];
+      }
+#pragma unroll
+      for (int u = 0; u < kColGroup; ++u) {
+        const int k0 = k00 + u * RS;
+        if (k0 < NF) {
+          const float2 a = s_buf[col * LS + k0], b = bv[u];
+          s_buf[col * LS + k0] = make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+        }
+      }
     }
     __syncthreads();
     lds_fft<NF, LS>(s_buf, s_tw, 1.f);
