@@ -51,17 +51,40 @@ __global__ void __launch_bounds__(kFftThreads)
   if constexpr (FWD) {
     for (int e = threadIdx.x; e < kFftLines * NF; e += kFftThreads) s_buf[(e / NF) * LS + e % NF] = make_float2(0.f, 0.f);
     __syncthreads();
-    for (int e = threadIdx.x; e < kFftLines * L; e += kFftThreads) {
-      const int l = e / L, i2 = e % L;
-      if (g0 + l < nlines) s_buf[l * LS + wrap_mode(i2, L, NF)] = in[(g0 + l) * L + i2];
+    // the source of element e is in[g0 L + e]: kColGroup loads in flight per thread, the index
+    // clamped to the last element so that they need no branch, then the stores of the valid ones
+    const int64_t last = nlines * L - 1;
+    for (int e0 = threadIdx.x; e0 < kFftLines * L; e0 += kFftThreads * kColGroup) {
+      float2 v[kColGroup];
+#pragma unroll
+      for (int u = 0; u < kColGroup; ++u) {
+        const int64_t src = g0 * L + e0 + u * kFftThreads;
+        v[u] = in[src < last ? src : last];
+      }
+#pragma unroll
+      for (int u = 0; u < kColGroup; ++u) {
+        const int e = e0 + u * kFftThreads, l = e / L, i2 = e % L;
+        if (e < kFftLines * L && g0 + l < nlines) s_buf[l * LS + wrap_mode(i2, L, NF)] = v[u];
+      }
     }
   } else {
-    for (int e = threadIdx.x; e < kFftLines * NF; e += kFftThreads) {
-      const int l = e / NF, k = e % NF;
-      const int64_t g = g0 + l;
-      if (g < nlines) {
-        const int64_t i0 = g / L, i1 = g % L;
-        s_buf[l * LS + k] = in[(i0 * NF + i1) * NF + k];
+    // GB loads in flight per thread (kFftLines * NF / kFftThreads elements each, a power of two);
+    // lines past the end read the last line
+    constexpr int PER = kFftLines * NF / kFftThreads;
+    constexpr int GB = PER < kColGroup ? PER : kColGroup;
+    static_assert(PER % GB == 0 && PER * kFftThreads == kFftLines * NF, "whole batches of loads");
+    for (int e0 = threadIdx.x; e0 < kFftLines * NF; e0 += kFftThreads * GB) {
+      float2 v[GB];
+#pragma unroll
+      for (int u = 0; u < GB; ++u) {
+        const int e = e0 + u * kFftThreads, l = e / NF, k = e % NF;
+        const int64_t g = g0 + l < nlines ? g0 + l : nlines - 1;
+        v[u] = in[((g / L) * NF + g % L) * NF + k];
+      }
+#pragma unroll
+      for (int u = 0; u < GB; ++u) {
+        const int e = e0 + u * kFftThreads;
+        if (g0 + e / NF < nlines) s_buf[(e / NF) * LS + e % NF] = v[u];
       }
     }
   }
