@@ -184,10 +184,25 @@ def preconditioner_from_log_diag(logd: torch.Tensor, L: int) -> Optional[Callabl
 
     With S the half-length roll, fft3c = S F S, and S F^-1 S (w o S F S r) = F^-1 ((S w) o F r)
     (the shift theorem's phases (-1)^k cancel in pairs), so the operator runs as two plain FFTs
-    and one multiply by the pre-rolled real weights."""
+    and one multiply by the pre-rolled real weights.  On the GPU at L = 64, 128, 256 or 512 it
+    runs as the Toeplitz operator with nf = L instead."""
     w = torch.exp(-logd).reshape(L, L, L)
     if not bool(torch.isfinite(w).all()) or bool((w <= 0).any()):
         return None
+    if w.is_cuda and L in (64, 128, 256, 512):
+        # the Toeplitz operator crop_L(ifft3c(lhs * fft3c(pad_nf(x)))) at nf = L (pad and crop are
+        # the identity) is this operator with lhs = w: its LDS passes fuse the multiply into the x
+        # transforms, where the plain form below runs three library kernels
+        wc = w.to(torch.complex64).contiguous()
+        lib, st = _native.load(), _native.stream_ptr(w.device)
+
+        def apply_native(r: torch.Tensor) -> torch.Tensor:
+            x = _native.dense(r.reshape(-1).to(torch.complex64))
+            y = torch.empty_like(x)
+            _native.check(lib.cov3d_toeplitz_apply(x.data_ptr(), wc.data_ptr(), y.data_ptr(), L, L, st))
+            return y
+
+        return apply_native
     ws = torch.fft.ifftshift(w).to(torch.float32).contiguous()
 
     def apply(r: torch.Tensor) -> torch.Tensor:

@@ -126,3 +126,22 @@ def test_shell_network_matches_best_shell_constant_diagonal(dev):
     assert learned["converged"] and learned["fallback"] is None and best["converged"]
     assert learned["iterations"] <= 1.25 * best["iterations"]
     assert learned["iterations"] < 0.5 * res["plain"]["iterations"]
+
+
+@pytest.mark.parametrize("L", [64, 128])
+def test_native_preconditioner_matches_fft_form(dev, L):
+    """At L = 64 and 128 the GPU preconditioner runs as the Toeplitz operator with nf = L; it must
+    equal ifft3c(w * fft3c(r)) computed with plain FFTs (the form used off the GPU, FP32) on the
+    host, for weights spanning four decades and a complex input."""
+    pc = pkg("precond")
+    g = torch.Generator().manual_seed(L)
+    logd = (torch.rand(L**3, generator=g, dtype=torch.float64) * 4 - 2) * np.log(10.0)
+    r = torch.complex(torch.randn(L**3, generator=g, dtype=torch.float64),
+                      torch.randn(L**3, generator=g, dtype=torch.float64))
+    ref = pc.preconditioner_from_log_diag(logd, L)(r)  # CPU: two FFTs and the multiply
+    apply = pc.preconditioner_from_log_diag(logd.to(dev), L)
+    assert apply.__name__ == "apply_native"
+    out = apply(r.to(torch.complex64).to(dev)).cpu().to(torch.complex128)
+    err = float((out - ref).norm() / ref.norm())
+    print(f"L={L}: native preconditioner rel err {err:.2e}")
+    assert err < 1e-5
