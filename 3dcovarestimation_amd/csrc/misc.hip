@@ -99,10 +99,28 @@ __global__ void __launch_bounds__(kMiscBlock)
     k_vdot(const float2* __restrict__ x, const float2* __restrict__ y, int64_t n,
            double* __restrict__ out) {
   __shared__ double sr[kMiscBlock], si[kMiscBlock];
+  // kDotBatch elements per thread per step with all their loads issued first (one element per
+  // step leaves one load pair in flight per thread: 256 blocks then wait on latency), the rest
+  // one at a time
+  constexpr int kDotBatch = 8;
   double ar = 0, ai = 0;
-  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
-       k += (int64_t)gridDim.x * blockDim.x) {
-    float2 a = x[k], b = y[k];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; k + (kDotBatch - 1) * stride < n; k += kDotBatch * stride) {
+    float2 a[kDotBatch], b[kDotBatch];
+#pragma unroll
+    for (int u = 0; u < kDotBatch; ++u) {
+      a[u] = x[k + u * stride];
+      b[u] = y[k + u * stride];
+    }
+#pragma unroll
+    for (int u = 0; u < kDotBatch; ++u) {
+      ar += (double)a[u].x * b[u].x + (double)a[u].y * b[u].y;
+      ai += (double)a[u].x * b[u].y - (double)a[u].y * b[u].x;
+    }
+  }
+  for (; k < n; k += stride) {
+    const float2 a = x[k], b = y[k];
     ar += (double)a.x * b.x + (double)a.y * b.y;
     ai += (double)a.x * b.y - (double)a.y * b.x;
   }

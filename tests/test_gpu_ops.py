@@ -280,3 +280,16 @@ def test_toeplitz_operator_closed_forms(dev, L, nf):
     err = rel_l2(y, ref)
     print(f"Toeplitz shift L={L} nf={nf}: {err:.2e}")
     assert err < 1e-5
+
+
+@pytest.mark.parametrize("n", [8 * 65536 - 1, 8 * 65536 + 12345, 2**21 + 3])
+def test_vdot_fp64_batched_lengths(dev, n):
+    """Lengths at which every thread of the 256 x 256 dot grid runs whole batches of 8 elements
+    and a ragged tail, checked against numpy's complex128 vdot."""
+    mean = pkg("mean")
+    rng = np.random.default_rng(n)
+    x = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    y = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    v = mean.vdot_f64(torch.tensor(x, device=dev), torch.tensor(y, device=dev))
+    ref = np.vdot(x.astype(np.complex128), y.astype(np.complex128))
+    assert abs(complex(v) - ref) < 1e-12 * np.sqrt(n) * np.abs(x).max() * np.abs(y).max() + 1e-9
