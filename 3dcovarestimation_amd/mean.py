@@ -165,6 +165,8 @@ def correct_upsampling_padding(lhs: torch.Tensor, rhs: torch.Tensor, cg_init: to
     """Solves the padded normal equations by CG from the direct estimate (mean.py:96-132)."""
     b = centered_ifft3(rhs, cropping_size=(L,) * 3).reshape(-1).to(torch.complex64)
     x0 = torch.complex(cg_init.reshape(-1).float(), torch.zeros_like(cg_init.reshape(-1).float()))
+    # the operator reads lhs as dense complex64: convert it once, not once per iteration
+    lhs = _native.dense(lhs.to(torch.complex64))
     x, info = complex_conjugate_gradient(lambda v: toeplitz_apply(v, lhs, L), b, x0=x0, rtol=rtol,
                                          precond=precond, return_info=True, maxiter=maxiter)
     vol = x.reshape((L, L, L)).real
